@@ -1,0 +1,300 @@
+// extern "C" entry points of libnqs_hip.so (declared in include/nqs_hip.h).
+#include <algorithm>
+#include <cstring>
+#include "model.h"
+#include "physics.h"
+
+namespace nqs {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+int pack_params(const Model& md, const double* theta, const double* gs, const double* wenv, void* packed,
+                hipStream_t st);
+
+#define NQS_TRY(x)                 \
+  do {                             \
+    int _rc = (x);                 \
+    if (_rc != NQS_OK) return _rc; \
+  } while (0)
+
+// Rows of the jet activations processed per launch sequence: bounds the workspace while
+// keeping every launch far above the 256-CU occupancy threshold.
+static const int64_t kRowBudget = 1LL << 20;
+
+static int64_t chunk_walkers(const Model& md, int64_t W, int cfg_per_walker, int C) {
+  const int64_t per = (int64_t)cfg_per_walker * C * md.N;
+  int64_t c = std::max<int64_t>(1, kRowBudget / per);
+  return std::min<int64_t>(c, std::max<int64_t>(W, 1));
+}
+
+static int64_t al(int64_t b) { return round_up(b, 256); }
+
+struct ElocPlan {
+  int64_t c1, c2;  // walkers per chunk in the Laplacian pass / the spin-flip pass
+  int64_t fwd, extra;
+};
+static ElocPlan eloc_plan(const Model& md, int64_t W) {
+  ElocPlan p;
+  const int N = md.N;
+  p.c1 = chunk_walkers(md, W, 1, 2 * N + 2);
+  p.c2 = chunk_walkers(md, W, N, 3);
+  p.fwd = std::max(fwd_bytes(md, p.c1, 2 * N + 2, 2 * N), fwd_bytes(md, p.c2 * N, 3, 2));
+  p.extra = al(W * 16) + al(W * 2LL * N * 16) + al(W * 16) + al(W * N * 16) + al(W * N * 2 * 16);
+  return p;
+}
+
+template <typename T>
+static int fwd_logpsi(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t W, double* out,
+                      char* ws, hipStream_t st) {
+  const int64_t cw = chunk_walkers(md, W, 1, 1);
+  char* p = ws;
+  FwdBufs<T> b = carve_fwd<T>(md, p, cw, 1, 0);
+  for (int64_t w0 = 0; w0 < W; w0 += cw) {
+    JetGeom jg{std::min(cw, W - w0), md.N, 1, 0, 0, 0};
+    FwdBufs<T> bb = b;
+    bb.lp = out + 2 * w0;
+    NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
+  }
+  return NQS_OK;
+}
+
+template <typename T>
+static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void* packed, const double* R,
+                     const int32_t* S, int64_t W, double* eloc, double* lp_out, double* grad_out, double* lap_out,
+                     char* ws, hipStream_t st) {
+  const int N = md.N;
+  const ElocPlan pl = eloc_plan(md, W);
+  char* p = ws;
+  char* fwd_base = p;
+  p += al(pl.fwd);
+  double* lp = (double*)p;
+  p += al(W * 16);
+  double* grad = (double*)p;
+  p += al(W * 2LL * N * 16);
+  double* lap = (double*)p;
+  p += al(W * 16);
+  double* lpf = (double*)p;
+  p += al(W * N * 16);
+  double* tanf = (double*)p;
+  p += al(W * N * 2 * 16);
+  if (lp_out) lp = lp_out;
+  if (grad_out) grad = grad_out;
+  if (lap_out) lap = lap_out;
+  // pass 1: value, 2N position tangents, Laplacian
+  {
+    char* q = fwd_base;
+    FwdBufs<T> b = carve_fwd<T>(md, q, pl.c1, 2 * N + 2, 2 * N);
+    for (int64_t w0 = 0; w0 < W; w0 += pl.c1) {
+      const int64_t n = std::min(pl.c1, W - w0);
+      JetGeom jg{n, N, 2 * N + 2, 2 * N, 1, 1};
+      FwdBufs<T> bb = b;
+      bb.lp = lp + 2 * w0;
+      bb.tan = grad + 2 * w0 * 2 * N;
+      bb.lap = lap + 2 * w0;
+      NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
+    }
+  }
+  // pass 2: every single-spin flip, value + the two tangents of the flipped electron
+  {
+    char* q = fwd_base;
+    FwdBufs<T> b = carve_fwd<T>(md, q, pl.c2 * N, 3, 2);
+    for (int64_t w0 = 0; w0 < W; w0 += pl.c2) {
+      const int64_t n = std::min(pl.c2, W - w0);
+      JetGeom jg{n * N, N, 3, 2, 0, 2};
+      FwdBufs<T> bb = b;
+      bb.lp = lpf + 2 * w0 * N;
+      bb.tan = tanf + 2 * w0 * N * 2;
+      NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
+    }
+  }
+  EAssembleArgs a;
+  a.W = W;
+  a.N = N;
+  a.m_star = h->m_star;
+  a.lambda_r = h->lambda_r;
+  a.h_z = h->h_z;
+  a.n_field = h->n_field;
+  a.field = h->field_dev;
+  a.R = R;
+  a.S = S;
+  a.lp = lp;
+  a.grad = grad;
+  a.lap = lap;
+  a.lpf = lpf;
+  a.tanf = tanf;
+  a.out = eloc;
+  return launch_eloc_assemble(a, st);
+}
+
+struct McmcPlan {
+  int64_t cw, fwd;
+};
+static McmcPlan mcmc_plan(const Model& md, int64_t W) {
+  McmcPlan p;
+  p.cw = chunk_walkers(md, W, 1, 1);
+  p.fwd = fwd_bytes(md, p.cw, 1, 0);
+  return p;
+}
+
+template <typename T>
+static int mcmc_impl(const Model& md, const void* packed, double* R, int32_t* S, double* logp, int64_t W, int ns,
+                     int nf, double step, const double* xi, const double* u_pos, const int32_t* fi,
+                     const double* u_flip, int32_t* acc_pos, int32_t* acc_spin, char* ws, hipStream_t st) {
+  const int N = md.N;
+  const McmcPlan pl = mcmc_plan(md, W);
+  char* p = ws;
+  char* fwd_base = p;
+  p += al(pl.fwd);
+  double* Rp = (double*)p;
+  p += al(W * N * 2 * 8);
+  int32_t* Sp = (int32_t*)p;
+  p += al(W * N * 4);
+  double* lpn = (double*)p;
+  p += al(W * 16);
+  char* q = fwd_base;
+  FwdBufs<T> b = carve_fwd<T>(md, q, pl.cw, 1, 0);
+  auto forward = [&](const double* RR, const int32_t* SS) -> int {
+    for (int64_t w0 = 0; w0 < W; w0 += pl.cw) {
+      JetGeom jg{std::min(pl.cw, W - w0), N, 1, 0, 0, 0};
+      FwdBufs<T> bb = b;
+      bb.lp = lpn + 2 * w0;
+      NQS_TRY(run_forward<T>(md, packed, jg, RR, SS, w0, bb, st));
+    }
+    return NQS_OK;
+  };
+  for (int s = 0; s < ns; ++s) {
+    NQS_TRY(launch_mcmc_propose(W, N, R, xi + (int64_t)s * W * N * 2, step, md.A, md.Ainv, Rp, st));
+    NQS_TRY(forward(Rp, S));
+    NQS_TRY(launch_mcmc_accept_pos(W, N, R, Rp, logp, lpn, u_pos + (int64_t)s * W, acc_pos, st));
+    for (int f = 0; f < nf; ++f) {
+      const int32_t* fif = fi + ((int64_t)s * nf + f) * W;
+      NQS_TRY(launch_mcmc_flip_propose(W, N, S, fif, Sp, st));
+      NQS_TRY(forward(R, Sp));
+      NQS_TRY(launch_mcmc_accept_spin(W, N, S, fif, logp, lpn, u_flip + ((int64_t)s * nf + f) * W, acc_spin, st));
+    }
+  }
+  return NQS_OK;
+}
+
+}  // namespace nqs
+
+using namespace nqs;
+
+extern "C" {
+
+int nqs_version(void) { return 1; }
+const char* nqs_last_error(void) { return g_last_error.c_str(); }
+
+int64_t nqs_param_count(const nqs_model_desc* m) {
+  Model md;
+  if (build_model(m, md) != NQS_OK) return -1;
+  return md.P;
+}
+
+int nqs_param_leaf(const nqs_model_desc* m, int32_t idx, char* name, int32_t name_len, int64_t shape[3],
+                   int32_t* ndim, int64_t* offset, int32_t* is_complex) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(idx >= 0 && idx < (int32_t)md.leaves.size(), NQS_ERR_ARG, "leaf index out of range");
+  const Leaf& l = md.leaves[idx];
+  if (name && name_len > 0) {
+    std::strncpy(name, l.name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  for (int i = 0; i < 3; ++i) shape[i] = l.shape[i];
+  *ndim = l.ndim;
+  *offset = l.offset;
+  *is_complex = l.cplx ? 1 : 0;
+  return NQS_OK;
+}
+
+int64_t nqs_packed_bytes(const nqs_model_desc* m) {
+  Model md;
+  if (build_model(m, md) != NQS_OK) return -1;
+  return md.packed_bytes;
+}
+
+int nqs_pack_params(const nqs_model_desc* m, const double* theta, const double* gs, const double* wenv, void* packed,
+                    void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(theta && gs && wenv && packed, NQS_ERR_ARG, "null pointer");
+  return pack_params(md, theta, gs, wenv, packed, (hipStream_t)stream);
+}
+
+int64_t nqs_workspace_bytes(const nqs_model_desc* m, int32_t op, int64_t W) {
+  Model md;
+  if (build_model(m, md) != NQS_OK || W < 0) return -1;
+  switch (op) {
+    case NQS_OP_LOGPSI:
+      return fwd_bytes(md, chunk_walkers(md, W, 1, 1), 1, 0);
+    case NQS_OP_ELOC: {
+      ElocPlan p = eloc_plan(md, W);
+      return al(p.fwd) + p.extra;
+    }
+    case NQS_OP_MCMC: {
+      McmcPlan p = mcmc_plan(md, W);
+      return al(p.fwd) + al(W * md.N * 16) + al(W * md.N * 4) + al(W * 16);
+    }
+    default:
+      set_error("unknown op");
+      return -1;
+  }
+}
+
+static int check_ws(const nqs_model_desc* m, int op, int64_t W, int64_t ws_bytes) {
+  const int64_t need = nqs_workspace_bytes(m, op, W);
+  NQS_REQUIRE(need >= 0, NQS_ERR_ARG, "bad descriptor");
+  NQS_REQUIRE(ws_bytes >= need, NQS_ERR_WORKSPACE,
+              "workspace too small: need " + std::to_string(need) + " bytes, got " + std::to_string(ws_bytes));
+  return NQS_OK;
+}
+
+int nqs_psiformer_fwd(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,
+                      double* logpsi, void* ws, int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && logpsi && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_TRY(check_ws(m, NQS_OP_LOGPSI, W, ws_bytes));
+  if (md.tsize == 8) return fwd_logpsi<double>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream);
+  return fwd_logpsi<float>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream);
+}
+
+int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void* packed, const double* R,
+             const int32_t* S, int64_t W, double* eloc, double* lp, double* grad, double* lap, void* ws,
+             int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(h && packed && R && S && eloc && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_REQUIRE(h->n_field == 0 || h->field_dev, NQS_ERR_ARG, "ext field terms without data");
+  NQS_TRY(check_ws(m, NQS_OP_ELOC, W, ws_bytes));
+  if (md.tsize == 8)
+    return eloc_impl<double>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
+  return eloc_impl<float>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
+}
+
+int nqs_ewald(const nqs_ewald_desc* e, const double* R, int64_t W, int32_t N, double* out, void* stream) {
+  NQS_REQUIRE(e && R && out && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_REQUIRE(e->n_img >= 0 && e->n_g >= 0 && (e->n_img == 0 || e->img_dev) && (e->n_g == 0 || (e->g_dev && e->rg_dev)),
+              NQS_ERR_ARG, "bad Ewald tables");
+  return launch_ewald(*e, R, W, N, out, (hipStream_t)stream);
+}
+
+int nqs_mcmc_sweeps(const nqs_model_desc* m, const void* packed, double* R, int32_t* S, double* logp, int64_t W,
+                    int32_t n_sweeps, int32_t n_flip, double step, const double* xi, const double* u_pos,
+                    const int32_t* flip_i, const double* u_flip, int32_t* acc_pos, int32_t* acc_spin, void* ws,
+                    int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && logp && W >= 0 && n_sweeps >= 0 && n_flip >= 0, NQS_ERR_ARG, "bad arguments");
+  NQS_REQUIRE(n_sweeps == 0 || (xi && u_pos), NQS_ERR_ARG, "missing position streams");
+  NQS_REQUIRE(n_sweeps == 0 || n_flip == 0 || (flip_i && u_flip), NQS_ERR_ARG, "missing spin streams");
+  NQS_TRY(check_ws(m, NQS_OP_MCMC, W, ws_bytes));
+  if (md.tsize == 8)
+    return mcmc_impl<double>(md, packed, R, S, logp, W, n_sweeps, n_flip, step, xi, u_pos, flip_i, u_flip, acc_pos,
+                             acc_spin, (char*)ws, (hipStream_t)stream);
+  return mcmc_impl<float>(md, packed, R, S, logp, W, n_sweeps, n_flip, step, xi, u_pos, flip_i, u_flip, acc_pos,
+                          acc_spin, (char*)ws, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,436 @@
+// Psiformer backbone jet kernels (value + tangents + Laplacian) for gfx950.
+//
+// A "jet" carries, per configuration, the value of an activation, its derivatives along nt
+// tangent directions (positions of electrons) and, optionally, its Laplacian summed over all
+// 2N position coordinates.  Layout [cfg][channel][token][feature]; linear layers act on every
+// channel alike (bias on the value channel only, see gemm.hip), the kernels here carry the
+// non-linear ops with the first- and second-order chain rule:
+//   f(u):      df = f'(u) du,          Lap f = f'(u) Lap u + f''(u) sum_c du_c^2
+//   a*b:       d(ab) = a db + b da,    Lap(ab) = a Lap b + b Lap a + 2 sum_c da_c db_c
+// Reference ops (src/nqs/psiformer.py): features 84-97, LayerNorm 76-77,136 (flax eps 1e-6,
+// fast variance), attention 50-64, tanh-GELU 79.
+#include "nqs_common.h"
+#include "kernels.h"
+
+namespace nqs {
+
+// ------------------------------------------------------------------ features (psiformer.py:84-97,128-130)
+// Row of token t: [x: sin m=1..F, cos m=1..F, y: sin, cos, onehot(s_t) (2), zero pad to kpad].
+template <typename T>
+__global__ void k_features(JetGeom j, const double* __restrict__ R, const int32_t* __restrict__ S,
+                           double ai00, double ai01, double ai10, double ai11, int F, int kpad,
+                           int64_t w0, T* __restrict__ X) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = j.N;
+  if (gid >= j.n_cfg * N) return;
+  const int64_t cfg = gid / N;
+  const int t = (int)(gid % N);
+  int64_t w;
+  int flip = -1;
+  if (j.mode == 2) {
+    w = w0 + cfg / N;
+    flip = (int)(cfg % N);
+  } else {
+    w = w0 + cfg;
+  }
+  const double rx = R[(w * N + t) * 2 + 0], ry = R[(w * N + t) * 2 + 1];
+  int s = S[w * N + t];
+  if (t == flip) s = 1 - s;
+  const double sa[2] = {rx * ai00 + ry * ai01, rx * ai10 + ry * ai11};
+  const double ainv[2][2] = {{ai00, ai01}, {ai10, ai11}};
+  const int C = j.C;
+  T* base = X + (cfg * C) * (int64_t)N * kpad;
+  const int64_t chs = (int64_t)N * kpad;  // channel stride
+  // value channel
+  T* v = base + (int64_t)t * kpad;
+  for (int a = 0; a < 2; ++a) {
+    const double ph = 2.0 * M_PI * sa[a];
+    for (int m = 1; m <= F; ++m) {
+      double sn, cs;
+      sincos(ph * (double)m, &sn, &cs);
+      v[a * 2 * F + (m - 1)] = (T)sn;
+      v[a * 2 * F + F + (m - 1)] = (T)cs;
+      // tangents
+      for (int c = 1; c <= j.nt; ++c) {
+        int e, b;
+        if (j.mode == 2) {
+          e = flip;
+          b = c - 1;
+        } else {
+          e = (c - 1) >> 1;
+          b = (c - 1) & 1;
+        }
+        T* tc = base + c * chs + (int64_t)t * kpad;
+        const double dang = (t == e) ? 2.0 * M_PI * m * ainv[a][b] : 0.0;
+        tc[a * 2 * F + (m - 1)] = (T)(cs * dang);
+        tc[a * 2 * F + F + (m - 1)] = (T)(-sn * dang);
+      }
+      if (j.has_lap) {
+        T* lc = base + (j.nt + 1) * chs + (int64_t)t * kpad;
+        const double w2 = (2.0 * M_PI * m) * (2.0 * M_PI * m) * (ainv[a][0] * ainv[a][0] + ainv[a][1] * ainv[a][1]);
+        lc[a * 2 * F + (m - 1)] = (T)(-sn * w2);
+        lc[a * 2 * F + F + (m - 1)] = (T)(-cs * w2);
+      }
+    }
+  }
+  for (int c = 0; c < C; ++c) {
+    T* row = base + c * chs + (int64_t)t * kpad;
+    row[4 * F] = (c == 0) ? (T)(s == 0 ? 1 : 0) : (T)0;
+    row[4 * F + 1] = (c == 0) ? (T)(s == 1 ? 1 : 0) : (T)0;
+    for (int k = 4 * F + 2; k < kpad; ++k) row[k] = (T)0;
+  }
+}
+
+template <typename T>
+int launch_features(const JetGeom& j, const double* R, const int32_t* S, const double* Ainv,
+                    int n_freq, int kpad, int64_t w0, T* X, hipStream_t st) {
+  const int64_t n = j.n_cfg * j.N;
+  if (n == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_features<T>, dim3((unsigned)cdiv(n, 128)), dim3(128), 0, st, j, R, S, Ainv[0],
+                     Ainv[1], Ainv[2], Ainv[3], n_freq, kpad, w0, X);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ LayerNorm jet
+// One wave per (cfg, token).  mu = E[x], var = E[x^2] - mu^2, rs = (var + eps)^-1/2,
+// y = (x - mu) rs scale + bias.
+template <typename T, int MAXF>
+__global__ __launch_bounds__(64) void k_layernorm_jet(JetGeom j, int d, const T* __restrict__ x,
+                                                      const T* __restrict__ scale,
+                                                      const T* __restrict__ bias, T* __restrict__ y) {
+  const int64_t row = blockIdx.x;  // cfg * N + t
+  const int lane = threadIdx.x;
+  const int64_t cfg = row / j.N;
+  const int t = (int)(row % j.N);
+  const int64_t chs = (int64_t)j.N * d;
+  const T* xb = x + cfg * j.C * chs + (int64_t)t * d;
+  T* yb = y + cfg * j.C * chs + (int64_t)t * d;
+  const T inv_d = (T)1 / (T)d;
+  T xv[MAXF], sc[MAXF], s1[MAXF];
+  T sum = 0, sum2 = 0;
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    const int f = lane + 64 * q;
+    xv[q] = (f < d) ? xb[f] : (T)0;
+    sc[q] = (f < d) ? scale[f] : (T)0;
+    s1[q] = 0;
+    sum += xv[q];
+    sum2 += xv[q] * xv[q];
+  }
+  sum = wave_sum(sum);
+  sum2 = wave_sum(sum2);
+  const T mu = sum * inv_d;
+  T var = sum2 * inv_d - mu * mu;
+  var = var > (T)0 ? var : (T)0;
+  const T rs = (T)1 / sqrt(var + (T)1e-6);
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    const int f = lane + 64 * q;
+    if (f < d) yb[f] = (xv[q] - mu) * rs * sc[q] + bias[f];
+  }
+  T acc_dx2 = 0, acc_dmu2 = 0, acc_dvar2 = 0;
+  for (int c = 1; c <= j.nt; ++c) {
+    const T* xc = xb + c * chs;
+    T dx[MAXF];
+    T sd = 0, sxd = 0, sdd = 0;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      dx[q] = (f < d) ? xc[f] : (T)0;
+      sd += dx[q];
+      sxd += xv[q] * dx[q];
+      sdd += dx[q] * dx[q];
+    }
+    sd = wave_sum(sd);
+    sxd = wave_sum(sxd);
+    const T dmu = sd * inv_d;
+    const T dvar = (T)2 * (sxd * inv_d - mu * dmu);
+    const T drs = (T)(-0.5) * rs * rs * rs * dvar;
+    T* yc = yb + c * chs;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      const T u = dx[q] - dmu;
+      if (f < d) yc[f] = (u * rs + (xv[q] - mu) * drs) * sc[q];
+      s1[q] += u * drs;
+    }
+    if (j.has_lap) {
+      acc_dx2 += wave_sum(sdd) * inv_d;
+      acc_dmu2 += dmu * dmu;
+      acc_dvar2 += dvar * dvar;
+    }
+  }
+  if (j.has_lap) {
+    const int c = j.nt + 1;
+    const T* xc = xb + c * chs;
+    T lx[MAXF];
+    T sl = 0, sxl = 0;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      lx[q] = (f < d) ? xc[f] : (T)0;
+      sl += lx[q];
+      sxl += xv[q] * lx[q];
+    }
+    sl = wave_sum(sl);
+    sxl = wave_sum(sxl);
+    const T lmu = sl * inv_d;
+    const T lvar = (T)2 * (sxl * inv_d) + (T)2 * acc_dx2 - (T)2 * mu * lmu - (T)2 * acc_dmu2;
+    const T rs3 = rs * rs * rs;
+    const T lrs = (T)(-0.5) * rs3 * lvar + (T)0.75 * rs3 * rs * rs * acc_dvar2;
+    T* yc = yb + c * chs;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      if (f < d) yc[f] = ((lx[q] - lmu) * rs + (xv[q] - mu) * lrs + (T)2 * s1[q]) * sc[q];
+    }
+  }
+}
+
+template <typename T>
+int launch_layernorm_jet(const JetGeom& j, int d, const T* x, const T* scale, const T* bias, T* y,
+                         hipStream_t st) {
+  const int64_t rows = j.n_cfg * j.N;
+  if (rows == 0) return NQS_OK;
+  NQS_REQUIRE(d <= 512, NQS_ERR_UNSUPPORTED, "LayerNorm width above 512 is not supported");
+  if (d <= 128)
+    hipLaunchKernelGGL((k_layernorm_jet<T, 2>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+  else if (d <= 256)
+    hipLaunchKernelGGL((k_layernorm_jet<T, 4>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+  else
+    hipLaunchKernelGGL((k_layernorm_jet<T, 8>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ tanh-GELU jet
+template <typename T>
+__device__ inline void gelu_derivs(T x, T& g, T& g1, T& g2) {
+  const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
+  const T u = k0 * (x + k1 * x * x * x);
+  const T th = tanh(u);
+  const T sech2 = (T)1 - th * th;
+  const T du = k0 * ((T)1 + (T)3 * k1 * x * x);
+  const T ddu = (T)6 * k0 * k1 * x;
+  g = (T)0.5 * x * ((T)1 + th);
+  g1 = (T)0.5 * ((T)1 + th) + (T)0.5 * x * sech2 * du;
+  g2 = sech2 * du + (T)0.5 * x * (sech2 * ddu - (T)2 * sech2 * th * du * du);
+}
+
+template <typename T>
+__global__ void k_gelu_jet(JetGeom j, int width, const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_cfg = (int64_t)j.N * width;
+  if (gid >= j.n_cfg * per_cfg) return;
+  const int64_t cfg = gid / per_cfg, rem = gid % per_cfg;
+  const int64_t base = cfg * j.C * per_cfg + rem;
+  const T xv = x[base];
+  T g, g1, g2;
+  gelu_derivs(xv, g, g1, g2);
+  y[base] = g;
+  T s2 = 0;
+  for (int c = 1; c <= j.nt; ++c) {
+    const T dx = x[base + c * per_cfg];
+    y[base + c * per_cfg] = g1 * dx;
+    s2 += dx * dx;
+  }
+  if (j.has_lap) {
+    const int64_t o = base + (int64_t)(j.nt + 1) * per_cfg;
+    y[o] = g1 * x[o] + g2 * s2;
+  }
+}
+
+template <typename T>
+int launch_gelu_jet(const JetGeom& j, int width, const T* x, T* y, hipStream_t st) {
+  const int64_t n = j.n_cfg * j.N * width;
+  if (n == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_gelu_jet<T>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, j, width, x, y);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ attention jet (psiformer.py:50-64)
+// One workgroup per (cfg, head).  qkv rows have 3d columns [q | k | v], head h owns columns
+// [h hd, (h+1) hd) of each.  S = Q K^T / sqrt(hd), P = softmax_rows(S), O = P V.
+//   dS_c = (dQ_c K^T + Q dK_c^T) sc
+//   dP_c = P (dS_c - abar_c),            abar_c = rowdot(P, dS_c)
+//   Lap S = (LapQ K^T + Q LapK^T + 2 sum_c dQ_c dK_c^T) sc
+//   Lap P = P (Lap S - rowdot(P, Lap S) + sum_c [(dS_c - abar_c)^2 - rowdot(P, (dS_c - abar_c)^2)])
+//   dO_c = dP_c V + P dV_c,   Lap O = Lap P V + P Lap V + 2 sum_c dP_c dV_c
+template <typename T>
+__global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh, const T* __restrict__ qkv,
+                                                       T* __restrict__ out) {
+  extern __shared__ char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int N = j.N, hd = d / nh;
+  const int NN = N * N, NH = N * hd;
+  T* Q0 = sm;
+  T* K0 = Q0 + NH;
+  T* V0 = K0 + NH;
+  T* Qc = V0 + NH;
+  T* Kc = Qc + NH;
+  T* Vc = Kc + NH;
+  T* P = Vc + NH;
+  T* Sc = P + NN;
+  T* accS = Sc + NN;
+  T* accP = accS + NN;
+  T* accPV = accP + NN;
+  T* ab = accPV + NH;  // N
+  T* ab2 = ab + N;     // N
+
+  const int64_t cfg = blockIdx.x / nh;
+  const int h = blockIdx.x % nh;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int64_t rs = 3 * (int64_t)d;   // qkv row stride
+  const int64_t chs_in = (int64_t)N * rs;
+  const int64_t chs_out = (int64_t)N * d;
+  const T* in0 = qkv + cfg * j.C * chs_in;
+  T* out0 = out + cfg * j.C * chs_out;
+  const T sc = (T)1 / sqrt((T)hd);
+
+  auto load = [&](int c, T* q, T* k, T* v) {
+    const T* b = in0 + c * chs_in;
+    for (int e = tid; e < NH; e += nthr) {
+      const int t = e / hd, q_ = e % hd;
+      q[e] = b[t * rs + h * hd + q_];
+      k[e] = b[t * rs + d + h * hd + q_];
+      v[e] = b[t * rs + 2 * d + h * hd + q_];
+    }
+  };
+  // value channel
+  load(0, Q0, K0, V0);
+  __syncthreads();
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N, b = e % N;
+    T s = 0;
+    for (int q = 0; q < hd; ++q) s += Q0[a * hd + q] * K0[b * hd + q];
+    P[e] = s * sc;
+  }
+  __syncthreads();
+  for (int a = tid; a < N; a += nthr) {
+    T mx = P[a * N];
+    for (int b = 1; b < N; ++b) mx = P[a * N + b] > mx ? P[a * N + b] : mx;
+    T z = 0;
+    for (int b = 0; b < N; ++b) {
+      const T e_ = exp(P[a * N + b] - mx);
+      P[a * N + b] = e_;
+      z += e_;
+    }
+    const T iz = (T)1 / z;
+    for (int b = 0; b < N; ++b) P[a * N + b] *= iz;
+  }
+  __syncthreads();
+  for (int e = tid; e < NH; e += nthr) {
+    const int a = e / hd, q = e % hd;
+    T o = 0;
+    for (int b = 0; b < N; ++b) o += P[a * N + b] * V0[b * hd + q];
+    out0[a * d + h * hd + q] = o;
+  }
+  if (j.nt == 0 && !j.has_lap) return;
+  for (int e = tid; e < NN; e += nthr) {
+    accS[e] = 0;
+    accP[e] = 0;
+  }
+  for (int e = tid; e < NH; e += nthr) accPV[e] = 0;
+  __syncthreads();
+  for (int c = 1; c <= j.nt; ++c) {
+    load(c, Qc, Kc, Vc);
+    __syncthreads();
+    for (int e = tid; e < NN; e += nthr) {
+      const int a = e / N, b = e % N;
+      T s = 0, s2 = 0;
+      for (int q = 0; q < hd; ++q) {
+        s += Qc[a * hd + q] * K0[b * hd + q] + Q0[a * hd + q] * Kc[b * hd + q];
+        s2 += Qc[a * hd + q] * Kc[b * hd + q];
+      }
+      Sc[e] = s * sc;
+      accS[e] += s2;
+    }
+    __syncthreads();
+    for (int a = tid; a < N; a += nthr) {
+      T m = 0;
+      for (int b = 0; b < N; ++b) m += P[a * N + b] * Sc[a * N + b];
+      T m2 = 0;
+      for (int b = 0; b < N; ++b) {
+        const T u = Sc[a * N + b] - m;
+        m2 += P[a * N + b] * u * u;
+      }
+      ab[a] = m;
+      ab2[a] = m2;
+    }
+    __syncthreads();
+    for (int e = tid; e < NN; e += nthr) {
+      const int a = e / N;
+      const T u = Sc[e] - ab[a];
+      accP[e] += u * u - ab2[a];
+      Sc[e] = P[e] * u;  // dP_c
+    }
+    __syncthreads();
+    T* oc = out0 + c * chs_out;
+    for (int e = tid; e < NH; e += nthr) {
+      const int a = e / hd, q = e % hd;
+      T o = 0, o2 = 0;
+      for (int b = 0; b < N; ++b) {
+        o += Sc[a * N + b] * V0[b * hd + q] + P[a * N + b] * Vc[b * hd + q];
+        o2 += Sc[a * N + b] * Vc[b * hd + q];
+      }
+      oc[a * d + h * hd + q] = o;
+      accPV[e] += o2;
+    }
+    __syncthreads();
+  }
+  if (!j.has_lap) return;
+  const int cl = j.nt + 1;
+  load(cl, Qc, Kc, Vc);
+  __syncthreads();
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N, b = e % N;
+    T s = 0;
+    for (int q = 0; q < hd; ++q) s += Qc[a * hd + q] * K0[b * hd + q] + Q0[a * hd + q] * Kc[b * hd + q];
+    Sc[e] = (s + (T)2 * accS[e]) * sc;  // Lap S
+  }
+  __syncthreads();
+  for (int a = tid; a < N; a += nthr) {
+    T m = 0;
+    for (int b = 0; b < N; ++b) m += P[a * N + b] * Sc[a * N + b];
+    ab[a] = m;
+  }
+  __syncthreads();
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N;
+    Sc[e] = P[e] * (Sc[e] - ab[a] + accP[e]);  // Lap P
+  }
+  __syncthreads();
+  T* oc = out0 + cl * chs_out;
+  for (int e = tid; e < NH; e += nthr) {
+    const int a = e / hd, q = e % hd;
+    T o = 0;
+    for (int b = 0; b < N; ++b) o += Sc[a * N + b] * V0[b * hd + q] + P[a * N + b] * Vc[b * hd + q];
+    oc[a * d + h * hd + q] = o + (T)2 * accPV[e];
+  }
+}
+
+template <typename T>
+int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* out, hipStream_t st) {
+  const int64_t blocks = j.n_cfg * n_heads;
+  if (blocks == 0) return NQS_OK;
+  const int N = j.N, hd = d / n_heads;
+  const size_t lds = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
+  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");
+  hipLaunchKernelGGL(k_attention_jet<T>, dim3((unsigned)blocks), dim3(256), lds, st, j, d, n_heads, qkv, out);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+#define NQS_INST(T)                                                                                   \
+  template int launch_features<T>(const JetGeom&, const double*, const int32_t*, const double*, int, \
+                                  int, int64_t, T*, hipStream_t);                                   \
+  template int launch_layernorm_jet<T>(const JetGeom&, int, const T*, const T*, const T*, T*,       \
+                                       hipStream_t);                                                \
+  template int launch_gelu_jet<T>(const JetGeom&, int, const T*, T*, hipStream_t);                  \
+  template int launch_attention_jet<T>(const JetGeom&, int, int, const T*, T*, hipStream_t);
+NQS_INST(double)
+NQS_INST(float)
+
+}  // namespace nqs

@@ -1,0 +1,410 @@
+// Determinant head of the Psiformer with forward-mode jets, float64 throughout.
+//
+// Per configuration (one workgroup):
+//   env[a,i]   = sum_G W[a,G,s_i] exp(i G.r_i)                      (psiformer.py:145-148)
+//   M_k[a,i]   = env[a,i] (1 + bf[k,a,i])                           (:151)
+//   logdet_k   = sum log|u_pp| + log(prod u_pp/|u_pp| * parity)      (:152-153, LU with partial
+//                pivoting on |re|+|im| as LAPACK getrf does)
+//   tangents   d_c logdet = tr(M^-1 dM_c)
+//   Laplacian  Lap logdet = tr(M^-1 Lap M) - sum_c tr((M^-1 dM_c)^2)
+//   log psi    = logsumexp_k(logdet_k + w_k) (K > 1, :155-158) + Jastrow (:162-188)
+// dM_c = denv_c (1 + bf) + env dbf_c, where denv_c is non-zero only in the column of the
+// electron that tangent c moves; Lap M = Lap env (1+bf) + env Lap bf + 2 sum_c denv_c dbf_c.
+#include "nqs_common.h"
+#include "kernels.h"
+
+namespace nqs {
+
+__device__ inline void block_sum2(double& a, double& b, double* red) {
+  // sum (a, b) over the workgroup; every thread receives the result.  red: >= 2*8 doubles
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * wave] = a;
+    red[2 * wave + 1] = b;
+  }
+  __syncthreads();
+  a = 0;
+  b = 0;
+  for (int w = 0; w < nw; ++w) {
+    a += red[2 * w];
+    b += red[2 * w + 1];
+  }
+  __syncthreads();
+}
+
+// Jastrow pair function u(r) and its first two radial derivatives.
+__device__ inline void jastrow_u(const DetParams& p, bool same, double r, double& u, double& u1, double& u2) {
+  u = u1 = u2 = 0;
+  if (p.jastrow_kind == 0) {
+    const double* c = p.jas + (same ? 0 : p.n_knots);
+    const double w = p.rbf_w;
+    for (int k = 0; k < p.n_knots; ++k) {
+      const double mu = (p.n_knots > 1) ? p.rbf_rcut * (double)k / (double)(p.n_knots - 1) : 0.0;
+      const double z = (r - mu) / w;
+      const double e = exp(-z * z);
+      u += c[k] * e;
+      u1 += c[k] * e * (-2.0 * z / w);
+      u2 += c[k] * e * (4.0 * z * z - 2.0) / (w * w);
+    }
+  } else {
+    const double la = p.jas[0];
+    const double a = (la > 0 ? la + log1p(exp(-la)) : log1p(exp(la))) + 0.05;
+    const double al = same ? p.cusp / 3.0 : p.cusp;
+    const double e = exp(-r / a);
+    u = al * a * (1.0 - e);
+    u1 = al * e;
+    u2 = -al / a * e;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const double* __restrict__ R,
+                                                 const int32_t* __restrict__ S, int64_t w0,
+                                                 const T* __restrict__ bf, double* __restrict__ out_lp,
+                                                 double* __restrict__ out_tan, double* __restrict__ out_lap) {
+  extern __shared__ double shd[];
+  const int N = j.N, NN = N * N, K = p.K, nt = j.nt;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  cd* env = reinterpret_cast<cd*>(shd);
+  cd* dex = env + NN;
+  cd* dey = dex + NN;
+  cd* lenv = dey + NN;
+  cd* M = lenv + NN;
+  cd* Mi = M + NN;
+  cd* Ac = Mi + NN;
+  cd* res = Ac + NN;                      // per det: [logdet, tan(nt), lap]
+  double* pos = reinterpret_cast<double*>(res + (size_t)K * (nt + 2));
+  int* spin = reinterpret_cast<int*>(pos + 2 * N);
+  int* piv = spin + N;
+  int* flag = piv + N;                    // [0] swap parity, [1] singular
+  double* red = reinterpret_cast<double*>(flag + 4);
+
+  const int64_t cfg = blockIdx.x;
+  int64_t w;
+  int flip = -1;
+  if (j.mode == 2) {
+    w = w0 + cfg / N;
+    flip = (int)(cfg % N);
+  } else {
+    w = w0 + cfg;
+  }
+  for (int i = tid; i < N; i += nthr) {
+    pos[2 * i] = R[(w * N + i) * 2];
+    pos[2 * i + 1] = R[(w * N + i) * 2 + 1];
+    int s = S[w * N + i];
+    spin[i] = (i == flip) ? 1 - s : s;
+  }
+  __syncthreads();
+  // ---- envelope and its derivatives (column i depends on r_i only)
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N, i = e % N;
+    cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0), lp = mk(0, 0);
+    for (int g = 0; g < p.n_g; ++g) {
+      const double Gx = p.gs[2 * g], Gy = p.gs[2 * g + 1];
+      double sn, cs;
+      sincos(Gx * pos[2 * i] + Gy * pos[2 * i + 1], &sn, &cs);
+      const double* wp = p.wenv + ((size_t)(a * p.n_g + g) * 2 + spin[i]) * 2;
+      const cd t = mk(wp[0], wp[1]) * mk(cs, sn);
+      v = v + t;
+      gx = gx + t * mk(0, Gx);
+      gy = gy + t * mk(0, Gy);
+      lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
+    }
+    env[e] = v;
+    dex[e] = gx;
+    dey[e] = gy;
+    lenv[e] = lp;
+  }
+  __syncthreads();
+  const int C = j.C;
+  const int64_t KN = (int64_t)K * N;
+  const int64_t rowlen = 2 * KN;
+  const int64_t chs = (int64_t)N * rowlen;
+  const T* bfc0 = bf + cfg * C * chs;
+  auto bfval = [&](int c, int k, int a, int i) -> cd {
+    const T* r = bfc0 + c * chs + (int64_t)i * rowlen;
+    return mk((double)r[k * N + a], (double)r[KN + k * N + a]);
+  };
+  auto tan_elec = [&](int c, int& e, int& b) {
+    if (j.mode == 2) {
+      e = flip;
+      b = c - 1;
+    } else {
+      e = (c - 1) >> 1;
+      b = (c - 1) & 1;
+    }
+  };
+
+  for (int k = 0; k < K; ++k) {
+    for (int e = tid; e < NN; e += nthr) {
+      const int a = e / N, i = e % N;
+      M[e] = env[e] * (mk(1, 0) + bfval(0, k, a, i));
+    }
+    if (tid == 0) {
+      flag[0] = 0;
+      flag[1] = 0;
+    }
+    __syncthreads();
+    // ---- LU with partial pivoting, in place
+    for (int q = 0; q < N; ++q) {
+      if (tid == 0) {
+        int best = q;
+        double bv = fabs(M[q * N + q].re) + fabs(M[q * N + q].im);
+        for (int r = q + 1; r < N; ++r) {
+          const double v = fabs(M[r * N + q].re) + fabs(M[r * N + q].im);
+          if (v > bv) {
+            bv = v;
+            best = r;
+          }
+        }
+        piv[q] = best;
+        if (best != q) flag[0] ^= 1;
+        if (bv == 0.0) flag[1] = 1;
+      }
+      __syncthreads();
+      const int pr = piv[q];
+      if (pr != q)
+        for (int c = tid; c < N; c += nthr) {
+          const cd t = M[q * N + c];
+          M[q * N + c] = M[pr * N + c];
+          M[pr * N + c] = t;
+        }
+      __syncthreads();
+      const cd pv = M[q * N + q];
+      const bool zero = (pv.re == 0.0 && pv.im == 0.0);
+      for (int r = q + 1 + tid; r < N; r += nthr) M[r * N + q] = zero ? mk(0, 0) : cdiv_(M[r * N + q], pv);
+      __syncthreads();
+      const int w_ = N - q - 1;
+      for (int e = tid; e < w_ * w_; e += nthr) {
+        const int r = q + 1 + e / w_, c = q + 1 + e % w_;
+        M[r * N + c] = M[r * N + c] - M[r * N + q] * M[q * N + c];
+      }
+      __syncthreads();
+    }
+    // ---- log det
+    if (tid == 0) {
+      double la = 0, sre = (flag[0] ? -1.0 : 1.0), sim = 0;
+      for (int q = 0; q < N; ++q) {
+        const cd u = M[q * N + q];
+        const double au = cabs_(u);
+        la += log(au);
+        const double ur = au > 0 ? u.re / au : 0.0, ui = au > 0 ? u.im / au : 0.0;
+        const double nr = sre * ur - sim * ui, ni = sre * ui + sim * ur;
+        sre = nr;
+        sim = ni;
+      }
+      // log(sign): principal log of a unit complex number (or log 0 = -inf)
+      const double ls = log(hypot(sre, sim));
+      res[(size_t)k * (nt + 2)] = mk(la + ls, atan2(sim, sre));
+    }
+    if (nt == 0 && !j.has_lap) {
+      __syncthreads();
+      continue;
+    }
+    // ---- explicit inverse Mi = U^-1 L^-1 P, column c by thread c
+    for (int c = tid; c < N; c += nthr) {
+      for (int r = 0; r < N; ++r) Mi[r * N + c] = mk(r == c ? 1.0 : 0.0, 0);
+      for (int q = 0; q < N; ++q) {
+        const int pr = piv[q];
+        if (pr != q) {
+          const cd t = Mi[q * N + c];
+          Mi[q * N + c] = Mi[pr * N + c];
+          Mi[pr * N + c] = t;
+        }
+      }
+      for (int r = 1; r < N; ++r) {
+        cd s = Mi[r * N + c];
+        for (int q = 0; q < r; ++q) s = s - M[r * N + q] * Mi[q * N + c];
+        Mi[r * N + c] = s;
+      }
+      for (int r = N - 1; r >= 0; --r) {
+        cd s = Mi[r * N + c];
+        for (int q = r + 1; q < N; ++q) s = s - M[r * N + q] * Mi[q * N + c];
+        Mi[r * N + c] = cdiv_(s, M[r * N + r]);
+      }
+    }
+    __syncthreads();
+    cd hsum = mk(0, 0);
+    for (int c = 1; c <= nt; ++c) {
+      int ec, bc;
+      tan_elec(c, ec, bc);
+      // dM_c in Ac
+      for (int e = tid; e < NN; e += nthr) {
+        const int a = e / N, i = e % N;
+        cd v = env[e] * bfval(c, k, a, i);
+        if (i == ec) v = v + (bc == 0 ? dex[e] : dey[e]) * (mk(1, 0) + bfval(0, k, a, i));
+        Ac[e] = v;
+      }
+      __syncthreads();
+      if (!j.has_lap) {
+        double tr = 0, ti = 0;
+        for (int e = tid; e < NN; e += nthr) {
+          const int i = e / N, a = e % N;
+          const cd t = Mi[i * N + a] * Ac[a * N + i];
+          tr += t.re;
+          ti += t.im;
+        }
+        block_sum2(tr, ti, red);
+        if (tid == 0) res[(size_t)k * (nt + 2) + c] = mk(tr, ti);
+      } else {
+        // A = Mi dM_c (N x N); each thread its elements, kept in registers for the trace terms
+        cd av[4];
+        int cnt = 0;
+        for (int e = tid; e < NN; e += nthr, ++cnt) {
+          const int i = e / N, jj = e % N;
+          cd s = mk(0, 0);
+          for (int a = 0; a < N; ++a) s = s + Mi[i * N + a] * Ac[a * N + jj];
+          av[cnt] = s;
+        }
+        __syncthreads();
+        cnt = 0;
+        for (int e = tid; e < NN; e += nthr, ++cnt) Ac[e] = av[cnt];
+        __syncthreads();
+        double tr = 0, ti = 0, hr = 0, hi = 0;
+        cnt = 0;
+        for (int e = tid; e < NN; e += nthr, ++cnt) {
+          const int i = e / N, jj = e % N;
+          if (i == jj) {
+            tr += av[cnt].re;
+            ti += av[cnt].im;
+          }
+          const cd t = av[cnt] * Ac[jj * N + i];
+          hr += t.re;
+          hi += t.im;
+        }
+        block_sum2(tr, ti, red);
+        block_sum2(hr, hi, red);
+        if (tid == 0) res[(size_t)k * (nt + 2) + c] = mk(tr, ti);
+        hsum = hsum + mk(hr, hi);
+      }
+    }
+    if (j.has_lap) {
+      const int cl = nt + 1;
+      for (int e = tid; e < NN; e += nthr) {
+        const int a = e / N, i = e % N;
+        const cd one_bf = mk(1, 0) + bfval(0, k, a, i);
+        cd v = lenv[e] * one_bf + env[e] * bfval(cl, k, a, i);
+        v = v + 2.0 * (dex[e] * bfval(1 + 2 * i, k, a, i) + dey[e] * bfval(2 + 2 * i, k, a, i));
+        Ac[e] = v;
+      }
+      __syncthreads();
+      double tr = 0, ti = 0;
+      for (int e = tid; e < NN; e += nthr) {
+        const int i = e / N, a = e % N;
+        const cd t = Mi[i * N + a] * Ac[a * N + i];
+        tr += t.re;
+        ti += t.im;
+      }
+      block_sum2(tr, ti, red);
+      if (tid == 0) res[(size_t)k * (nt + 2) + nt + 1] = mk(tr, ti) - hsum;
+    }
+    __syncthreads();
+  }
+  // ---- log-sum-exp over determinants + Jastrow, by thread 0 (K x nt work)
+  if (tid == 0) {
+    cd L;
+    double amax = -INFINITY;
+    for (int k = 0; k < K; ++k) {
+      cd z = res[(size_t)k * (nt + 2)];
+      if (K > 1) z = z + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]);
+      res[(size_t)k * (nt + 2)] = z;
+      amax = z.re > amax ? z.re : amax;
+    }
+    if (K == 1) {
+      L = res[0];
+    } else {
+      if (!isfinite(amax)) amax = 0.0;
+      cd s = mk(0, 0);
+      for (int k = 0; k < K; ++k) s = s + cexp_(res[(size_t)k * (nt + 2)] - mk(amax, 0));
+      L = clog_(s) + mk(amax, 0);
+      // weights p_k = exp(z_k - L)
+      for (int k = 0; k < K; ++k) res[(size_t)k * (nt + 2)] = cexp_(res[(size_t)k * (nt + 2)] - L);
+    }
+    cd lap = mk(0, 0), g2 = mk(0, 0);
+    for (int c = 1; c <= nt; ++c) {
+      cd g = mk(0, 0);
+      for (int k = 0; k < K; ++k) {
+        const cd pk = (K == 1) ? mk(1, 0) : res[(size_t)k * (nt + 2)];
+        const cd gk = res[(size_t)k * (nt + 2) + c];
+        g = g + pk * gk;
+        if (j.has_lap) lap = lap + pk * (gk * gk);
+      }
+      red[2 + 2 * c] = g.re;  // stash (red has room: sized for 2 + 2 (nt+1))
+      red[3 + 2 * c] = g.im;
+      g2 = g2 + g * g;
+    }
+    if (j.has_lap) {
+      for (int k = 0; k < K; ++k) {
+        const cd pk = (K == 1) ? mk(1, 0) : res[(size_t)k * (nt + 2)];
+        lap = lap + pk * res[(size_t)k * (nt + 2) + nt + 1];
+      }
+      lap = lap - g2;
+    }
+    // Jastrow
+    if (p.use_jastrow) {
+      double J = 0, lJ = 0;
+      for (int c = 1; c <= nt; ++c) red[2 + 2 * (nt + 1) + c] = 0;  // dJ per tangent
+      for (int a = 0; a < N; ++a)
+        for (int b = a + 1; b < N; ++b) {
+          const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
+          fx -= rint(fx);
+          fy -= rint(fy);
+          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
+          const double d2 = mx * mx + my * my;
+          const double r = sqrt(d2 + 1e-12);
+          double u, u1, u2;
+          jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
+          J += u;
+          lJ += 2.0 * (u2 * d2 / (r * r) + u1 * (2.0 / r - d2 / (r * r * r)));
+          for (int c = 1; c <= nt; ++c) {
+            int ec, bc;
+            tan_elec(c, ec, bc);
+            const double comp = (bc == 0 ? mx : my) * u1 / r;
+            if (ec == a) red[2 + 2 * (nt + 1) + c] += comp;
+            if (ec == b) red[2 + 2 * (nt + 1) + c] -= comp;
+          }
+        }
+      L = L + mk(J, 0);
+      if (j.has_lap) lap = lap + mk(lJ, 0);
+      for (int c = 1; c <= nt; ++c) red[2 + 2 * c] += red[2 + 2 * (nt + 1) + c];
+    }
+    out_lp[2 * cfg] = L.re;
+    out_lp[2 * cfg + 1] = L.im;
+    for (int c = 1; c <= nt; ++c) {
+      out_tan[(cfg * nt + (c - 1)) * 2] = red[2 + 2 * c];
+      out_tan[(cfg * nt + (c - 1)) * 2 + 1] = red[3 + 2 * c];
+    }
+    if (j.has_lap && out_lap) {
+      out_lap[2 * cfg] = lap.re;
+      out_lap[2 * cfg + 1] = lap.im;
+    }
+  }
+}
+
+// Note on rint vs jnp.round: both round half to even; minimum image is exact either way.
+template <typename T>
+int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const int32_t* S, int64_t w0,
+                   const T* bf, double* out_lp, double* out_tan, double* out_lap, hipStream_t st) {
+  if (j.n_cfg == 0) return NQS_OK;
+  const int N = j.N, NN = N * N;
+  NQS_REQUIRE(NN <= 4 * 256, NQS_ERR_UNSUPPORTED, "determinant head supports N <= 32");
+  size_t lds = sizeof(cd) * (size_t)(7 * NN + (size_t)p.K * (j.nt + 2)) + sizeof(double) * 2 * N +
+               sizeof(int) * (2 * N + 4) + sizeof(double) * (size_t)(8 + 2 + 2 * (j.nt + 1) + (j.nt + 1) + 8);
+  lds = round_up((int64_t)lds, 16);
+  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "determinant head exceeds LDS");
+  hipLaunchKernelGGL(k_det_jet<T>, dim3((unsigned)j.n_cfg), dim3(256), lds, st, j, p, R, S, w0, bf, out_lp,
+                     out_tan, out_lap);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+template int launch_det_jet<double>(const JetGeom&, const DetParams&, const double*, const int32_t*, int64_t,
+                                    const double*, double*, double*, double*, hipStream_t);
+template int launch_det_jet<float>(const JetGeom&, const DetParams&, const double*, const int32_t*, int64_t,
+                                   const float*, double*, double*, double*, hipStream_t);
+
+}  // namespace nqs

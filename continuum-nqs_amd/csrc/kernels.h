@@ -1,0 +1,67 @@
+// Internal launcher declarations shared by the kernel files and the C ABI layer.
+#pragma once
+#include "nqs_common.h"
+
+namespace nqs {
+
+template <typename T>
+struct GemmArgs {
+  const T* A;  // M x K, row stride lda
+  const T* B;  // N x K, row stride ldb
+  T* C;        // M x N, row stride ldc
+  int64_t M, N, K, lda, ldb, ldc;
+  int64_t strideA, strideB, strideC;  // batch strides (gridDim.z)
+  T alpha, beta;
+  const T* bias;        // N or NULL
+  int64_t bias_period;  // bias added to rows with (row % bias_period) < bias_len
+  int64_t bias_len;
+  int tri;     // 0 full, 1 lower-triangular tiles only (col <= row)
+  int mirror;  // also store C[col][row]
+};
+
+template <typename T>
+int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st);
+
+// Jet geometry: activations are laid out [cfg][channel][token][feature]; channel 0 is the
+// value, channels 1..nt the tangents, channel nt+1 the Laplacian when has_lap.
+struct JetGeom {
+  int64_t n_cfg;  // configurations in this launch
+  int N;          // tokens (electrons)
+  int C;          // channels = 1 + nt + has_lap
+  int nt;         // tangent channels
+  int has_lap;
+  int mode;       // 0 value, 1 full Laplacian (nt = 2N), 2 spin-flip (cfg = w*N + i, nt = 2)
+};
+
+template <typename T>
+int launch_features(const JetGeom& j, const double* R, const int32_t* S, const double* Ainv,
+                    int n_freq, int kpad, int64_t w0, T* X, hipStream_t st);
+template <typename T>
+int launch_layernorm_jet(const JetGeom& j, int d, const T* x, const T* scale, const T* bias, T* y,
+                         hipStream_t st);
+template <typename T>
+int launch_gelu_jet(const JetGeom& j, int width, const T* x, T* y, hipStream_t st);
+template <typename T>
+int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* out,
+                         hipStream_t st);
+
+// Determinant head: envelope, K spinor determinants, log-sum-exp, Jastrow.  Outputs per cfg:
+// logpsi (complex), tangents (nt complex), Laplacian (complex, when has_lap).
+struct DetParams {
+  int N, n_g, K, use_jastrow, jastrow_kind, n_knots;
+  double A[4], Ainv[4];
+  double cusp, rbf_rcut, rbf_w;
+  const double* gs;       // (n_g, 2)
+  const double* wenv;     // (N, n_g, 2) complex interleaved (includes dW)
+  const double* det_logw; // (K) complex or NULL
+  const double* jas;      // jastrow params: rbf [c_same(n_knots), c_opp(n_knots)]; cusp [loga]
+};
+template <typename T>
+int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const int32_t* S,
+                   int64_t w0, const T* bf, double* out_logpsi, double* out_tan, double* out_lap,
+                   hipStream_t st);
+
+int launch_ewald(const nqs_ewald_desc& e, const double* R, int64_t W, int N, double* out,
+                 hipStream_t st);
+
+}  // namespace nqs

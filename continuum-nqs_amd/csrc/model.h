@@ -1,0 +1,62 @@
+// Model plan: real_ravel parameter layout, packed kernel layout, workspace plans.
+#pragma once
+#include <string>
+#include <vector>
+#include "nqs_common.h"
+#include "kernels.h"
+
+namespace nqs {
+
+struct Leaf {
+  std::string name;  // "params/TransformerBlock_0/attn/q/kernel"
+  int64_t shape[3];
+  int ndim;
+  bool cplx;
+  int64_t offset;  // in theta
+  int64_t size;    // elements per real/imag part
+};
+
+// Offsets (in elements of the backbone type T) of the packed backbone tensors.
+struct LayerPack {
+  int64_t ln0_s, ln0_b, qkvT, qkv_b, oT, o_b, ln1_s, ln1_b, w0T, b0, w1T, b1;
+  int64_t qkv, o, w0, w1;  // non-transposed (in x out) copies for the backward pass
+};
+
+struct Model {
+  nqs_model_desc m;
+  int N, d, depth, nh, ff, F, K, n_g, kin, kpad, KN;
+  double A[4], Ainv[4];
+  int64_t P;
+  std::vector<Leaf> leaves;
+  // packed layout: backbone part in T elements, then a float64 part (byte offset dbl_off)
+  int64_t embT, emb_b, emb, lnf_s, lnf_b, bfT, bf_b, bfW;
+  std::vector<LayerPack> layers;
+  int64_t n_backbone;  // elements of T
+  int64_t dbl_off;     // bytes
+  int64_t gs_off, wenv_off, dlw_off, jas_off, n_dbl;  // offsets in doubles from dbl_off
+  int64_t packed_bytes;
+  int tsize;           // sizeof(T)
+};
+
+int build_model(const nqs_model_desc* m, Model& md);
+const Leaf* find_leaf(const Model& md, const std::string& name);
+
+// Per-chunk buffers of a jet forward.
+template <typename T>
+struct FwdBufs {
+  T *X0, *x, *h, *big, *att;
+  double *lp, *tan, *lap;
+};
+
+DetParams det_params(const Model& md, const void* packed);
+
+template <typename T>
+int run_forward(const Model& md, const void* packed, const JetGeom& jg, const double* R, const int32_t* S,
+                int64_t w0, const FwdBufs<T>& b, hipStream_t st);
+
+// bytes of FwdBufs for n_cfg configurations with C channels and nt tangents
+int64_t fwd_bytes(const Model& md, int64_t n_cfg, int C, int nt);
+template <typename T>
+FwdBufs<T> carve_fwd(const Model& md, char*& p, int64_t n_cfg, int C, int nt);
+
+}  // namespace nqs

@@ -1,0 +1,115 @@
+"""ctypes binding of libnqs_hip.so (the C ABI in include/nqs_hip.h).
+
+The library is loaded from this directory; there is no fallback path: when the library or a
+GPU is missing, the compute entry points raise.  Device memory is torch (ROCm) tensors, the
+stream is torch's current stream.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnqs_hip.so")
+
+NQS_OK, NQS_ERR_ARG, NQS_ERR_HIP, NQS_ERR_WORKSPACE, NQS_ERR_NUMERIC, NQS_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+NQS_FP64, NQS_MIXED = 0, 1
+OP_LOGPSI, OP_ELOC, OP_MCMC, OP_GRAD = 0, 1, 2, 3
+
+c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("n_elec", c_i32), ("n_g", c_i32), ("embed_dim", c_i32), ("depth", c_i32),
+                ("n_heads", c_i32), ("ff_mult", c_i32), ("n_freq", c_i32), ("n_dets", c_i32),
+                ("learnable_env", c_i32), ("use_jastrow", c_i32), ("jastrow_kind", c_i32),
+                ("n_knots", c_i32), ("precision", c_i32), ("reserved", c_i32),
+                ("cell", c_dbl * 4), ("cell_inv", c_dbl * 4), ("cusp", c_dbl)]
+
+
+class HamDesc(ctypes.Structure):
+    _fields_ = [("m_star", c_dbl), ("lambda_r", c_dbl), ("h_z", c_dbl), ("n_field", c_i32),
+                ("reserved", c_i32), ("field_dev", c_vp)]
+
+
+class EwaldDesc(ctypes.Structure):
+    _fields_ = [("alpha", c_dbl), ("eta", c_dbl), ("v_g0", c_dbl), ("zeta_m", c_dbl),
+                ("n_img", c_i32), ("n_g", c_i32), ("img_dev", c_vp), ("g_dev", c_vp), ("rg_dev", c_vp)]
+
+
+# (name, restype, argtypes) of every exported symbol; the CPU tests check each one resolves.
+SIGNATURES = [
+    ("nqs_version", c_i32, []),
+    ("nqs_last_error", ctypes.c_char_p, []),
+    ("nqs_param_count", c_i64, [ctypes.POINTER(ModelDesc)]),
+    ("nqs_param_leaf", c_i32, [ctypes.POINTER(ModelDesc), c_i32, ctypes.c_char_p, c_i32,
+                               ctypes.POINTER(c_i64), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
+                               ctypes.POINTER(c_i32)]),
+    ("nqs_packed_bytes", c_i64, [ctypes.POINTER(ModelDesc)]),
+    ("nqs_pack_params", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("nqs_workspace_bytes", c_i64, [ctypes.POINTER(ModelDesc), c_i32, c_i64]),
+    ("nqs_psiformer_fwd", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_eloc", c_i32, [ctypes.POINTER(ModelDesc), ctypes.POINTER(HamDesc), c_vp, c_vp, c_vp, c_i64, c_vp,
+                         c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_ewald", c_i32, [ctypes.POINTER(EwaldDesc), c_vp, c_i64, c_i32, c_vp, c_vp]),
+    ("nqs_mcmc_sweeps", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_dbl,
+                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+]
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc, what):
+    if rc != NQS_OK:
+        msg = load().nqs_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed with code {rc}: {msg}")
+
+
+def ptr(t):
+    """Device pointer of a contiguous torch tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    assert t.is_contiguous(), "tensor must be contiguous"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def device():
+    """The GPU every product tensor lives on; raises when no GPU is present."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the nqs HIP path needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+_WS = {}
+
+
+def workspace(nbytes, tag="default"):
+    """A cached device byte buffer of at least nbytes (grown on demand, reused across calls)."""
+    import torch
+    buf = _WS.get(tag)
+    if buf is None or buf.numel() < nbytes:
+        _WS.pop(tag, None)
+        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+        _WS[tag] = buf
+    return buf

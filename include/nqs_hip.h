@@ -1,0 +1,141 @@
+/* C ABI of the gfx950 VMC hot path (libnqs_hip.so).
+ *
+ * The reference (wenqin-chen/continuum-nqs) is pure Python/JAX: its "interface" for this path is
+ * the set of closure factories that the Python host layer (continuum-nqs_amd/nqs/) re-exposes
+ * with the same names.  Each entry point below is what one of those factories' returned
+ * callables binds to (the ctypes stubs are in continuum-nqs_amd/nqs/_lib.py):
+ *
+ *   nqs_psiformer_fwd        <- make_logpsi(model, params)(R, S)         src/nqs/psiformer.py:192-196
+ *   nqs_eloc                 <- make_local_energy(...)(R, S)             src/nqs/local_energy.py:40-117
+ *   nqs_ewald                <- make_ewald[_lattice](...)[2](R)           src/nqs/ewald.py:26-73
+ *   nqs_mcmc_sweeps          <- make_mcmc(...) sweep, scanned            src/nqs/sampler.py:25-91
+ *   nqs_psiformer_grad       <- per_sample_O / jax.grad of log psi       src/nqs/minsr.py:56-65
+ *   nqs_minsr_*              <- minsr_direction(O, eps, lam)             src/nqs/minsr.py:68-81
+ *   nqs_param_*              <- real_ravel(params) ordering              src/nqs/minsr.py:30-53
+ *
+ * Conventions: every pointer argument named *_dev is device memory owned by the caller;
+ * complex arrays are interleaved (re, im) float64; positions R are (W, N, 2) float64, spins S
+ * are (W, N) int32 in {0, 1}.  Calls are asynchronous on `stream` (a hipStream_t, may be NULL
+ * for the default stream), never allocate device memory, and use only the workspace the
+ * caller passes (size from nqs_workspace_bytes).  Return value 0 = success, < 0 = error code;
+ * nqs_last_error() gives a message for the calling thread.  No exception crosses the ABI.
+ */
+#ifndef NQS_HIP_H
+#define NQS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NQS_OK = 0,
+  NQS_ERR_ARG = -1,
+  NQS_ERR_HIP = -2,
+  NQS_ERR_WORKSPACE = -3,
+  NQS_ERR_NUMERIC = -4,
+  NQS_ERR_UNSUPPORTED = -5
+};
+
+/* precision of the backbone (features, attention, MLP, LayerNorm, backflow); the determinant,
+ * envelope, Jastrow, Ewald and all linear algebra of MinSR's solve stay float64 in both. */
+enum { NQS_FP64 = 0, NQS_MIXED = 1 };
+
+/* ops for nqs_workspace_bytes */
+enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3 };
+
+/* Architecture + cell of ContinuumSpinorPsiformer (src/nqs/psiformer.py:100-116). */
+typedef struct nqs_model_desc {
+  int32_t n_elec;        /* N electrons (= orbitals per determinant) */
+  int32_t n_g;           /* plane waves of the spinor envelope */
+  int32_t embed_dim;     /* d */
+  int32_t depth;         /* transformer blocks */
+  int32_t n_heads;
+  int32_t ff_mult;
+  int32_t n_freq;        /* Fourier harmonics per axis */
+  int32_t n_dets;        /* K generalized spinor determinants */
+  int32_t learnable_env; /* adds the complex leaf dW (N, n_g, 2) */
+  int32_t use_jastrow;
+  int32_t jastrow_kind;  /* 0 = rbf, 1 = cusp */
+  int32_t n_knots;
+  int32_t precision;     /* NQS_FP64 or NQS_MIXED */
+  int32_t reserved;
+  double cell[4];        /* A row-major; columns are the lattice vectors a1, a2 */
+  double cell_inv[4];    /* A^-1 row-major, as the caller computes it (positions are wrapped
+                            with exactly these numbers) */
+  double cusp;
+} nqs_model_desc;
+
+/* Single-particle Hamiltonian of make_local_energy (src/nqs/local_energy.py:40-44).
+ * The external field is a Fourier series: f(r) = sum_t c_t cos(q_t.r) + s_t sin(q_t.r),
+ * term t = 8 doubles (qx, qy, cx, cy, cz, sx, sy, sz) in device memory; a constant field is a
+ * term with q = 0.  n_field = 0 means ext_field = None. */
+typedef struct nqs_hamiltonian_desc {
+  double m_star;
+  double lambda_r;
+  double h_z;
+  int32_t n_field;
+  int32_t reserved;
+  const double* field_dev;
+} nqs_hamiltonian_desc;
+
+/* Precomputed Ewald tables of make_ewald_lattice (src/nqs/ewald.py:26-58). */
+typedef struct nqs_ewald_desc {
+  double alpha;
+  double eta;
+  double v_g0;
+  double zeta_m;
+  int32_t n_img;
+  int32_t n_g;
+  const double* img_dev; /* (n_img, 2) lattice images A n */
+  const double* g_dev;   /* (n_g, 2) reciprocal vectors, G = 0 excluded */
+  const double* rg_dev;  /* (n_g,) (2 pi / area) erfc(|G|/2 eta)/|G| */
+} nqs_ewald_desc;
+
+int nqs_version(void);
+const char* nqs_last_error(void);
+
+/* ---- parameters: flat float64 vector theta in real_ravel order (minsr.py:30-53) */
+int64_t nqs_param_count(const nqs_model_desc* m);
+/* Leaf idx of the flattened tree; returns NQS_ERR_ARG past the last leaf. */
+int nqs_param_leaf(const nqs_model_desc* m, int32_t idx, char* name, int32_t name_len,
+                   int64_t shape[3], int32_t* ndim, int64_t* offset, int32_t* is_complex);
+int64_t nqs_packed_bytes(const nqs_model_desc* m);
+/* theta -> kernel layout (transposed weights in the backbone precision, float64 envelope).
+ * gs_dev (n_g, 2) float64; wenv_dev (N, n_g, 2) complex interleaved. */
+int nqs_pack_params(const nqs_model_desc* m, const double* theta_dev, const double* gs_dev,
+                    const double* wenv_dev, void* packed_dev, void* stream);
+
+int64_t nqs_workspace_bytes(const nqs_model_desc* m, int32_t op, int64_t n_walkers);
+
+/* ---- log psi apply: logpsi_dev (W) complex */
+int nqs_psiformer_fwd(const nqs_model_desc* m, const void* packed_dev, const double* R_dev,
+                      const int32_t* S_dev, int64_t n_walkers, double* logpsi_dev, void* ws_dev,
+                      int64_t ws_bytes, void* stream);
+
+/* ---- local energy (kinetic by forward Laplacian + Rashba SOC + Zeeman + ext field).
+ * eloc_dev (W) complex; logpsi_dev (W) complex or NULL; grad_dev (W, 2N) complex or NULL;
+ * lap_dev (W) complex or NULL (the Laplacian of log psi). */
+int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void* packed_dev,
+             const double* R_dev, const int32_t* S_dev, int64_t n_walkers, double* eloc_dev,
+             double* logpsi_dev, double* grad_dev, double* lap_dev, void* ws_dev, int64_t ws_bytes,
+             void* stream);
+
+/* ---- Ewald Coulomb energy per walker: energy_dev (W) float64 */
+int nqs_ewald(const nqs_ewald_desc* e, const double* R_dev, int64_t n_walkers, int32_t n_elec,
+              double* energy_dev, void* stream);
+
+/* ---- Metropolis sweeps over positions and spins with caller-supplied streams
+ * xi (n_sweeps, W, N, 2) f64, u_pos (n_sweeps, W) f64, flip_i (n_sweeps, n_flip, W) i32,
+ * u_flip (n_sweeps, n_flip, W) f64.  R, S, logp (W) = 2 Re log psi of the current state, are
+ * updated in place; acc_pos (W), acc_spin (W) count accepted moves (int32, accumulated). */
+int nqs_mcmc_sweeps(const nqs_model_desc* m, const void* packed_dev, double* R_dev, int32_t* S_dev,
+                    double* logp_dev, int64_t n_walkers, int32_t n_sweeps, int32_t n_flip,
+                    double step, const double* xi_dev, const double* u_pos_dev,
+                    const int32_t* flip_i_dev, const double* u_flip_dev, int32_t* acc_pos_dev,
+                    int32_t* acc_spin_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""CPU pins of the oracle (no GPU): closed forms, identities and invariants from the reference's
+own tests, restated against oracle/ (tests/test_nqs_local_energy.py, test_nqs_psiformer.py,
+test_nqs_spiral.py, test_nqs_ewald.py, test_nqs_cell.py, test_nqs_minsr.py,
+test_nqs_rashba_reference.py)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import bands, ewald as oew, local_energy as ole, minsr as oms, psiformer as opf, sampler as osa
+
+
+def _rashba(N=5, L=7.0, lr=1.0, hz=0.5):
+    return bands.rashba_gs(L, N, 1.0, lr, hz, n_cut=10, twist=(0.1, 0.17))
+
+
+def test_bands_closed_form_and_brute_force():
+    g = _rashba()
+    assert abs(g["E_gs"] - g["E_closed"]) < 1e-12
+    # brute force: lowest N eigenvalues of the block-diagonal plane-wave Hamiltonian
+    L, N = 7.0, 5
+    n = np.arange(-10, 11)
+    ev = []
+    for a in n:
+        for b in n:
+            kx, ky = (2 * np.pi * a + 0.1) / L, (2 * np.pi * b + 0.17) / L
+            ev += list(np.linalg.eigvalsh(bands.bloch_h(kx, ky, 1.0, 1.0, 0.5)))
+    assert abs(sum(sorted(ev)[:N]) - g["E_gs"]) < 1e-12
+    # Rashba ring minimum E = -lambda^2 m*/2 at |k| = lambda m* (h_z = 0)
+    lo, _ = bands.band_energies(1.0, 0.0, 1.0, 1.0, 0.0)
+    assert abs(lo + 0.5) < 1e-15
+
+
+def _slater_logpsi(g):
+    return lambda R, S: opf.slater_logpsi(g["Gs"], g["W"], R, S)
+
+
+@pytest.mark.parametrize("N,L,lr,hz", [(5, 7.0, 1.0, 0.5), (7, 8.0, 1.0, 0.5)])
+def test_oracle_zero_variance(N, L, lr, hz):
+    g = bands.rashba_gs(L, N, 1.0, lr, hz, n_cut=10, twist=(0.1, 0.17))
+    E = ole.make_local_energy(_slater_logpsi(g), 1.0, lr, hz)
+    rng = np.random.default_rng(0)
+    vals = np.array([E(torch.tensor(rng.uniform(0, L, (N, 2))), torch.tensor(rng.integers(0, 2, N))) for _ in range(6)])
+    assert np.std(vals.real) < 1e-7 and abs(vals.real.mean() - g["E_gs"]) < 1e-7 and np.max(np.abs(vals.imag)) < 1e-7
+
+
+def test_oracle_zero_variance_fails_for_a_planted_sign_error():
+    """A flipped SOC sign (sigma_y -> -sigma_y) must break the certificate: the pin is sensitive."""
+    g = _rashba()
+    E = ole.make_local_energy(_slater_logpsi(g), 1.0, 1.0, 0.5)
+    saved = ole._SY
+    try:
+        ole._SY = ((0.0, 1.0j), (-1.0j, 0.0))
+        rng = np.random.default_rng(0)
+        vals = np.array([E(torch.tensor(rng.uniform(0, 7, (5, 2))), torch.tensor(rng.integers(0, 2, 5)))
+                         for _ in range(6)])
+    finally:
+        ole._SY = saved
+    assert np.std(vals.real) > 1e-3
+
+
+def test_oracle_polarized_limit():
+    N, L, hz = 5, 9.0, 4.0
+    g = bands.rashba_gs(L, N, 1.0, 0.0, hz, n_cut=10, twist=(0.1, 0.17))
+    E = ole.make_local_energy(_slater_logpsi(g), 1.0, 0.0, hz)
+    R = torch.tensor(np.random.default_rng(1).uniform(0, L, (N, 2)))
+    e = E(R, torch.zeros(N, dtype=torch.long))
+    ekin = float(np.sum(g["Gs"] ** 2) / 2.0)
+    assert abs(e.real - (ekin - 0.5 * hz * N)) < 1e-7 and abs(e.real - g["E_gs"]) < 1e-7
+
+
+def test_oracle_spiral_zero_variance():
+    res = bands.spiral_gs(7, 8.0, (1, 0), 0.3, lambda_r=1.0, h_z=0.0, n_cut=10, twist=(0.1, 0.17))
+    assert res["boundary_weight"] < 1e-13 and res["gap"] > 1e-6
+    lp = lambda R, S: opf.slater_logpsi(res["Gs"], res["W"], R, S)
+    E = ole.make_local_energy(lp, 1.0, 1.0, 0.0, ext_field=bands.spiral_field(0.3, res["Q"]))
+    rng = np.random.default_rng(0)
+    vals = np.array([E(torch.tensor(rng.uniform(0, 8, (7, 2))), torch.tensor(rng.integers(0, 2, 7))) for _ in range(4)])
+    assert np.std(vals.real) < 1e-7 and abs(vals.real.mean() - res["E_gs"]) < 1e-7
+
+
+def _arch(**kw):
+    g = _rashba()
+    base = dict(embed_dim=16, depth=2, n_heads=4, n_freq=4)
+    base.update(kw)
+    return g, opf.Arch(5, g["Gs"], g["W"], 7.0, **base)
+
+
+def test_oracle_psiformer_zero_backflow_is_slater_and_antisymmetric():
+    g, a = _arch(n_dets=1)
+    th = opf.init_theta(a, 0)
+    rng = np.random.default_rng(3)
+    R, S = torch.tensor(rng.uniform(0, 7, (5, 2))), torch.tensor(rng.integers(0, 2, 5))
+    d = complex(opf.logpsi(a, th, R, S)) - complex(opf.slater_logpsi(g["Gs"], g["W"], R, S))
+    assert abs(d.real) < 1e-12 and abs((d.imag + np.pi) % (2 * np.pi) - np.pi) < 1e-12
+    g, a = _arch(n_dets=3, use_jastrow=True, learnable_env=True)
+    th = opf.init_theta(a, 1, backflow_scale=0.1, scale_all=0.05)
+    perm = torch.tensor([1, 0, 2, 3, 4])
+    r = np.exp(complex(opf.logpsi(a, th, R[perm], S[perm])) - complex(opf.logpsi(a, th, R, S)))
+    assert abs(r + 1.0) < 1e-10
+
+
+def test_oracle_library_ops_match_torch_definitions():
+    """LayerNorm / tanh-GELU / softmax restatements against torch.nn's implementations of the same
+    definitions (LayerNorm: biased variance; eps 1e-6)."""
+    x = torch.randn(7, 16, dtype=torch.float64)
+    p = {"n/scale": torch.randn(16, dtype=torch.float64), "n/bias": torch.randn(16, dtype=torch.float64)}
+    ref = torch.nn.functional.layer_norm(x, (16,), p["n/scale"], p["n/bias"], eps=1e-6)
+    assert torch.max(torch.abs(opf.layer_norm(x, p, "n") - ref)) < 1e-12
+    ref = torch.nn.functional.gelu(x, approximate="tanh")
+    assert torch.max(torch.abs(opf.gelu_tanh(x) - ref)) < 1e-14
+
+
+def test_oracle_backbone_permutation_equivariant():
+    g, a = _arch()
+    th = opf.init_theta(a, 2, scale_all=0.05)
+    p = opf.unravel(a, th)
+    rng = np.random.default_rng(4)
+    R, S = torch.tensor(rng.uniform(0, 7, (5, 2))), torch.tensor(rng.integers(0, 2, 5))
+    perm = torch.tensor([3, 0, 4, 1, 2])
+    x = opf.backbone(a, p, R, S)
+    xp = opf.backbone(a, p, R[perm], S[perm])
+    assert torch.max(torch.abs(x[perm] - xp)) < 1e-12
+
+
+def test_oracle_ewald_invariants():
+    L = 7.0
+    vals, zs = [], []
+    for c in (0.8, 1.0, 1.25):
+        ep, z, _ = oew.make_ewald(L, 1.0, ewald_alpha=c * math.sqrt(math.pi) / L, n_real=8, n_recip=14)
+        vals.append(ep([1.3, -2.1]))
+        zs.append(z)
+    assert np.ptp(vals) < 1e-7 and np.ptp(zs) < 1e-7
+    ep, zM, ce = oew.make_ewald(8.0, 1.0, n_real=6, n_recip=12)
+    R = np.random.default_rng(0).uniform(0, 8, (6, 2))
+    E = ce(R)
+    assert abs(E - ce(R + np.array([2.3, -1.1]))) < 1e-8
+    assert abs(ep([1.1, 2.7]) - ep([-1.1, -2.7])) < 1e-10
+    assert abs(ep([1e-4, 0.0]) * 1e-4 - 1.0) < 2e-2
+
+
+def test_oracle_minsr_equals_paramsr_and_O_matches_fd():
+    g, a = _arch(embed_dim=12, n_heads=3)
+    th = opf.init_theta(a, 0, backflow_scale=0.05)
+    rng = np.random.default_rng(4)
+    R = rng.uniform(0, 7, (6, 5, 2))
+    S = rng.integers(0, 2, (6, 5))
+    O = oms.per_sample_O(a, th, R, S)
+    eps = torch.tensor(rng.normal(size=6) + 1j * rng.normal(size=6))
+    for lam in (1e-2, 1e-4):
+        d1, d2 = oms.minsr_direction(O, eps, lam), oms.paramsr_direction(O, eps, lam)
+        assert float(torch.max(torch.abs(d1 - d2)) / torch.max(torch.abs(d2))) < 1e-6
+    # central finite differences on a few parameters
+    h = 1e-5
+    for k in (3, 500, th.shape[0] - 1):
+        e = torch.zeros_like(th)
+        e[k] = h
+        fp = complex(opf.logpsi(a, th + e, torch.tensor(R[0]), torch.tensor(S[0])))
+        fm = complex(opf.logpsi(a, th - e, torch.tensor(R[0]), torch.tensor(S[0])))
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - complex(O[0, k])) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_oracle_accept_rule():
+    """sampler.py:41-43: accept iff u < exp(min(0, d)); d = 2 Re L' - 2 Re L; NaN rejects."""
+    W, N = 4, 2
+    L0 = np.array([0.0, 0.0, 0.0, 0.0])
+    Lnew = np.array([0.1, -0.5, np.nan, -np.inf])
+    logpsi_b = lambda R, S: Lnew.astype(complex)
+    R = np.full((W, N, 2), 0.5)
+    S = np.zeros((W, N), int)
+    u = np.array([0.99, math.exp(-1.0) - 1e-12, 0.0, 0.0])
+    st = osa.make_streams(1, W, N, 0, 0)
+    st["xi"][:] = 0.0
+    _, _, acc, _, _ = osa.sweep(logpsi_b, R, S, np.eye(2), 0.1, st["xi"][0], u, st["flip_i"][0], st["u_flip"][0],
+                                lp_cur=2 * L0)
+    assert acc.tolist() == [True, True, False, False]
