@@ -1,0 +1,157 @@
+"""Benchmark of one VMC step of the continuum spinful Psiformer on one MI355X.
+
+Workload (the headline configuration): N = 16 electrons, rs = 3 (L = rs sqrt(pi N)), Rashba
+lambda = 0.8, Zeeman h_z = 0.05, d_model = 128, 4 layers, 8 heads, K = 8 determinants, 8
+Fourier harmonics (P = 567,696 real parameters), 8192 walkers; synthetic seeded weights
+(backflow ~ (1+i) N(0, 0.05^2)), walkers uniform in the cell then burnt in.  One step =
+10 Metropolis sweeps (1 joint move + 2 spin flips each) + E_loc (forward-Laplacian kinetic,
+Rashba SOC, Zeeman) + Ewald Coulomb + median/MAD clip + the per-walker gradient O (hand-written
+backward) + MinSR (Gram on MFMA, float64 Cholesky, one refinement step, push-through) + the
+parameter update.
+
+--precision mixed: float32 backbone, bf16 MFMA Gram with f32 accumulation, float64
+determinants / Ewald / Cholesky.  --precision fp64: everything float64 (the plain FP64 GPU
+path).  Prints one JSON line; --gpus > 1 is refused (this build measures one GPU).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "continuum-nqs_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np
+import torch
+
+METRIC = "walker-steps/s incl. MinSR at N=16, mixed vs FP64 GPU; |dE| vs FP64 oracle"
+N_E, RS, LAM, HZ = 16, 3.0, 0.8, 0.05
+ARCH = dict(embed_dim=128, depth=4, n_heads=8, n_freq=8, n_dets=8)
+PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12}
+
+
+def build_system(precision, W, seed=0):
+    from nqs.rashba_reference import noninteracting_gs, occupied_orbital_matrix, planewave_envelope
+    from nqs.psiformer import ContinuumSpinorPsiformer, ravel_params
+    L = RS * math.sqrt(math.pi * N_E)
+    g = noninteracting_gs(N_E, L, 1.0, LAM, HZ, n_cut=10, twist=(0.1, 0.17))
+    Gs, Wenv = planewave_envelope(*occupied_orbital_matrix(g))
+    model = ContinuumSpinorPsiformer(n_elec=N_E, Gs=Gs, W=Wenv, L=L, precision=precision, **ARCH)
+    params = model.init(seed)
+    rng = np.random.default_rng(seed + 1)
+    kr = params["params"]["backflow"]["kernel"]
+    z = torch.tensor(rng.standard_normal(tuple(kr.shape)) * 0.05)
+    params["params"]["backflow"]["kernel"] = (z + 1j * z).to(torch.complex128)
+    theta = ravel_params(params)
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 2)
+    R = torch.rand(W, N_E, 2, generator=gen, dtype=torch.float64, device=dev) * L
+    S = torch.randint(0, 2, (W, N_E), generator=gen, device=dev).to(torch.int32)
+    return dict(model=model, theta=theta, R=R, S=S, L=L, Gs=Gs, Wenv=Wenv)
+
+
+def cpu_baseline(sysd, n_walkers=8, seed=0):
+    """The oracle's step (torch float64 on the host cores) on a bounded sample of walkers."""
+    from oracle import psiformer as opf, ewald as oew, cpu_step
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    a = opf.Arch(N_E, sysd["Gs"], sysd["Wenv"], sysd["L"], **ARCH)
+    th = sysd["theta"].cpu()
+    _, _, ce = oew.make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
+    R = sysd["R"][:n_walkers].cpu().numpy()
+    S = sysd["S"][:n_walkers].cpu().numpy().astype(np.int64)
+    tm = {}
+    t0 = time.perf_counter()
+    cpu_step.vmc_step(a, th, R, S, lambda_r=LAM, h_z=HZ, coulomb=ce, n_sweeps=10, step=0.3 * RS, lam=1e-4,
+                      lr=0.05, seed=seed, timings=tm)
+    dt = time.perf_counter() - t0
+    return {"value": n_walkers / dt, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+            "sample": f"one full step of the float64 torch oracle on {n_walkers} of the walkers "
+                      f"(10 sweeps, dense-Hessian E_loc, reverse-mode O, LU MinSR); "
+                      f"stage seconds {json.dumps({k: round(v, 3) for k, v in tm.items()})}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walkers", type=int, default=8192)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--sweeps", type=int, default=10)
+    ap.add_argument("--burn", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-walkers", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: runs on several GPUs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    from nqs import _lib
+    from nqs.ewald import make_ewald
+    from nqs.minsr import VMCStep
+
+    W = args.walkers
+    sysd = build_system(args.precision, W)
+    _, _, coul = make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
+    gram = "bf16" if args.precision == "mixed" else "fp64"
+    vs = VMCStep(sysd["model"], sysd["theta"], sysd["R"], sysd["S"], m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul,
+                 lr=0.05, lam=1e-4, n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1)
+    if args.burn:
+        vs.chain.run(args.burn, 0.3 * RS, 2, key=7)
+    lib = _lib.load()
+    for k in range(args.warmup):
+        vs.step(key=100 + k)
+    torch.cuda.synchronize()
+    lib.nqs_minsr_profile(1, None)
+    vs.timers = []
+    gram_ms = []
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for k in range(args.steps):
+        vs.step(key=1000 + k)
+        ms5 = (ctypes.c_float * 5)()
+        _lib.check(lib.nqs_minsr_profile(-1, ms5), "nqs_minsr_profile")
+        gram_ms.append(list(ms5))
+    t1.record()
+    t1.synchronize()
+    lib.nqs_minsr_profile(0, None)
+    ms = t0.elapsed_time(t1) / args.steps
+    stages = np.array([[a.elapsed_time(b) for a, b in zip(t[:-1], t[1:])] for t in vs.timers]).mean(0)
+    mins = np.array(gram_ms).mean(0)
+    P = sysd["model"].n_params()
+    n = 2 * W
+    gram_flops = float(n) * (n + 1) * P          # n(n+1)/2 outputs of the symmetric rank-P update, 2P flops each
+    achieved = gram_flops / (mins[1] / 1e3)
+    peak = PEAK["bf16" if gram == "bf16" else "fp64"]
+    stats = vs.last["stats"].cpu().numpy()
+    out = {"metric": METRIC, "value": W / (ms / 1000.0), "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+bf16+f64" if args.precision == "mixed" else "f64",
+           "data": "synthetic",
+           "config": {"workload": "N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers=8192",
+                      "walkers": W, "params": P, "sweeps": args.sweeps, "precision": args.precision, "gram": gram},
+           "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
+                        "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / peak, "traffic": None, "gram_ms": float(mins[1])},
+           "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
+                        "minsr": float(stages[3]), "update_repack": float(stages[4]),
+                        "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
+                        "minsr_cholesky": float(mins[2]), "minsr_solve_refine": float(mins[3]),
+                        "minsr_pushthrough": float(mins[4])},
+           "E_mean": float(stats[0]), "E_err": float(stats[1])}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sysd, args.cpu_walkers)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

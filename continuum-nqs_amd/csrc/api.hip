@@ -3,6 +3,7 @@
 #include <cstring>
 #include "model.h"
 #include "physics.h"
+#include "minsr.h"
 
 namespace nqs {
 
@@ -11,11 +12,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int pack_params(const Model& md, const double* theta, const double* gs, const double* wenv, void* packed,
                 hipStream_t st);
 
-#define NQS_TRY(x)                 \
-  do {                             \
-    int _rc = (x);                 \
-    if (_rc != NQS_OK) return _rc; \
-  } while (0)
 
 // Rows of the jet activations processed per launch sequence: bounds the workspace while
 // keeping every launch far above the 256-CU occupancy threshold.
@@ -126,6 +122,12 @@ static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void*
   return launch_eloc_assemble(a, st);
 }
 
+static int64_t grad_chunk(const Model& md, int64_t W) {
+  const int64_t per = std::max<int64_t>(1, grad_bytes(md, 1));
+  const int64_t c = std::max<int64_t>(1, (8LL << 30) / per);
+  return std::min<int64_t>(c, std::max<int64_t>(W, 1));
+}
+
 struct McmcPlan {
   int64_t cw, fwd;
 };
@@ -173,6 +175,119 @@ static int mcmc_impl(const Model& md, const void* packed, double* R, int32_t* S,
       NQS_TRY(launch_mcmc_accept_spin(W, N, S, fif, logp, lpn, u_flip + ((int64_t)s * nf + f) * W, acc_spin, st));
     }
   }
+  return NQS_OK;
+}
+
+struct MinsrWs {
+  double* mean;
+  __bf16* Ob;
+  float* T32;
+  double* A64;
+  double* Linv;
+  double* panel;
+  double *et, *y, *r, *u, *v, *ts, *sc;
+  int64_t bytes;
+};
+static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
+  MinsrWs w;
+  const int64_t n = 2 * d->n_walkers;
+  char* p = base;
+  auto grab = [&](int64_t b) {
+    char* r = p;
+    p += al(b);
+    return r;
+  };
+  w.mean = (double*)grab(2 * d->n_params * 8);
+  w.Ob = d->gram == NQS_GRAM_BF16 ? (__bf16*)grab(n * d->ldo * 2) : nullptr;
+  w.T32 = d->gram != NQS_GRAM_FP64 ? (float*)grab(n * n * 4) : nullptr;
+  w.A64 = (double*)grab(n * n * 8);
+  w.Linv = (double*)grab(n * 64 * 8);
+  w.panel = (double*)grab(n * 64 * 8);
+  w.et = (double*)grab(n * 8);
+  w.y = (double*)grab(n * 8);
+  w.r = (double*)grab(n * 8);
+  w.u = (double*)grab(d->ldo * 8);
+  w.v = (double*)grab(n * 8);
+  w.ts = (double*)grab(2 * n * 8);
+  w.sc = (double*)grab(8 * 8);
+  w.bytes = p - base;
+  return w;
+}
+
+// Optional stage timing of nqs_minsr_direction: hipEvents recorded on the call's stream
+// between centre | Gram | Cholesky | solve + refinement | push-through.
+static bool g_minsr_profile = false;
+static hipEvent_t g_minsr_ev[6];
+static bool g_minsr_ev_ok = false;
+static int minsr_mark(int i, hipStream_t st) {
+  if (!g_minsr_profile) return NQS_OK;
+  if (!g_minsr_ev_ok) {
+    for (int k = 0; k < 6; ++k) NQS_CHECK_HIP(hipEventCreate(&g_minsr_ev[k]));
+    g_minsr_ev_ok = true;
+  }
+  NQS_CHECK_HIP(hipEventRecord(g_minsr_ev[i], st));
+  return NQS_OK;
+}
+
+template <typename T>
+static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* dth, int32_t* info, char* ws,
+                      hipStream_t st) {
+  const int64_t W = d->n_walkers, n = 2 * W, P = d->n_params, ldo = d->ldo;
+  MinsrWs w = minsr_ws(d, ws);
+  NQS_TRY(minsr_mark(0, st));
+  NQS_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), st));
+  NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, st));
+  NQS_TRY(make_etil(eps, W, w.et, st));
+  NQS_TRY(minsr_mark(1, st));
+  if (d->gram == NQS_GRAM_BF16) {
+    NQS_TRY(gram_bf16(w.Ob, ldo, n, ldo, w.T32, n, st));
+    NQS_TRY(to_f64_shift<float>(w.T32, n, n, 0.0, w.A64, n, st));
+  } else {
+    GemmArgs<T> g;
+    g.A = O;
+    g.B = O;
+    g.M = n;
+    g.N = n;
+    g.K = P;
+    g.lda = g.ldb = ldo;
+    g.strideA = g.strideB = g.strideC = 0;
+    g.alpha = (T)1;
+    g.beta = (T)0;
+    g.bias = nullptr;
+    g.bias_period = 1;
+    g.bias_len = 0;
+    g.tri = 1;
+    g.mirror = 0;
+    if (sizeof(T) == 8) {
+      g.C = (T*)w.A64;
+      g.ldc = n;
+      NQS_TRY(gemm_nt<T>(g, 1, st));
+      
+    } else {
+      g.C = (T*)w.T32;
+      g.ldc = n;
+      NQS_TRY(gemm_nt<T>(g, 1, st));
+      NQS_TRY(to_f64_shift<float>(w.T32, n, n, 0.0, w.A64, n, st));
+    }
+  }
+  NQS_TRY(deflate_centring(w.A64, n, W, d->lam, w.ts, w.sc, st));
+  NQS_TRY(minsr_mark(2, st));
+  NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st));
+  NQS_TRY(minsr_mark(3, st));
+  NQS_CHECK_HIP(hipMemcpyAsync(w.y, w.et, n * 8, hipMemcpyDeviceToDevice, st));
+  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.y, st));
+  for (int it = 0; it < d->refine; ++it) {
+    // r = etil - Otil (Otil^T y) - lam y ;  y += (L L^T)^-1 r
+    NQS_TRY(gemv_cols<T>(O, ldo, n, P, w.y, w.u, st));
+    NQS_TRY(gemv_rows<T>(O, ldo, n, P, w.u, w.v, 1.0, 0.0, 0, st));
+    NQS_TRY(axpby(n, 1.0, w.et, -1.0, w.v, w.r, st));
+    NQS_TRY(axpby(n, 1.0, w.r, -d->lam, w.y, w.r, st));
+    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.r, st));
+    NQS_TRY(axpby(n, 1.0, w.y, 1.0, w.r, w.y, st));
+  }
+  NQS_TRY(minsr_mark(4, st));
+  NQS_TRY(gemv_cols<T>(O, ldo, n, P, w.y, dth, st));
+  NQS_TRY(minsr_mark(5, st));
   return NQS_OK;
 }
 
@@ -232,6 +347,8 @@ int64_t nqs_workspace_bytes(const nqs_model_desc* m, int32_t op, int64_t W) {
       ElocPlan p = eloc_plan(md, W);
       return al(p.fwd) + p.extra;
     }
+    case NQS_OP_GRAD:
+      return grad_bytes(md, grad_chunk(md, W));
     case NQS_OP_MCMC: {
       McmcPlan p = mcmc_plan(md, W);
       return al(p.fwd) + al(W * md.N * 16) + al(W * md.N * 4) + al(W * 16);
@@ -271,6 +388,65 @@ int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void*
   if (md.tsize == 8)
     return eloc_impl<double>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
   return eloc_impl<float>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
+}
+
+int nqs_psiformer_grad(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,
+                       void* O, int64_t ldo, double* logpsi, void* ws, int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && O && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_REQUIRE(ldo >= md.P, NQS_ERR_ARG, "ldo smaller than the parameter count");
+  NQS_TRY(check_ws(m, NQS_OP_GRAD, W, ws_bytes));
+  const int64_t cw = grad_chunk(md, W);
+  for (int64_t w0 = 0; w0 < W; w0 += cw) {
+    const int64_t n = std::min(cw, W - w0);
+    double* lp = logpsi ? logpsi + 2 * w0 : nullptr;
+    if (md.tsize == 8)
+      NQS_TRY(run_backward<double>(md, packed, R, S, w0, n, W, (double*)O, ldo, lp, (char*)ws, (hipStream_t)stream));
+    else
+      NQS_TRY(run_backward<float>(md, packed, R, S, w0, n, W, (float*)O, ldo, lp, (char*)ws, (hipStream_t)stream));
+  }
+  return NQS_OK;
+}
+
+int64_t nqs_minsr_workspace_bytes(const nqs_minsr_desc* d) {
+  if (!d || d->n_walkers <= 0 || d->n_params <= 0 || d->ldo < d->n_params) {
+    set_error("bad MinSR descriptor");
+    return -1;
+  }
+  return minsr_ws(d, nullptr).bytes;
+}
+
+int nqs_minsr_direction(const nqs_minsr_desc* d, void* O, const double* eps, double* dth, int32_t* info, void* ws,
+                        int64_t ws_bytes, void* stream) {
+  NQS_REQUIRE(d && O && eps && dth && info && ws, NQS_ERR_ARG, "null pointer");
+  const int64_t need = nqs_minsr_workspace_bytes(d);
+  NQS_REQUIRE(need >= 0, NQS_ERR_ARG, "bad MinSR descriptor");
+  NQS_REQUIRE(ws_bytes >= need, NQS_ERR_WORKSPACE, "MinSR workspace too small: need " + std::to_string(need));
+  NQS_REQUIRE(d->ldo % 64 == 0, NQS_ERR_ARG, "ldo must be a multiple of 64");
+  NQS_REQUIRE((2 * d->n_walkers) % 64 == 0, NQS_ERR_UNSUPPORTED, "2W must be a multiple of 64");
+  NQS_REQUIRE(d->o_dtype == NQS_FP64 || d->o_dtype == NQS_MIXED, NQS_ERR_ARG, "bad O dtype");
+  NQS_REQUIRE(!(d->gram == NQS_GRAM_FP64 && d->o_dtype != NQS_FP64), NQS_ERR_ARG, "a float64 Gram needs a float64 O");
+  NQS_REQUIRE(!(d->gram == NQS_GRAM_FP32 && d->o_dtype != NQS_MIXED), NQS_ERR_ARG, "a float32 Gram needs a float32 O");
+  NQS_REQUIRE(d->gram >= 0 && d->gram <= 2 && d->refine >= 0, NQS_ERR_ARG, "bad Gram mode");
+  hipStream_t st = (hipStream_t)stream;
+  if (d->o_dtype == NQS_FP64) return minsr_impl<double>(d, (double*)O, eps, dth, info, (char*)ws, st);
+  return minsr_impl<float>(d, (float*)O, eps, dth, info, (char*)ws, st);
+}
+
+int nqs_minsr_profile(int32_t enable, float* ms_out) {
+  if (enable >= 0) g_minsr_profile = enable != 0;
+  if (ms_out) {
+    NQS_REQUIRE(g_minsr_ev_ok, NQS_ERR_ARG, "no profiled MinSR call yet");
+    NQS_CHECK_HIP(hipEventSynchronize(g_minsr_ev[5]));
+    for (int k = 0; k < 5; ++k) NQS_CHECK_HIP(hipEventElapsedTime(&ms_out[k], g_minsr_ev[k], g_minsr_ev[k + 1]));
+  }
+  return NQS_OK;
+}
+
+int nqs_clip_energies(const double* e, int64_t W, double clip_k, double* e_clip, double* stats, void* stream) {
+  NQS_REQUIRE(e && e_clip && stats, NQS_ERR_ARG, "null pointer");
+  return clip_stats(e, W, clip_k, e_clip, stats, (hipStream_t)stream);
 }
 
 int nqs_ewald(const nqs_ewald_desc* e, const double* R, int64_t W, int32_t N, double* out, void* stream) {

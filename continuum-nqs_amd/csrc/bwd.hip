@@ -1,0 +1,318 @@
+// Backward kernels for the per-walker parameter gradient O[w, k] = d log psi_w / d theta_k
+// (src/nqs/minsr.py:56-65: grad of Re and Im of log psi with respect to the real ravel theta).
+//
+// log psi is complex and the backbone real, so every cotangent of a backbone activation is
+// complex: two real channels (d Re L, d Im L) laid out [cfg][channel][token][feature].  Linear
+// layers pass cotangents through the NT GEMM (gemm.hip) against the untransposed weights; the
+// kernels here cover the determinant head, LayerNorm, GELU, attention (recomputing the softmax
+// from the stored Q, K rather than keeping it per walker) and the per-walker weight / bias
+// gradients, which are written straight into the rows of the O matrix:
+//   row w            = Re O_w,      row W_total + w = Im O_w,      column = real_ravel index.
+#include "nqs_common.h"
+#include "kernels.h"
+#include "bwd.h"
+
+namespace nqs {
+
+// ------------------------------------------------------------------ per-walker weight gradient
+// O[c][w][seg(j)] (i, j) = sum_t X[w][t][i] * G[w][c][t][j]   (X: value activations, G: cotangents)
+// Output column j belongs to segment s = j / seg_w with its own theta base; inside a segment the
+// leaf is (in, seg_w) row-major.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs<T> a) {
+  extern __shared__ char smem_raw[];
+  T* Xs = reinterpret_cast<T*>(smem_raw);  // N x IB
+  T* Gs = Xs + a.N * 16;                   // N x out
+  const int IB = 16;
+  const int64_t cfg = blockIdx.y >> 1;
+  const int c = blockIdx.y & 1;
+  const int i0 = blockIdx.x * IB;
+  const int tid = threadIdx.x;
+  const int N = a.N;
+  const T* X = a.X + cfg * (int64_t)N * a.ldx;
+  const T* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg;
+  for (int e = tid; e < N * IB; e += blockDim.x) {
+    const int t = e / IB, ii = e % IB;
+    Xs[e] = (i0 + ii < a.in) ? X[(int64_t)t * a.ldx + i0 + ii] : (T)0;
+  }
+  for (int e = tid; e < N * a.out; e += blockDim.x) {
+    const int t = e / a.out, j = e % a.out;
+    Gs[e] = G[(int64_t)t * a.ldg + j];
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
+  T* O = a.O + row * a.ldo;
+  const int nrow = min(IB, a.in - i0);
+  for (int e = tid; e < nrow * a.out; e += blockDim.x) {
+    const int ii = e / a.out, j = e % a.out;
+    T s = 0;
+    for (int t = 0; t < N; ++t) s += Xs[t * IB + ii] * Gs[t * a.out + j];
+    const int sg = j / a.seg_w, jj = j % a.seg_w;
+    O[a.seg_base[sg] + (int64_t)(i0 + ii) * a.seg_w + jj] = s;
+  }
+}
+
+template <typename T>
+int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
+  if (a.n_cfg == 0) return NQS_OK;
+  const size_t lds = sizeof(T) * (size_t)a.N * (16 + a.out);
+  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "weight-gradient tile exceeds LDS");
+  NQS_REQUIRE(a.out % a.seg_w == 0 && a.out / a.seg_w <= 3, NQS_ERR_ARG, "bad output segmentation");
+  hipLaunchKernelGGL(k_wgrad<T>, dim3((unsigned)cdiv(a.in, 16), (unsigned)(a.n_cfg * 2)), dim3(256), lds, st, a);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// bias gradient: O[c][w][seg(j)] = sum_t G[w][c][t][j]
+template <typename T>
+__global__ void k_bgrad(WgradArgs<T> a) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n_cfg * 2 * a.out) return;
+  const int j = (int)(q % a.out);
+  const int64_t cc = q / a.out;  // cfg * 2 + c
+  const int64_t cfg = cc >> 1;
+  const int c = (int)(cc & 1);
+  const T* G = a.G + (cc * a.N) * a.ldg;
+  T s = 0;
+  for (int t = 0; t < a.N; ++t) s += G[(int64_t)t * a.ldg + j];
+  const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
+  const int sg = j / a.seg_w, jj = j % a.seg_w;
+  a.O[row * a.ldo + a.seg_base[sg] + jj] = s;
+}
+
+template <typename T>
+int launch_bgrad(const WgradArgs<T>& a, hipStream_t st) {
+  const int64_t n = a.n_cfg * 2 * a.out;
+  if (n == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_bgrad<T>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, a);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ LayerNorm backward
+// y = (x - mu) rs s + b.  For each cotangent channel: g = ybar s,
+// xbar = rs (g - mean(g) - xhat mean(g xhat));  O[s] = sum_t ybar xhat;  O[b] = sum_t ybar.
+// One wave per configuration, looping over its tokens; xbar is added to xb (accumulate = 1) or
+// stored (accumulate = 0).
+template <typename T, int MAXF>
+__global__ __launch_bounds__(64) void k_ln_bwd(LnBwdArgs<T> a) {
+  const int64_t cfg = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int d = a.d, N = a.N;
+  const T inv_d = (T)1 / (T)d;
+  T ds[2][MAXF], db[2][MAXF], sc[MAXF];
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    const int f = lane + 64 * q;
+    sc[q] = (f < d) ? a.scale[f] : (T)0;
+    ds[0][q] = ds[1][q] = db[0][q] = db[1][q] = 0;
+  }
+  for (int t = 0; t < N; ++t) {
+    const T* x = a.x + (cfg * N + t) * (int64_t)d;
+    T xv[MAXF];
+    T s1 = 0, s2 = 0;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      xv[q] = (f < d) ? x[f] : (T)0;
+      s1 += xv[q];
+      s2 += xv[q] * xv[q];
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    const T mu = s1 * inv_d;
+    T var = s2 * inv_d - mu * mu;
+    var = var > (T)0 ? var : (T)0;
+    const T rs = (T)1 / sqrt(var + (T)1e-6);
+    T xh[MAXF];
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) xh[q] = (xv[q] - mu) * rs;
+    for (int c = 0; c < 2; ++c) {
+      const T* yb = a.ybar + ((cfg * 2 + c) * N + t) * (int64_t)d;
+      T* xb = a.xbar + ((cfg * 2 + c) * N + t) * (int64_t)d;
+      T yv[MAXF], g1 = 0, g2 = 0;
+#pragma unroll
+      for (int q = 0; q < MAXF; ++q) {
+        const int f = lane + 64 * q;
+        yv[q] = (f < d) ? yb[f] : (T)0;
+        const T g = yv[q] * sc[q];
+        g1 += g;
+        g2 += g * xh[q];
+        ds[c][q] += yv[q] * xh[q];
+        db[c][q] += yv[q];
+      }
+      g1 = wave_sum(g1) * inv_d;
+      g2 = wave_sum(g2) * inv_d;
+#pragma unroll
+      for (int q = 0; q < MAXF; ++q) {
+        const int f = lane + 64 * q;
+        if (f < d) {
+          const T v = rs * (yv[q] * sc[q] - g1 - xh[q] * g2);
+          xb[f] = a.accumulate ? xb[f] + v : v;
+        }
+      }
+    }
+  }
+  for (int c = 0; c < 2; ++c) {
+    T* O = a.O + ((int64_t)c * a.W_total + a.w0 + cfg) * a.ldo;
+#pragma unroll
+    for (int q = 0; q < MAXF; ++q) {
+      const int f = lane + 64 * q;
+      if (f < d) {
+        O[a.off_scale + f] = ds[c][q];
+        O[a.off_bias + f] = db[c][q];
+      }
+    }
+  }
+}
+
+template <typename T>
+int launch_ln_bwd(const LnBwdArgs<T>& a, hipStream_t st) {
+  if (a.n_cfg == 0) return NQS_OK;
+  if (a.d <= 128)
+    hipLaunchKernelGGL((k_ln_bwd<T, 2>), dim3((unsigned)a.n_cfg), dim3(64), 0, st, a);
+  else if (a.d <= 256)
+    hipLaunchKernelGGL((k_ln_bwd<T, 4>), dim3((unsigned)a.n_cfg), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_ln_bwd<T, 8>), dim3((unsigned)a.n_cfg), dim3(64), 0, st, a);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ GELU backward: ubar = gbar * gelu'(u)
+template <typename T>
+__global__ void k_gelu_bwd(int64_t n_cfg, int64_t per_cfg, const T* __restrict__ u, const T* __restrict__ gbar,
+                           T* __restrict__ ubar) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_cfg * 2 * per_cfg) return;
+  const int64_t cc = q / per_cfg, r = q % per_cfg;
+  const T x = u[(cc >> 1) * per_cfg + r];
+  const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
+  const T th = tanh(k0 * (x + k1 * x * x * x));
+  const T g1 = (T)0.5 * ((T)1 + th) + (T)0.5 * x * ((T)1 - th * th) * k0 * ((T)1 + (T)3 * k1 * x * x);
+  ubar[q] = gbar[q] * g1;
+}
+
+template <typename T>
+int launch_gelu_bwd(int64_t n_cfg, int64_t per_cfg, const T* u, const T* gbar, T* ubar, hipStream_t st) {
+  const int64_t n = n_cfg * 2 * per_cfg;
+  if (n == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_gelu_bwd<T>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, n_cfg, per_cfg, u, gbar, ubar);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ attention backward (per cfg, head)
+// Recomputes P = softmax(Q K^T sc) from the stored qkv, then per cotangent channel:
+//   Vbar = P^T Obar,  Pbar = Obar V^T,  Sbar = P (Pbar - rowdot(Pbar, P)),
+//   Qbar = sc Sbar K,  Kbar = sc Sbar^T Q.
+template <typename T>
+__global__ __launch_bounds__(256) void k_attn_bwd(int64_t n_cfg, int N, int d, int nh, const T* __restrict__ qkv,
+                                                  const T* __restrict__ obar, T* __restrict__ qkvbar) {
+  extern __shared__ char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int hd = d / nh, NN = N * N, NH = N * hd;
+  T* Q = sm;
+  T* K = Q + NH;
+  T* V = K + NH;
+  T* Ob = V + NH;
+  T* P = Ob + NH;
+  T* Sb = P + NN;
+  T* rd = Sb + NN;  // N
+  const int64_t cfg = blockIdx.x / nh;
+  const int h = blockIdx.x % nh;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int64_t rs = 3LL * d;
+  const T* in = qkv + cfg * (int64_t)N * rs;
+  const T sc = (T)1 / sqrt((T)hd);
+  for (int e = tid; e < NH; e += nthr) {
+    const int t = e / hd, q = e % hd;
+    Q[e] = in[t * rs + h * hd + q];
+    K[e] = in[t * rs + d + h * hd + q];
+    V[e] = in[t * rs + 2 * d + h * hd + q];
+  }
+  __syncthreads();
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N, b = e % N;
+    T s = 0;
+    for (int q = 0; q < hd; ++q) s += Q[a * hd + q] * K[b * hd + q];
+    P[e] = s * sc;
+  }
+  __syncthreads();
+  for (int a = tid; a < N; a += nthr) {
+    T mx = P[a * N];
+    for (int b = 1; b < N; ++b) mx = P[a * N + b] > mx ? P[a * N + b] : mx;
+    T z = 0;
+    for (int b = 0; b < N; ++b) {
+      const T e_ = exp(P[a * N + b] - mx);
+      P[a * N + b] = e_;
+      z += e_;
+    }
+    for (int b = 0; b < N; ++b) P[a * N + b] /= z;
+  }
+  __syncthreads();
+  for (int c = 0; c < 2; ++c) {
+    const T* ob = obar + ((cfg * 2 + c) * N) * (int64_t)d;
+    T* out = qkvbar + ((cfg * 2 + c) * N) * rs;
+    for (int e = tid; e < NH; e += nthr) {
+      const int t = e / hd, q = e % hd;
+      Ob[e] = ob[t * d + h * hd + q];
+    }
+    __syncthreads();
+    // Vbar and Pbar
+    for (int e = tid; e < NH; e += nthr) {
+      const int b = e / hd, q = e % hd;
+      T s = 0;
+      for (int a = 0; a < N; ++a) s += P[a * N + b] * Ob[a * hd + q];
+      out[b * rs + 2 * d + h * hd + q] = s;
+    }
+    for (int e = tid; e < NN; e += nthr) {
+      const int a = e / N, b = e % N;
+      T s = 0;
+      for (int q = 0; q < hd; ++q) s += Ob[a * hd + q] * V[b * hd + q];
+      Sb[e] = s;
+    }
+    __syncthreads();
+    for (int a = tid; a < N; a += nthr) {
+      T s = 0;
+      for (int b = 0; b < N; ++b) s += Sb[a * N + b] * P[a * N + b];
+      rd[a] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < NN; e += nthr) Sb[e] = P[e] * (Sb[e] - rd[e / N]) * sc;
+    __syncthreads();
+    for (int e = tid; e < NH; e += nthr) {
+      const int a = e / hd, q = e % hd;
+      T sq = 0, sk = 0;
+      for (int b = 0; b < N; ++b) {
+        sq += Sb[a * N + b] * K[b * hd + q];
+        sk += Sb[b * N + a] * Q[b * hd + q];
+      }
+      out[a * rs + h * hd + q] = sq;
+      out[a * rs + d + h * hd + q] = sk;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+int launch_attn_bwd(int64_t n_cfg, int N, int d, int nh, const T* qkv, const T* obar, T* qkvbar, hipStream_t st) {
+  if (n_cfg == 0) return NQS_OK;
+  const int hd = d / nh;
+  const size_t lds = sizeof(T) * (size_t)(4 * N * hd + 2 * N * N + N);
+  hipLaunchKernelGGL(k_attn_bwd<T>, dim3((unsigned)(n_cfg * nh)), dim3(256), lds, st, n_cfg, N, d, nh, qkv, obar,
+                     qkvbar);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+#define NQS_INST_B(T)                                                                                          \
+  template int launch_wgrad<T>(const WgradArgs<T>&, hipStream_t);                                            \
+  template int launch_bgrad<T>(const WgradArgs<T>&, hipStream_t);                                            \
+  template int launch_ln_bwd<T>(const LnBwdArgs<T>&, hipStream_t);                                           \
+  template int launch_gelu_bwd<T>(int64_t, int64_t, const T*, const T*, T*, hipStream_t);                    \
+  template int launch_attn_bwd<T>(int64_t, int, int, int, const T*, const T*, T*, hipStream_t);
+NQS_INST_B(double)
+NQS_INST_B(float)
+
+}  // namespace nqs

@@ -12,6 +12,7 @@
 // electron that tangent c moves; Lap M = Lap env (1+bf) + env Lap bf + 2 sum_c denv_c dbf_c.
 #include "nqs_common.h"
 #include "kernels.h"
+#include "bwd.h"
 
 namespace nqs {
 
@@ -33,6 +34,95 @@ __device__ inline void block_sum2(double& a, double& b, double* red) {
     b += red[2 * w + 1];
   }
   __syncthreads();
+}
+
+
+// LU with partial pivoting of the N x N complex matrix M (LDS, row-major) by the whole
+// workgroup; pivot = largest |re|+|im| in the column (LAPACK getrf's choice).  piv[q] = row
+// swapped with q; flag[0] = parity of the swaps, flag[1] = 1 if a pivot was exactly zero.
+__device__ void wg_lu(cd* M, int N, int* piv, int* flag) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (tid == 0) {
+    flag[0] = 0;
+    flag[1] = 0;
+  }
+  __syncthreads();
+  for (int q = 0; q < N; ++q) {
+    if (tid == 0) {
+      int best = q;
+      double bv = fabs(M[q * N + q].re) + fabs(M[q * N + q].im);
+      for (int r = q + 1; r < N; ++r) {
+        const double v = fabs(M[r * N + q].re) + fabs(M[r * N + q].im);
+        if (v > bv) {
+          bv = v;
+          best = r;
+        }
+      }
+      piv[q] = best;
+      if (best != q) flag[0] ^= 1;
+      if (bv == 0.0) flag[1] = 1;
+    }
+    __syncthreads();
+    const int pr = piv[q];
+    if (pr != q)
+      for (int c = tid; c < N; c += nthr) {
+        const cd t = M[q * N + c];
+        M[q * N + c] = M[pr * N + c];
+        M[pr * N + c] = t;
+      }
+    __syncthreads();
+    const cd pv = M[q * N + q];
+    const bool zero = (pv.re == 0.0 && pv.im == 0.0);
+    for (int r = q + 1 + tid; r < N; r += nthr) M[r * N + q] = zero ? mk(0, 0) : cdiv_(M[r * N + q], pv);
+    __syncthreads();
+    const int w_ = N - q - 1;
+    for (int e = tid; e < w_ * w_; e += nthr) {
+      const int r = q + 1 + e / w_, c = q + 1 + e % w_;
+      M[r * N + c] = M[r * N + c] - M[r * N + q] * M[q * N + c];
+    }
+    __syncthreads();
+  }
+}
+
+// Mi = U^-1 L^-1 P from the LU factors in M; column c computed by thread c.  Ends synced.
+__device__ void wg_inverse(const cd* M, cd* Mi, int N, const int* piv) {
+  for (int c = threadIdx.x; c < N; c += blockDim.x) {
+    for (int r = 0; r < N; ++r) Mi[r * N + c] = mk(r == c ? 1.0 : 0.0, 0);
+    for (int q = 0; q < N; ++q) {
+      const int pr = piv[q];
+      if (pr != q) {
+        const cd t = Mi[q * N + c];
+        Mi[q * N + c] = Mi[pr * N + c];
+        Mi[pr * N + c] = t;
+      }
+    }
+    for (int r = 1; r < N; ++r) {
+      cd s = Mi[r * N + c];
+      for (int q = 0; q < r; ++q) s = s - M[r * N + q] * Mi[q * N + c];
+      Mi[r * N + c] = s;
+    }
+    for (int r = N - 1; r >= 0; --r) {
+      cd s = Mi[r * N + c];
+      for (int q = r + 1; q < N; ++q) s = s - M[r * N + q] * Mi[q * N + c];
+      Mi[r * N + c] = cdiv_(s, M[r * N + r]);
+    }
+  }
+  __syncthreads();
+}
+
+// log det from LU factors: sum log|u_qq| + log(prod u_qq/|u_qq| * parity) (principal branch).
+__device__ inline cd lu_logdet(const cd* M, int N, int parity) {
+  double la = 0, sre = parity ? -1.0 : 1.0, sim = 0;
+  for (int q = 0; q < N; ++q) {
+    const cd u = M[q * N + q];
+    const double au = cabs_(u);
+    la += log(au);
+    const double ur = au > 0 ? u.re / au : 0.0, ui = au > 0 ? u.im / au : 0.0;
+    const double nr = sre * ur - sim * ui, ni = sre * ui + sim * ur;
+    sre = nr;
+    sim = ni;
+  }
+  return mk(la + log(hypot(sre, sim)), atan2(sim, sre));
 }
 
 // Jastrow pair function u(r) and its first two radial derivatives.
@@ -143,90 +233,14 @@ __global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const d
       const int a = e / N, i = e % N;
       M[e] = env[e] * (mk(1, 0) + bfval(0, k, a, i));
     }
-    if (tid == 0) {
-      flag[0] = 0;
-      flag[1] = 0;
-    }
     __syncthreads();
-    // ---- LU with partial pivoting, in place
-    for (int q = 0; q < N; ++q) {
-      if (tid == 0) {
-        int best = q;
-        double bv = fabs(M[q * N + q].re) + fabs(M[q * N + q].im);
-        for (int r = q + 1; r < N; ++r) {
-          const double v = fabs(M[r * N + q].re) + fabs(M[r * N + q].im);
-          if (v > bv) {
-            bv = v;
-            best = r;
-          }
-        }
-        piv[q] = best;
-        if (best != q) flag[0] ^= 1;
-        if (bv == 0.0) flag[1] = 1;
-      }
-      __syncthreads();
-      const int pr = piv[q];
-      if (pr != q)
-        for (int c = tid; c < N; c += nthr) {
-          const cd t = M[q * N + c];
-          M[q * N + c] = M[pr * N + c];
-          M[pr * N + c] = t;
-        }
-      __syncthreads();
-      const cd pv = M[q * N + q];
-      const bool zero = (pv.re == 0.0 && pv.im == 0.0);
-      for (int r = q + 1 + tid; r < N; r += nthr) M[r * N + q] = zero ? mk(0, 0) : cdiv_(M[r * N + q], pv);
-      __syncthreads();
-      const int w_ = N - q - 1;
-      for (int e = tid; e < w_ * w_; e += nthr) {
-        const int r = q + 1 + e / w_, c = q + 1 + e % w_;
-        M[r * N + c] = M[r * N + c] - M[r * N + q] * M[q * N + c];
-      }
-      __syncthreads();
-    }
-    // ---- log det
-    if (tid == 0) {
-      double la = 0, sre = (flag[0] ? -1.0 : 1.0), sim = 0;
-      for (int q = 0; q < N; ++q) {
-        const cd u = M[q * N + q];
-        const double au = cabs_(u);
-        la += log(au);
-        const double ur = au > 0 ? u.re / au : 0.0, ui = au > 0 ? u.im / au : 0.0;
-        const double nr = sre * ur - sim * ui, ni = sre * ui + sim * ur;
-        sre = nr;
-        sim = ni;
-      }
-      // log(sign): principal log of a unit complex number (or log 0 = -inf)
-      const double ls = log(hypot(sre, sim));
-      res[(size_t)k * (nt + 2)] = mk(la + ls, atan2(sim, sre));
-    }
+    wg_lu(M, N, piv, flag);
+    if (tid == 0) res[(size_t)k * (nt + 2)] = lu_logdet(M, N, flag[0]);
     if (nt == 0 && !j.has_lap) {
       __syncthreads();
       continue;
     }
-    // ---- explicit inverse Mi = U^-1 L^-1 P, column c by thread c
-    for (int c = tid; c < N; c += nthr) {
-      for (int r = 0; r < N; ++r) Mi[r * N + c] = mk(r == c ? 1.0 : 0.0, 0);
-      for (int q = 0; q < N; ++q) {
-        const int pr = piv[q];
-        if (pr != q) {
-          const cd t = Mi[q * N + c];
-          Mi[q * N + c] = Mi[pr * N + c];
-          Mi[pr * N + c] = t;
-        }
-      }
-      for (int r = 1; r < N; ++r) {
-        cd s = Mi[r * N + c];
-        for (int q = 0; q < r; ++q) s = s - M[r * N + q] * Mi[q * N + c];
-        Mi[r * N + c] = s;
-      }
-      for (int r = N - 1; r >= 0; --r) {
-        cd s = Mi[r * N + c];
-        for (int q = r + 1; q < N; ++q) s = s - M[r * N + q] * Mi[q * N + c];
-        Mi[r * N + c] = cdiv_(s, M[r * N + r]);
-      }
-    }
-    __syncthreads();
+    wg_inverse(M, Mi, N, piv);
     cd hsum = mk(0, 0);
     for (int c = 1; c <= nt; ++c) {
       int ec, bc;
@@ -406,5 +420,215 @@ template int launch_det_jet<double>(const JetGeom&, const DetParams&, const doub
                                     const double*, double*, double*, double*, hipStream_t);
 template int launch_det_jet<float>(const JetGeom&, const DetParams&, const double*, const int32_t*, int64_t,
                                    const float*, double*, double*, double*, hipStream_t);
+
+
+// ------------------------------------------------------------------ determinant-head backward
+// dL/dM_k[a,i] = p_k Minv_k[i,a] (p_k the log-sum-exp weights, 1 for K = 1), hence
+//   dL/dbf[k,a,i] = p_k env[a,i] Minv_k[i,a],   dL/denv[a,i] = sum_k p_k (1 + bf[k,a,i]) Minv_k[i,a].
+// The backflow Dense has 2KN real outputs y = [Re bf | Im bf]; its complex cotangent is
+// [dL/dbf, i dL/dbf], stored as two real channels (Re, Im).  Complex leaves (det_logw, dW)
+// give O(re part) = dL/dtheta and O(im part) = i dL/dtheta.
+template <typename T>
+__global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __restrict__ R,
+                                                 const int32_t* __restrict__ S, int64_t w0, const T* __restrict__ bf,
+                                                 T* __restrict__ ybar, double* __restrict__ out_lp, T* __restrict__ O,
+                                                 int64_t ldo, int64_t W_total, DetBwdOffsets off) {
+  extern __shared__ double shd[];
+  const int N = p.N, NN = N * N, K = p.K;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  cd* env = reinterpret_cast<cd*>(shd);
+  cd* envb = env + NN;
+  cd* M = envb + NN;
+  cd* Mi = M + NN;
+  cd* res = Mi + NN;  // K: logdet, then weights
+  double* pos = reinterpret_cast<double*>(res + K);
+  int* spin = reinterpret_cast<int*>(pos + 2 * N);
+  int* piv = spin + N;
+  int* flag = piv + N;
+  const int64_t cfg = blockIdx.x;
+  const int64_t w = w0 + cfg;
+  for (int i = tid; i < N; i += nthr) {
+    pos[2 * i] = R[(w * N + i) * 2];
+    pos[2 * i + 1] = R[(w * N + i) * 2 + 1];
+    spin[i] = S[w * N + i];
+  }
+  __syncthreads();
+  for (int e = tid; e < NN; e += nthr) {
+    const int a = e / N, i = e % N;
+    cd v = mk(0, 0);
+    for (int g = 0; g < p.n_g; ++g) {
+      double sn, cs;
+      sincos(p.gs[2 * g] * pos[2 * i] + p.gs[2 * g + 1] * pos[2 * i + 1], &sn, &cs);
+      const double* wp = p.wenv + ((size_t)(a * p.n_g + g) * 2 + spin[i]) * 2;
+      v = v + mk(wp[0], wp[1]) * mk(cs, sn);
+    }
+    env[e] = v;
+    envb[e] = mk(0, 0);
+  }
+  __syncthreads();
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN;
+  const T* bfc = bf + cfg * (int64_t)N * rowlen;
+  auto bfval = [&](int k, int a, int i) -> cd {
+    const T* r = bfc + (int64_t)i * rowlen;
+    return mk((double)r[k * N + a], (double)r[KN + k * N + a]);
+  };
+  // pass 1: log dets
+  for (int k = 0; k < K; ++k) {
+    for (int e = tid; e < NN; e += nthr) M[e] = env[e] * (mk(1, 0) + bfval(k, e / N, e % N));
+    __syncthreads();
+    wg_lu(M, N, piv, flag);
+    if (tid == 0) res[k] = lu_logdet(M, N, flag[0]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cd L;
+    if (K == 1) {
+      L = res[0];
+      res[0] = mk(1, 0);
+    } else {
+      double amax = -INFINITY;
+      for (int k = 0; k < K; ++k) {
+        res[k] = res[k] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]);
+        amax = res[k].re > amax ? res[k].re : amax;
+      }
+      if (!isfinite(amax)) amax = 0.0;
+      cd s = mk(0, 0);
+      for (int k = 0; k < K; ++k) s = s + cexp_(res[k] - mk(amax, 0));
+      L = clog_(s) + mk(amax, 0);
+      for (int k = 0; k < K; ++k) res[k] = cexp_(res[k] - L);
+    }
+    double J = 0;
+    if (p.use_jastrow) {
+      for (int a = 0; a < N; ++a)
+        for (int b = a + 1; b < N; ++b) {
+          const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
+          fx -= rint(fx);
+          fy -= rint(fy);
+          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
+          const double r = sqrt(mx * mx + my * my + 1e-12);
+          double u, u1, u2;
+          jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
+          J += u;
+        }
+    }
+    if (out_lp) {
+      out_lp[2 * cfg] = L.re + J;
+      out_lp[2 * cfg + 1] = L.im;
+    }
+  }
+  __syncthreads();
+  // pass 2: inverses, backflow cotangent, envelope cotangent
+  T* y0 = ybar + (cfg * 2 + 0) * (int64_t)N * rowlen;
+  T* y1 = ybar + (cfg * 2 + 1) * (int64_t)N * rowlen;
+  for (int k = 0; k < K; ++k) {
+    for (int e = tid; e < NN; e += nthr) M[e] = env[e] * (mk(1, 0) + bfval(k, e / N, e % N));
+    __syncthreads();
+    wg_lu(M, N, piv, flag);
+    wg_inverse(M, Mi, N, piv);
+    const cd pk = res[k];
+    for (int e = tid; e < NN; e += nthr) {
+      const int a = e / N, i = e % N;
+      const cd mi = pk * Mi[i * N + a];
+      const cd g = env[e] * mi;  // dL/dbf[k,a,i]
+      const int64_t col = (int64_t)i * rowlen + k * N + a;
+      y0[col] = (T)g.re;
+      y0[col + KN] = (T)(-g.im);
+      y1[col] = (T)g.im;
+      y1[col + KN] = (T)g.re;
+      envb[e] = envb[e] + (mk(1, 0) + bfval(k, a, i)) * mi;
+    }
+    __syncthreads();
+  }
+  T* Ore = O + (w) * ldo;
+  T* Oim = O + (W_total + w) * ldo;
+  if (off.dlw >= 0) {
+    for (int k = tid; k < K; k += nthr) {
+      const cd pk = res[k];
+      Ore[off.dlw + k] = (T)pk.re;
+      Oim[off.dlw + k] = (T)pk.im;
+      Ore[off.dlw + K + k] = (T)(-pk.im);
+      Oim[off.dlw + K + k] = (T)pk.re;
+    }
+  }
+  if (off.dW >= 0) {
+    const int64_t nW = (int64_t)N * p.n_g * 2;
+    for (int64_t q = tid; q < nW; q += nthr) {
+      const int a = (int)(q / (p.n_g * 2)), g = (int)((q / 2) % p.n_g), s = (int)(q % 2);
+      cd v = mk(0, 0);
+      for (int i = 0; i < N; ++i) {
+        if (spin[i] != s) continue;
+        double sn, cs;
+        sincos(p.gs[2 * g] * pos[2 * i] + p.gs[2 * g + 1] * pos[2 * i + 1], &sn, &cs);
+        v = v + envb[a * N + i] * mk(cs, sn);
+      }
+      Ore[off.dW + q] = (T)v.re;
+      Oim[off.dW + q] = (T)v.im;
+      Ore[off.dW + nW + q] = (T)(-v.im);
+      Oim[off.dW + nW + q] = (T)v.re;
+    }
+  }
+  if (p.use_jastrow) {
+    const int nk = p.jastrow_kind == 0 ? p.n_knots : 1;
+    for (int q = tid; q < 2 * nk; q += nthr) {
+      // q < nk: same-spin knot (or cusp loga), else opposite-spin knot
+      const bool same_leaf = q < nk;
+      const int kk = q % nk;
+      double v = 0;
+      for (int a = 0; a < N; ++a)
+        for (int b = a + 1; b < N; ++b) {
+          const bool same = spin[a] == spin[b];
+          const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
+          fx -= rint(fx);
+          fy -= rint(fy);
+          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
+          const double r = sqrt(mx * mx + my * my + 1e-12);
+          if (p.jastrow_kind == 0) {
+            if (same != same_leaf) continue;
+            const double mu = (p.n_knots > 1) ? p.rbf_rcut * (double)kk / (double)(p.n_knots - 1) : 0.0;
+            const double z = (r - mu) / p.rbf_w;
+            v += exp(-z * z);
+          } else {
+            if (!same_leaf) continue;
+            const double la = p.jas[0];
+            const double aa = (la > 0 ? la + log1p(exp(-la)) : log1p(exp(la))) + 0.05;
+            const double sig = 1.0 / (1.0 + exp(-la));
+            const double al = same ? p.cusp / 3.0 : p.cusp;
+            const double e = exp(-r / aa);
+            v += (al * (1.0 - e) - al * (r / aa) * e) * sig;
+          }
+        }
+      int64_t o;
+      if (p.jastrow_kind == 0)
+        o = same_leaf ? off.jas_same + kk : off.jas_opp + kk;
+      else {
+        if (!same_leaf) continue;
+        o = off.jas_loga;
+      }
+      Ore[o] = (T)v;
+      Oim[o] = (T)0;
+    }
+  }
+}
+
+template <typename T>
+int launch_det_bwd(int64_t n_cfg, const DetParams& p, const double* R, const int32_t* S, int64_t w0, const T* bf,
+                   T* ybar, double* out_lp, T* O, int64_t ldo, int64_t W_total, const DetBwdOffsets& off,
+                   hipStream_t st) {
+  if (n_cfg == 0) return NQS_OK;
+  const int N = p.N, NN = N * N;
+  size_t lds = sizeof(cd) * (size_t)(4 * NN + p.K) + sizeof(double) * 2 * N + sizeof(int) * (2 * N + 4);
+  lds = round_up((int64_t)lds, 16);
+  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "determinant backward exceeds LDS");
+  hipLaunchKernelGGL(k_det_bwd<T>, dim3((unsigned)n_cfg), dim3(256), lds, st, p, R, S, w0, bf, ybar, out_lp, O, ldo,
+                     W_total, off);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+template int launch_det_bwd<double>(int64_t, const DetParams&, const double*, const int32_t*, int64_t, const double*,
+                                    double*, double*, double*, int64_t, int64_t, const DetBwdOffsets&, hipStream_t);
+template int launch_det_bwd<float>(int64_t, const DetParams&, const double*, const int32_t*, int64_t, const float*,
+                                   float*, double*, float*, int64_t, int64_t, const DetBwdOffsets&, hipStream_t);
 
 }  // namespace nqs

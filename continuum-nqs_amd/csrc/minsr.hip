@@ -1,0 +1,609 @@
+// MinSR sample-space solve (src/nqs/minsr.py:68-81):
+//   Otil = [Re dO; Im dO] (2W x P, columns centred over walkers), etil = [Re ec; Im ec]
+//   T = Otil Otil^T,  (T + lam I) y = etil,  dtheta = Otil^T y
+// Kernels: column means / centring (+ bf16 copy), the Gram as a lower-triangular SYRK on MFMA
+// (bf16 in, f32 accumulate; or the f64 / f32 MFMA GEMM), a float64 blocked right-looking
+// Cholesky (diagonal block factor + inverse in one workgroup, panel and trailing update on the
+// f64 MFMA GEMM), blocked triangular solves, one step of iterative refinement and the
+// push-through GEMV.  Also the median/MAD clip of the local energies (minsr.py:142-152).
+#include "nqs_common.h"
+#include "kernels.h"
+#include "minsr.h"
+
+namespace nqs {
+
+// ------------------------------------------------------------------ centring
+template <typename T>
+__global__ void k_colmean(const T* __restrict__ O, int64_t ldo, int64_t W, int64_t P, double* __restrict__ mean) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int half = blockIdx.y;
+  if (k >= P) return;
+  const T* base = O + (int64_t)half * W * ldo + k;
+  double s = 0;
+  for (int64_t w = 0; w < W; ++w) s += (double)base[w * ldo];
+  mean[(int64_t)half * P + k] = s / (double)W;
+}
+
+template <typename T>
+__global__ void k_center(T* __restrict__ O, int64_t ldo, int64_t W, int64_t P, const double* __restrict__ mean,
+                         __bf16* __restrict__ Ob) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y;
+  if (k >= ldo) return;
+  const int half = r >= W ? 1 : 0;
+  T v = (T)0;
+  if (k < P) {
+    v = (T)((double)O[r * ldo + k] - mean[(int64_t)half * P + k]);
+    O[r * ldo + k] = v;
+  }
+  if (Ob) Ob[r * ldo + k] = (__bf16)(float)v;
+}
+
+template <typename T>
+int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, hipStream_t st) {
+  hipLaunchKernelGGL(k_colmean<T>, dim3((unsigned)cdiv(P, 256), 2), dim3(256), 0, st, O, ldo, W, P, mean);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_center<T>, dim3((unsigned)cdiv(ldo, 256), (unsigned)(2 * W)), dim3(256), 0, st, O, ldo, W, P,
+                     mean, Ob);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// etil = [Re(e - mean e); Im(e - mean e)] (one workgroup)
+__global__ void k_etil(const double* __restrict__ e, int64_t W, double* __restrict__ et) {
+  __shared__ double red[2 * 16];
+  double sr = 0, si = 0;
+  for (int64_t w = threadIdx.x; w < W; w += blockDim.x) {
+    sr += e[2 * w];
+    si += e[2 * w + 1];
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = sr;
+    red[2 * (threadIdx.x >> 6) + 1] = si;
+  }
+  __syncthreads();
+  double mr = 0, mi = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+    mr += red[2 * q];
+    mi += red[2 * q + 1];
+  }
+  mr /= (double)W;
+  mi /= (double)W;
+  for (int64_t w = threadIdx.x; w < W; w += blockDim.x) {
+    et[w] = e[2 * w] - mr;
+    et[W + w] = e[2 * w + 1] - mi;
+  }
+}
+
+// ------------------------------------------------------------------ bf16 Gram (lower-triangular SYRK)
+// C[i][j] (i >= j) = sum_k A[i][k] A[j][k]; A bf16 row-major (n x K), C f32.  Block tile
+// 128 x 128 x 64 on 4 waves (64 x 64 each = 2 x 2 of v_mfma_f32_32x32x16_bf16), LDS double
+// buffer with a 16-byte row pad, global loads of 16 bytes per lane.  Only tiles on or below
+// the diagonal are launched (linear tile index -> (bm, bn)).
+#define GB_BM 128
+#define GB_BK 64
+#define GB_LD (GB_BK + 8)
+__global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A, int64_t lda, int64_t K,
+                                                   float* __restrict__ C, int64_t ldc) {
+  __shared__ __bf16 As[2][GB_BM][GB_LD];
+  __shared__ __bf16 Bs[2][GB_BM][GB_LD];
+  const int64_t t = blockIdx.x;
+  int bm = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(bm + 1) * (bm + 2) / 2 <= t) ++bm;
+  while ((int64_t)bm * (bm + 1) / 2 > t) --bm;
+  const int bn = (int)(t - (int64_t)bm * (bm + 1) / 2);
+  const int64_t m0 = (int64_t)bm * GB_BM, n0 = (int64_t)bn * GB_BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  f16v acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  // each thread moves 4 x 16 bytes of A and of B per stage: idx = tid + 256 q, row = idx / 8, chunk = idx % 8
+  uint4 ra[4], rb[4];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, row = idx >> 3, ch = idx & 7;
+      ra[q] = *reinterpret_cast<const uint4*>(A + (m0 + row) * lda + k0 + ch * 8);
+      rb[q] = *reinterpret_cast<const uint4*>(A + (n0 + row) * lda + k0 + ch * 8);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + 256 * q, row = idx >> 3, ch = idx & 7;
+      *reinterpret_cast<uint4*>(&As[buf][row][ch * 8]) = ra[q];
+      *reinterpret_cast<uint4*>(&Bs[buf][row][ch * 8]) = rb[q];
+    }
+  };
+  const int64_t nk = K / GB_BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t kb = 0; kb < nk; ++kb) {
+    const int cur = kb & 1;
+    if (kb + 1 < nk) gload((kb + 1) * GB_BK);
+#pragma unroll
+    for (int kk = 0; kk < GB_BK; kk += 16) {
+      bf8v a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf8v*>(&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *reinterpret_cast<const bf8v*>(&Bs[cur][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kb + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (row >= col) C[row * ldc + col] = acc[i][j][r];
+      }
+}
+
+int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st) {
+  NQS_REQUIRE(n % GB_BM == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs 2W to be a multiple of 128");
+  NQS_REQUIRE(K % GB_BK == 0 && lda % 8 == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs K % 64 == 0 and lda % 8 == 0");
+  NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
+  const int64_t nt = n / GB_BM;
+  const int64_t tiles = nt * (nt + 1) / 2;
+  hipLaunchKernelGGL(k_syrk_bf16, dim3((unsigned)tiles), dim3(256), 0, st, A, lda, K, C, ldc);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ T (lower) -> float64 + lam I
+template <typename T>
+__global__ void k_to_f64_shift(const T* __restrict__ Tin, int64_t ldt, int64_t n, double lam, double* __restrict__ A,
+                               int64_t lda) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y;
+  if (c > r || c >= n) return;
+  double v = (double)Tin[r * ldt + c];
+  if (r == c) v += lam;
+  A[r * lda + c] = v;
+}
+
+template <typename T>
+int to_f64_shift(const T* Tin, int64_t ldt, int64_t n, double lam, double* A, int64_t lda, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_f64_shift<T>, dim3((unsigned)cdiv(n, 256), (unsigned)n), dim3(256), 0, st, Tin, ldt, n, lam,
+                     A, lda);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ deflation of the centring null space
+// Centring makes the Re rows and the Im rows of Otil sum to zero, so T u = T v = 0 for
+// u = 1_Re/sqrt(W), v = 1_Im/sqrt(W).  etil is orthogonal to u and v and Otil^T u = Otil^T v = 0,
+// so the solution and dtheta do not depend on T's restriction to span{u, v}; rounding of a
+// low-precision Gram, however, can push those two eigenvalues below -lam.  The lower triangle of
+// T is replaced by  Pi T Pi + c (u u^T + v v^T),  Pi = I - u u^T - v v^T, c = mean diag T, which
+// has the same solution in exact arithmetic and is positive definite.
+// ts[i] = (T u)_i, ts[n + i] = (T v)_i, from the lower triangle: row part (c <= r) ...
+__global__ __launch_bounds__(256) void k_defl_rows(const double* __restrict__ A, int64_t lda, int64_t W,
+                                                   double* __restrict__ ts) {
+  __shared__ double red[8];
+  const int64_t r = blockIdx.x;
+  const int64_t n = 2 * W;
+  double su = 0, sv = 0;
+  for (int64_t c = threadIdx.x; c <= r; c += blockDim.x) {
+    const double a = A[r * lda + c];
+    if (c < W) su += a; else sv += a;
+  }
+  su = wave_sum(su);
+  sv = wave_sum(sv);
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = su;
+    red[2 * (threadIdx.x >> 6) + 1] = sv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      a += red[2 * q];
+      b += red[2 * q + 1];
+    }
+    const double s = 1.0 / sqrt((double)W);
+    ts[r] = a * s;
+    ts[n + r] = b * s;
+  }
+}
+// ... plus the column part (r > c), one thread per column, coalesced over consecutive columns
+__global__ void k_defl_cols(const double* __restrict__ A, int64_t lda, int64_t W, double* __restrict__ ts) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = 2 * W;
+  if (c >= n) return;
+  double su = 0, sv = 0;
+  for (int64_t r = c + 1; r < n; ++r) {
+    const double a = A[r * lda + c];
+    if (r < W) su += a; else sv += a;
+  }
+  const double s = 1.0 / sqrt((double)W);
+  ts[c] += su * s;
+  ts[n + c] += sv * s;
+}
+// scalars: alpha = u.Tu, beta = u.Tv, gamma = v.Tv, c = mean diagonal (one workgroup)
+__global__ void k_defl_scalars(const double* __restrict__ A, int64_t lda, int64_t W, const double* __restrict__ ts,
+                               double* __restrict__ sc) {
+  __shared__ double red[4 * 16];
+  const int64_t n = 2 * W;
+  const double s = 1.0 / sqrt((double)W);
+  double al = 0, be = 0, ga = 0, dg = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (i < W) {
+      al += s * ts[i];
+      be += s * ts[n + i];
+    } else {
+      ga += s * ts[n + i];
+    }
+    dg += A[i * lda + i];
+  }
+  al = wave_sum(al);
+  be = wave_sum(be);
+  ga = wave_sum(ga);
+  dg = wave_sum(dg);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[4 * w] = al;
+    red[4 * w + 1] = be;
+    red[4 * w + 2] = ga;
+    red[4 * w + 3] = dg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[4] = {0, 0, 0, 0};
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q)
+      for (int k = 0; k < 4; ++k) t[k] += red[4 * q + k];
+    sc[0] = t[0];
+    sc[1] = t[1];
+    sc[2] = t[2];
+    sc[3] = t[3] / (double)n;
+  }
+}
+__global__ void k_defl_apply(double* __restrict__ A, int64_t lda, int64_t W, double lam,
+                             const double* __restrict__ ts, const double* __restrict__ sc) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y;
+  const int64_t n = 2 * W;
+  if (c > r) return;
+  const double s = 1.0 / sqrt((double)W);
+  const double ur = r < W ? s : 0.0, vr = r < W ? 0.0 : s;
+  const double uc = c < W ? s : 0.0, vc = c < W ? 0.0 : s;
+  const double ar = ts[r], br = ts[n + r], ac = ts[c], bc = ts[n + c];
+  const double al = sc[0], be = sc[1], ga = sc[2], cc = sc[3];
+  double d = -(ur * ac + vr * bc) - (ar * uc + br * vc) + al * ur * uc + be * (ur * vc + vr * uc) + ga * vr * vc;
+  d += cc * (ur * uc + vr * vc);
+  if (r == c) d += lam;
+  A[r * lda + c] += d;
+}
+
+int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, double* sc, hipStream_t st) {
+  const int64_t n = 2 * W;
+  hipLaunchKernelGGL(k_defl_rows, dim3((unsigned)n), dim3(256), 0, st, A, lda, W, ts);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_defl_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, A, lda, W, ts);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_defl_scalars, dim3(1), dim3(1024), 0, st, A, lda, W, ts, sc);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_defl_apply, dim3((unsigned)cdiv(n, 256), (unsigned)n), dim3(256), 0, st, A, lda, W, lam, ts, sc);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ Cholesky (lower, float64, nb = 64)
+#define CH_NB 64
+// Factor the nb x nb diagonal block at (j0, j0) in place (lower), store its inverse (lower)
+// in Linv (nb x nb).  info[0] = first non-positive pivot + 1 (0 when the block is SPD).
+__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0, int nb,
+                                                    double* __restrict__ Linv, int* __restrict__ info) {
+  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double X[CH_NB][CH_NB + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e % nb;
+    L[r][c] = (c <= r) ? A[(j0 + r) * lda + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    if (tid == 0) {
+      const double d = L[c][c];
+      if (!(d > 0.0)) {
+        if (info[0] == 0) info[0] = (int)(j0 + c + 1);
+        L[c][c] = 1.0;
+      } else {
+        L[c][c] = sqrt(d);
+      }
+    }
+    __syncthreads();
+    for (int r = c + 1 + tid; r < nb; r += blockDim.x) L[r][c] /= L[c][c];
+    __syncthreads();
+    const int w = nb - c - 1;
+    for (int e = tid; e < w * w; e += blockDim.x) {
+      const int r = c + 1 + e / w, k = c + 1 + e % w;
+      if (k <= r) L[r][k] -= L[r][c] * L[k][c];
+    }
+    __syncthreads();
+  }
+  // X = L^-1 by forward substitution, column c by thread c
+  for (int c = tid; c < nb; c += blockDim.x) {
+    for (int r = 0; r < nb; ++r) {
+      double s = (r == c) ? 1.0 : 0.0;
+      for (int k = c; k < r; ++k) s -= L[r][k] * X[k][c];
+      X[r][c] = (r >= c) ? s / L[r][r] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e % nb;
+    A[(j0 + r) * lda + j0 + c] = (c <= r) ? L[r][c] : 0.0;
+    Linv[e] = X[r][c];
+  }
+}
+
+__global__ void k_copy_panel(const double* __restrict__ P, int64_t m, int nb, double* __restrict__ A, int64_t lda,
+                             int64_t r0, int64_t c0) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m * nb) return;
+  const int64_t r = q / nb, c = q % nb;
+  A[(r0 + r) * lda + c0 + c] = P[q];
+}
+
+static GemmArgs<double> g64(const double* A, const double* B, double* C, int64_t M, int64_t N, int64_t K, int64_t lda,
+                            int64_t ldb, int64_t ldc, double alpha, double beta, int tri) {
+  GemmArgs<double> g;
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.ldc = ldc;
+  g.strideA = g.strideB = g.strideC = 0;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.bias = nullptr;
+  g.bias_period = 1;
+  g.bias_len = 0;
+  g.tri = tri;
+  g.mirror = 0;
+  return g;
+}
+
+// A (n x n, lower) -> L with A = L L^T, in place.  Linv: (n/nb) diagonal-block inverses,
+// panel: n x nb scratch.
+int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st) {
+  NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "Cholesky size must be a multiple of 64");
+  for (int64_t j0 = 0; j0 < n; j0 += CH_NB) {
+    double* Li = Linv + (j0 / CH_NB) * CH_NB * CH_NB;
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, st, A, lda, j0, CH_NB, Li, info);
+    NQS_LAUNCH_CHECK();
+    const int64_t m = n - j0 - CH_NB;
+    if (m == 0) break;
+    // panel = A21 Linv^T  (NT GEMM with B = Linv, K = nb)
+    NQS_TRY(gemm_nt<double>(g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0),
+                            1, st));
+    // A22 -= panel panel^T (lower tiles)
+    NQS_TRY(gemm_nt<double>(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, m, CH_NB, CH_NB, CH_NB, lda, -1.0,
+                                1.0, 1),
+                            1, st));
+    hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256)), dim3(256), 0, st, panel, m, CH_NB, A, lda,
+                       j0 + CH_NB, j0);
+    NQS_LAUNCH_CHECK();
+  }
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ triangular solves with the factor
+// b_blk <- Linv_blk b_blk (trans = 0) or Linv_blk^T b_blk (trans = 1); one workgroup
+__global__ void k_trsv_diag(const double* __restrict__ Li, double* __restrict__ b, int nb, int trans) {
+  __shared__ double x[CH_NB];
+  const int t = threadIdx.x;
+  if (t < nb) x[t] = b[t];
+  __syncthreads();
+  if (t < nb) {
+    double s = 0;
+    for (int k = 0; k < nb; ++k) s += (trans ? Li[k * nb + t] : Li[t * nb + k]) * x[k];
+    b[t] = s;
+  }
+}
+// forward: rows r >= r0: b[r] -= sum_c L[r][j0 + c] z[c]
+__global__ void k_trsv_fwd_update(const double* __restrict__ A, int64_t lda, int64_t j0, int nb, int64_t n,
+                                  double* __restrict__ b) {
+  const int64_t r = j0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0;
+  for (int c = 0; c < nb; ++c) s += A[r * lda + j0 + c] * b[j0 + c];
+  b[r] -= s;
+}
+// backward: columns c < j0: z[c] -= sum_r L[j0 + r][c] y[j0 + r]
+__global__ void k_trsv_bwd_update(const double* __restrict__ A, int64_t lda, int64_t j0, int nb,
+                                  double* __restrict__ z) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= j0) return;
+  double s = 0;
+  for (int r = 0; r < nb; ++r) s += A[(j0 + r) * lda + c] * z[j0 + r];
+  z[c] -= s;
+}
+
+int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st) {
+  for (int64_t j0 = 0; j0 < n; j0 += CH_NB) {
+    hipLaunchKernelGGL(k_trsv_diag, dim3(1), dim3(64), 0, st, Linv + (j0 / CH_NB) * CH_NB * CH_NB, x + j0, CH_NB, 0);
+    const int64_t m = n - j0 - CH_NB;
+    if (m > 0)
+      hipLaunchKernelGGL(k_trsv_fwd_update, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, st, L, lda, j0, CH_NB, n, x);
+  }
+  for (int64_t j0 = n - CH_NB; j0 >= 0; j0 -= CH_NB) {
+    hipLaunchKernelGGL(k_trsv_diag, dim3(1), dim3(64), 0, st, Linv + (j0 / CH_NB) * CH_NB * CH_NB, x + j0, CH_NB, 1);
+    if (j0 > 0)
+      hipLaunchKernelGGL(k_trsv_bwd_update, dim3((unsigned)cdiv(j0, 256)), dim3(256), 0, st, L, lda, j0, CH_NB, x);
+  }
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ GEMVs over O and T
+// y[r] = alpha sum_c A[r][c] x[c] + beta y[r]; one workgroup per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_rows(const T* __restrict__ A, int64_t lda, int64_t ncol,
+                                                   const double* __restrict__ x, double* __restrict__ y, double alpha,
+                                                   double beta, int sym_lower) {
+  __shared__ double red[8];
+  const int64_t r = blockIdx.x;
+  double s = 0;
+  if (!sym_lower) {
+    for (int64_t c = threadIdx.x; c < ncol; c += blockDim.x) s += (double)A[r * lda + c] * x[c];
+  } else {
+    // symmetric matrix stored in the lower triangle: row part c <= r, column part c > r
+    for (int64_t c = threadIdx.x; c < ncol; c += blockDim.x)
+      s += (double)(c <= r ? A[r * lda + c] : A[c * lda + r]) * x[c];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+    y[r] = alpha * t + (beta != 0.0 ? beta * y[r] : 0.0);
+  }
+}
+// y[c] = sum_r A[r][c] x[r]; one thread per column
+template <typename T>
+__global__ void k_gemv_cols(const T* __restrict__ A, int64_t lda, int64_t nrow, int64_t ncol,
+                            const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  double s = 0;
+  for (int64_t r = 0; r < nrow; ++r) s += (double)A[r * lda + c] * x[r];
+  y[c] = s;
+}
+
+template <typename T>
+int gemv_rows(const T* A, int64_t lda, int64_t nrow, int64_t ncol, const double* x, double* y, double alpha,
+              double beta, int sym_lower, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_rows<T>, dim3((unsigned)nrow), dim3(256), 0, st, A, lda, ncol, x, y, alpha, beta,
+                     sym_lower);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+template <typename T>
+int gemv_cols(const T* A, int64_t lda, int64_t nrow, int64_t ncol, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_cols<T>, dim3((unsigned)cdiv(ncol, 256)), dim3(256), 0, st, A, lda, nrow, ncol, x, y);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, const double* __restrict__ y,
+                       double b, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+int axpby(int64_t n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, n, a, x, y, b, out);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ median / MAD clip (minsr.py:142-152)
+// One workgroup; values sorted in LDS by a bitonic network (W <= 8192 per pass of 16384 slots).
+__device__ void bitonic_sort(double* v, int n2) {
+  for (int k = 2; k <= n2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const double a = v[i], b = v[l];
+          if ((a > b) == up) {
+            v[i] = b;
+            v[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_clip_stats(const double* __restrict__ e, int W, double clip_k,
+                                                     double* __restrict__ e_clip, double* __restrict__ stats) {
+  // every operation rounded on its own, as numpy evaluates the reference's expressions
+#pragma clang fp contract(off)
+  extern __shared__ double v[];
+  int n2 = 1;
+  while (n2 < W) n2 <<= 1;
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) v[i] = (i < W) ? e[2 * i] : INFINITY;
+  __syncthreads();
+  bitonic_sort(v, n2);
+  const double med = (W & 1) ? v[W / 2] : 0.5 * (v[W / 2 - 1] + v[W / 2]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) v[i] = (i < W) ? fabs(e[2 * i] - med) : INFINITY;
+  __syncthreads();
+  bitonic_sort(v, n2);
+  const double mad = ((W & 1) ? v[W / 2] : 0.5 * (v[W / 2 - 1] + v[W / 2])) + 1e-12;
+  // rounded as numpy does (k * MAD first, then the sum), no fused multiply-add
+  const double kmad = clip_k * mad;
+  const double lo = med - kmad, hi = med + kmad;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) {
+    const double x = e[2 * i];
+    e_clip[2 * i] = x < lo ? lo : (x > hi ? hi : x);
+    e_clip[2 * i + 1] = e[2 * i + 1];
+  }
+  if (threadIdx.x == 0) {
+    double s = 0, s2 = 0;
+    for (int i = 0; i < W; ++i) s += e[2 * i];
+    const double mean = s / W;
+    for (int i = 0; i < W; ++i) s2 += (e[2 * i] - mean) * (e[2 * i] - mean);
+    stats[0] = mean;
+    stats[1] = sqrt(s2 / W) / sqrt((double)W);
+    stats[2] = med;
+    stats[3] = mad;
+  }
+}
+
+int clip_stats(const double* e, int64_t W, double clip_k, double* e_clip, double* stats, hipStream_t st) {
+  NQS_REQUIRE(W >= 1 && W <= 16384, NQS_ERR_UNSUPPORTED, "clip supports 1 <= W <= 16384");
+  int n2 = 1;
+  while (n2 < W) n2 <<= 1;
+  static bool attr = false;
+  if (!attr) {
+    NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_clip_stats, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_clip_stats, dim3(1), dim3(1024), sizeof(double) * n2, st, e, (int)W, clip_k, e_clip, stats);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+int make_etil(const double* e, int64_t W, double* et, hipStream_t st) {
+  hipLaunchKernelGGL(k_etil, dim3(1), dim3(1024), 0, st, e, W, et);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+template int minsr_center<double>(double*, int64_t, int64_t, int64_t, double*, __bf16*, hipStream_t);
+template int minsr_center<float>(float*, int64_t, int64_t, int64_t, double*, __bf16*, hipStream_t);
+template int to_f64_shift<double>(const double*, int64_t, int64_t, double, double*, int64_t, hipStream_t);
+template int to_f64_shift<float>(const float*, int64_t, int64_t, double, double*, int64_t, hipStream_t);
+template int gemv_rows<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, double, double, int,
+                               hipStream_t);
+template int gemv_rows<float>(const float*, int64_t, int64_t, int64_t, const double*, double*, double, double, int,
+                              hipStream_t);
+template int gemv_cols<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, hipStream_t);
+template int gemv_cols<float>(const float*, int64_t, int64_t, int64_t, const double*, double*, hipStream_t);
+
+}  // namespace nqs

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include "model.h"
+#include "bwd.h"
 
 namespace nqs {
 
@@ -189,11 +190,6 @@ static int copy2d(const double* src, int64_t rows, int64_t cols, int64_t src_rs,
   return NQS_OK;
 }
 
-#define NQS_TRY(x)              \
-  do {                          \
-    int _rc = (x);              \
-    if (_rc != NQS_OK) return _rc; \
-  } while (0)
 
 template <typename T>
 static int pack_backbone(const Model& md, const double* th, T* pk, hipStream_t st) {
@@ -400,5 +396,171 @@ template int run_forward<double>(const Model&, const void*, const JetGeom&, cons
                                  const FwdBufs<double>&, hipStream_t);
 template int run_forward<float>(const Model&, const void*, const JetGeom&, const double*, const int32_t*, int64_t,
                                 const FwdBufs<float>&, hipStream_t);
+
+
+// ------------------------------------------------------------------ backward driver
+int64_t grad_bytes(const Model& md, int64_t n_cfg) {
+  const int64_t rows = n_cfg * md.N, t = md.tsize;
+  const int64_t d = md.d, ffd = (int64_t)md.ff * md.d, KN2 = 2LL * md.KN;
+  const int64_t wq = std::max<int64_t>(3 * d, KN2);
+  int64_t b = al(rows * md.kpad * t) + (md.depth + 1) * al(rows * d * t);
+  b += 4 * al(rows * d * t) + al(rows * 3 * d * t) + 2 * al(rows * ffd * t) + al(rows * d * t) + al(rows * KN2 * t);
+  b += al(2 * rows * d * t) + al(2 * rows * ffd * t) + al(2 * rows * d * t) + al(2 * rows * wq * t);
+  return b;
+}
+
+template <typename T>
+int run_backward(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t w0, int64_t n_cfg,
+                 int64_t W_total, T* O, int64_t ldo, double* out_lp, char* ws, hipStream_t st) {
+  const T* pk = (const T*)packed;
+  const int N = md.N, d = md.d, ffd = md.ff * md.d, KN = md.KN, depth = md.depth;
+  const int64_t rows = n_cfg * N;
+  char* p = ws;
+  auto grab = [&](int64_t elems) {
+    T* r = (T*)p;
+    p += al(elems * (int64_t)sizeof(T));
+    return r;
+  };
+  T* X0 = grab(rows * md.kpad);
+  std::vector<T*> xs(depth + 1);
+  for (int l = 0; l <= depth; ++l) xs[l] = grab(rows * d);
+  T* hA = grab(rows * d);
+  T* attA = grab(rows * d);
+  T* xmid = grab(rows * d);
+  T* hB = grab(rows * d);
+  T* qkv = grab(rows * 3 * d);
+  T* U = grab(rows * ffd);
+  T* Gv = grab(rows * ffd);
+  T* xf = grab(rows * d);
+  T* bfo = grab(rows * 2 * KN);
+  T* xb = grab(2 * rows * d);
+  T* gb = grab(2 * rows * ffd);
+  T* hb = grab(2 * rows * d);
+  T* qb = grab(2 * rows * std::max(3 * d, 2 * KN));
+  const JetGeom jg{n_cfg, N, 1, 0, 0, 0};
+  const int64_t per = N, blen = N;
+  auto leaf = [&](const std::string& n) -> int64_t {
+    const Leaf* l = find_leaf(md, n);
+    return l ? l->offset : -1;
+  };
+  auto wg = [&](const T* X, int64_t ldx, int in, const T* G, int out, std::initializer_list<int64_t> bases,
+                int seg_w) {
+    WgradArgs<T> a;
+    a.X = X;
+    a.ldx = ldx;
+    a.in = in;
+    a.G = G;
+    a.ldg = out;
+    a.out = out;
+    a.N = N;
+    a.n_cfg = n_cfg;
+    a.w0 = w0;
+    a.W_total = W_total;
+    a.O = O;
+    a.ldo = ldo;
+    int q = 0;
+    for (int64_t b : bases) a.seg_base[q++] = b;
+    for (; q < 3; ++q) a.seg_base[q] = 0;
+    a.seg_w = seg_w;
+    return a;
+  };
+  auto lnb = [&](const T* x, const T* ybar, T* xbar, const T* scale, int acc, const std::string& name) {
+    LnBwdArgs<T> a;
+    a.x = x;
+    a.ybar = ybar;
+    a.xbar = xbar;
+    a.scale = scale;
+    a.d = d;
+    a.N = N;
+    a.accumulate = acc;
+    a.n_cfg = n_cfg;
+    a.w0 = w0;
+    a.W_total = W_total;
+    a.O = O;
+    a.ldo = ldo;
+    a.off_scale = leaf(name + "/scale");
+    a.off_bias = leaf(name + "/bias");
+    return launch_ln_bwd<T>(a, st);
+  };
+  auto gemm = [&](const T* A, const T* B, T* C, int64_t M, int64_t Nn, int64_t K, const T* bias, T beta) {
+    return gemm_nt<T>(gargs<T>(A, B, C, M, Nn, K, K, K, Nn, bias, per, blen, beta), 1, st);
+  };
+  // block forward from x_in into x_out (x_out = x_in + attn + mlp), keeping hA, qkv, attA, xmid, hB, U, Gv
+  auto block_fwd = [&](int l, const T* x_in, T* x_out) -> int {
+    const LayerPack& P = md.layers[l];
+    NQS_TRY(launch_layernorm_jet<T>(jg, d, x_in, pk + P.ln0_s, pk + P.ln0_b, hA, st));
+    NQS_TRY(gemm(hA, pk + P.qkvT, qkv, rows, 3 * d, d, pk + P.qkv_b, (T)0));
+    NQS_TRY(launch_attention_jet<T>(jg, d, md.nh, qkv, attA, st));
+    NQS_CHECK_HIP(hipMemcpyAsync(xmid, x_in, rows * d * sizeof(T), hipMemcpyDeviceToDevice, st));
+    NQS_TRY(gemm(attA, pk + P.oT, xmid, rows, d, d, pk + P.o_b, (T)1));
+    NQS_TRY(launch_layernorm_jet<T>(jg, d, xmid, pk + P.ln1_s, pk + P.ln1_b, hB, st));
+    NQS_TRY(gemm(hB, pk + P.w0T, U, rows, ffd, d, pk + P.b0, (T)0));
+    NQS_TRY(launch_gelu_jet<T>(jg, ffd, U, Gv, st));
+    if (x_out) {
+      NQS_CHECK_HIP(hipMemcpyAsync(x_out, xmid, rows * d * sizeof(T), hipMemcpyDeviceToDevice, st));
+      NQS_TRY(gemm(Gv, pk + P.w1T, x_out, rows, d, ffd, pk + P.b1, (T)1));
+    }
+    return NQS_OK;
+  };
+  // ---- forward with block-input checkpoints
+  NQS_TRY(launch_features<T>(jg, R, S, md.Ainv, md.F, md.kpad, w0, X0, st));
+  NQS_TRY(gemm_nt<T>(gargs<T>(X0, pk + md.embT, xs[0], rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per, blen,
+                              (T)0),
+                     1, st));
+  for (int l = 0; l < depth; ++l) NQS_TRY(block_fwd(l, xs[l], xs[l + 1]));
+  NQS_TRY(launch_layernorm_jet<T>(jg, d, xs[depth], pk + md.lnf_s, pk + md.lnf_b, xf, st));
+  NQS_TRY(gemm(xf, pk + md.bfT, bfo, rows, 2 * KN, d, pk + md.bf_b, (T)0));
+  // ---- determinant head
+  DetBwdOffsets off;
+  off.dlw = leaf("params/det_logw");
+  off.dW = leaf("params/dW");
+  off.jas_same = leaf("params/jas_c_same");
+  off.jas_opp = leaf("params/jas_c_opp");
+  off.jas_loga = leaf("params/jastrow_loga");
+  const DetParams dp = det_params(md, packed);
+  NQS_TRY(launch_det_bwd<T>(n_cfg, dp, R, S, w0, bfo, qb, out_lp, O, ldo, W_total, off, st));
+  // ---- backflow Dense (2KN real outputs = [Re | Im]; theta: re block then im block)
+  const int64_t bk = leaf("params/backflow/kernel"), bb = leaf("params/backflow/bias");
+  NQS_TRY(launch_wgrad<T>(wg(xf, d, d, qb, 2 * KN, {bk, bk + (int64_t)d * KN}, KN), st));
+  NQS_TRY(launch_bgrad<T>(wg(xf, d, d, qb, 2 * KN, {bb}, 2 * KN), st));
+  NQS_TRY(gemm(qb, pk + md.bfW, hb, 2 * rows, d, 2 * KN, nullptr, (T)0));
+  NQS_TRY(lnb(xs[depth], hb, xb, pk + md.lnf_s, 0, "params/LayerNorm_0"));
+  // ---- blocks in reverse, recomputing each block's activations from its checkpoint
+  for (int l = depth - 1; l >= 0; --l) {
+    const LayerPack& P = md.layers[l];
+    const std::string b = "params/TransformerBlock_" + std::to_string(l) + "/";
+    NQS_TRY(block_fwd(l, xs[l], nullptr));
+    // MLP out: x_out = xmid + Gv W1 + b1
+    NQS_TRY(gemm(xb, pk + P.w1, gb, 2 * rows, ffd, d, nullptr, (T)0));
+    NQS_TRY(launch_wgrad<T>(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/kernel")}, d), st));
+    NQS_TRY(launch_bgrad<T>(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/bias")}, d), st));
+    NQS_TRY(launch_gelu_bwd<T>(n_cfg, (int64_t)N * ffd, U, gb, gb, st));
+    NQS_TRY(gemm(gb, pk + P.w0, hb, 2 * rows, d, ffd, nullptr, (T)0));
+    NQS_TRY(launch_wgrad<T>(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/kernel")}, ffd), st));
+    NQS_TRY(launch_bgrad<T>(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/bias")}, ffd), st));
+    NQS_TRY(lnb(xmid, hb, xb, pk + P.ln1_s, 1, b + "LayerNorm_1"));
+    // attention out: xmid = x_in + attA Wo + bo
+    NQS_TRY(gemm(xb, pk + P.o, hb, 2 * rows, d, d, nullptr, (T)0));
+    NQS_TRY(launch_wgrad<T>(wg(attA, d, d, xb, d, {leaf(b + "attn/o/kernel")}, d), st));
+    NQS_TRY(launch_bgrad<T>(wg(attA, d, d, xb, d, {leaf(b + "attn/o/bias")}, d), st));
+    NQS_TRY(launch_attn_bwd<T>(n_cfg, N, d, md.nh, qkv, hb, qb, st));
+    NQS_TRY(gemm(qb, pk + P.qkv, hb, 2 * rows, d, 3 * d, nullptr, (T)0));
+    NQS_TRY(launch_wgrad<T>(wg(hA, d, d, qb, 3 * d,
+                               {leaf(b + "attn/q/kernel"), leaf(b + "attn/k/kernel"), leaf(b + "attn/v/kernel")}, d),
+                            st));
+    NQS_TRY(launch_bgrad<T>(
+        wg(hA, d, d, qb, 3 * d, {leaf(b + "attn/q/bias"), leaf(b + "attn/k/bias"), leaf(b + "attn/v/bias")}, d), st));
+    NQS_TRY(lnb(xs[l], hb, xb, pk + P.ln0_s, 1, b + "LayerNorm_0"));
+  }
+  // ---- embedding
+  NQS_TRY(launch_wgrad<T>(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/kernel")}, d), st));
+  NQS_TRY(launch_bgrad<T>(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/bias")}, d), st));
+  return NQS_OK;
+}
+
+template int run_backward<double>(const Model&, const void*, const double*, const int32_t*, int64_t, int64_t, int64_t,
+                                  double*, int64_t, double*, char*, hipStream_t);
+template int run_backward<float>(const Model&, const void*, const double*, const int32_t*, int64_t, int64_t, int64_t,
+                                 float*, int64_t, double*, char*, hipStream_t);
 
 }  // namespace nqs

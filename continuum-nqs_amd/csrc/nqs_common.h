@@ -40,6 +40,12 @@ void set_error(const std::string& msg);
     }                                                                                  \
   } while (0)
 
+#define NQS_TRY(x)                 \
+  do {                             \
+    int _rc = (x);                 \
+    if (_rc != NQS_OK) return _rc; \
+  } while (0)
+
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline int64_t round_up(int64_t a, int64_t b) { return cdiv(a, b) * b; }
 

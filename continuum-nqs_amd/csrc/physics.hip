@@ -176,12 +176,13 @@ __global__ void k_mcmc_propose(int64_t W, int N, const double* __restrict__ R, c
                                double i01, double i10, double i11, double* __restrict__ Rp) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= W * N) return;
-  const double x = __dadd_rn(R[2 * q], __dmul_rn(step, xi[2 * q]));
-  const double y = __dadd_rn(R[2 * q + 1], __dmul_rn(step, xi[2 * q + 1]));
-  const double sx = fmod_pos(__dadd_rn(__dmul_rn(x, i00), __dmul_rn(y, i01)));
-  const double sy = fmod_pos(__dadd_rn(__dmul_rn(x, i10), __dmul_rn(y, i11)));
-  Rp[2 * q] = __dadd_rn(__dmul_rn(sx, a00), __dmul_rn(sy, a01));
-  Rp[2 * q + 1] = __dadd_rn(__dmul_rn(sx, a10), __dmul_rn(sy, a11));
+  // plain operators: the contract(off) region keeps each product and sum rounded on its own
+  const double x = R[2 * q] + step * xi[2 * q];
+  const double y = R[2 * q + 1] + step * xi[2 * q + 1];
+  const double sx = fmod_pos(x * i00 + y * i01);
+  const double sy = fmod_pos(x * i10 + y * i11);
+  Rp[2 * q] = sx * a00 + sy * a01;
+  Rp[2 * q + 1] = sx * a10 + sy * a11;
 }
 
 // Position accept: d = 2 Re L' - logp; accept iff u < exp(min(0, d)).

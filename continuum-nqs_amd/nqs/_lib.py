@@ -37,6 +37,11 @@ class EwaldDesc(ctypes.Structure):
                 ("n_img", c_i32), ("n_g", c_i32), ("img_dev", c_vp), ("g_dev", c_vp), ("rg_dev", c_vp)]
 
 
+class MinsrDesc(ctypes.Structure):
+    _fields_ = [("n_walkers", c_i64), ("n_params", c_i64), ("ldo", c_i64), ("o_dtype", c_i32), ("gram", c_i32),
+                ("refine", c_i32), ("reserved", c_i32), ("lam", c_dbl)]
+
+
 # (name, restype, argtypes) of every exported symbol; the CPU tests check each one resolves.
 SIGNATURES = [
     ("nqs_version", c_i32, []),
@@ -51,6 +56,12 @@ SIGNATURES = [
     ("nqs_psiformer_fwd", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_eloc", c_i32, [ctypes.POINTER(ModelDesc), ctypes.POINTER(HamDesc), c_vp, c_vp, c_vp, c_i64, c_vp,
                          c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_psiformer_grad", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp,
+                                   c_i64, c_vp]),
+    ("nqs_minsr_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
+    ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),
+    ("nqs_minsr_profile", c_i32, [c_i32, ctypes.POINTER(ctypes.c_float)]),
     ("nqs_ewald", c_i32, [ctypes.POINTER(EwaldDesc), c_vp, c_i64, c_i32, c_vp, c_vp]),
     ("nqs_mcmc_sweeps", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_dbl,
                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),

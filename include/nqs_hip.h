@@ -121,6 +121,16 @@ int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void*
              double* logpsi_dev, double* grad_dev, double* lap_dev, void* ws_dev, int64_t ws_bytes,
              void* stream);
 
+/* ---- per-walker gradient of log psi with respect to theta (the O matrix of MinSR)
+ * Hand-written backward: a forward pass keeping only the block inputs, then the reverse sweep
+ * recomputing each block's attention/MLP activations.  O_dev is (2W, ldo) with ldo >= P, in
+ * float64 for NQS_FP64 models and float32 for NQS_MIXED: row w = Re O_w, row W + w = Im O_w,
+ * column k = real_ravel index (complex leaves: re part then im part).  Columns >= P are not
+ * written.  logpsi_dev (W) complex or NULL. */
+int nqs_psiformer_grad(const nqs_model_desc* m, const void* packed_dev, const double* R_dev,
+                       const int32_t* S_dev, int64_t n_walkers, void* O_dev, int64_t ldo,
+                       double* logpsi_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+
 /* ---- Ewald Coulomb energy per walker: energy_dev (W) float64 */
 int nqs_ewald(const nqs_ewald_desc* e, const double* R_dev, int64_t n_walkers, int32_t n_elec,
               double* energy_dev, void* stream);
@@ -134,6 +144,37 @@ int nqs_mcmc_sweeps(const nqs_model_desc* m, const void* packed_dev, double* R_d
                     double step, const double* xi_dev, const double* u_pos_dev,
                     const int32_t* flip_i_dev, const double* u_flip_dev, int32_t* acc_pos_dev,
                     int32_t* acc_spin_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+
+/* ---- MinSR natural-gradient direction (src/nqs/minsr.py:68-81)
+ *   Otil = [Re dO; Im dO] (columns centred over walkers), etil = [Re ec; Im ec],
+ *   dtheta = Otil^T (Otil Otil^T + lam I)^-1 etil
+ * The Gram runs on MFMA in the chosen precision; the shifted Gram is factored by a float64
+ * Cholesky; `refine` steps of iterative refinement use the matrix-free residual
+ * etil - Otil (Otil^T y) - lam y from the float32/float64 O; the push-through reads O. */
+enum { NQS_GRAM_FP64 = 0, NQS_GRAM_FP32 = 1, NQS_GRAM_BF16 = 2 };
+typedef struct nqs_minsr_desc {
+  int64_t n_walkers; /* W (2W must be a multiple of 128 for the bf16 Gram, of 64 otherwise) */
+  int64_t n_params;  /* P */
+  int64_t ldo;       /* row stride of O, >= P, multiple of 64; columns >= P must hold zeros */
+  int32_t o_dtype;   /* NQS_FP64: O is float64; NQS_MIXED: O is float32 */
+  int32_t gram;      /* NQS_GRAM_* (FP64 needs a float64 O) */
+  int32_t refine;    /* iterative refinement steps */
+  int32_t reserved;
+  double lam;        /* diagonal shift added to the unnormalised Gram */
+} nqs_minsr_desc;
+int64_t nqs_minsr_workspace_bytes(const nqs_minsr_desc* d);
+/* O_dev (2W, ldo) is centred in place.  eps_dev (W) complex local energies (already clipped);
+ * dtheta_dev (P) float64; info_dev (int32, device): 0, or 1 + the first non-positive pivot. */
+int nqs_minsr_direction(const nqs_minsr_desc* d, void* O_dev, const double* eps_dev, double* dtheta_dev,
+                        int32_t* info_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+/* Stage timing of nqs_minsr_direction (for measurement, no reference counterpart): enable >= 0
+ * switches hipEvent recording on (1) or off (0) for later calls; ms_out (5 floats, host) receives
+ * the last profiled call's centre, Gram, Cholesky, solve+refinement, push-through times. */
+int nqs_minsr_profile(int32_t enable, float* ms_out);
+/* Median/MAD clip of Re e (minsr.py:145-148), W <= 16384: e_clip_dev (W) complex (Im kept);
+ * stats_dev[4] = mean Re e (unclipped), std(Re e)/sqrt(W), median, MAD. */
+int nqs_clip_energies(const double* e_dev, int64_t n_walkers, double clip_k, double* e_clip_dev,
+                      double* stats_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,142 @@
+"""GPU parity of the hand-written backward (per-walker O) and of the MinSR solve.
+
+O: against the oracle's reverse-mode autodiff (torch.func.grad of Re and Im log psi, the
+restatement of minsr.py:56-65) to 1e-9 relative, and against central finite differences
+(h = 1e-5) of the kernel's own float64 log psi.  MinSR: against the oracle's sample-space LU
+solve and the explicit parameter-space solve (minsr.py:76-88, tests/test_nqs_minsr.py:46-61,
+rel < 1e-6); bf16 / fp32 Gram variants against the float64 one with their stated tolerances.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import bands, psiformer as opf, minsr as oms
+from nqs.psiformer import ContinuumSpinorPsiformer, make_logpsi
+from nqs import minsr as nms
+
+pytestmark = pytest.mark.gpu
+
+ARCHS = {
+    "base": dict(embed_dim=16, depth=2, n_heads=4, n_freq=4),
+    "kdet_jastrow_env": dict(embed_dim=16, depth=2, n_heads=2, n_freq=3, n_dets=3, use_jastrow=True,
+                             learnable_env=True, n_knots=6),
+    "cusp": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=2, use_jastrow=True, jastrow_kind="cusp"),
+}
+
+
+def _pair(name, precision="fp64", N=5, L=7.0, seed=3):
+    g = bands.rashba_gs(L, N, 1.0, 1.0, 0.5, n_cut=10, twist=(0.1, 0.17))
+    kw = ARCHS[name]
+    arch = opf.Arch(N, g["Gs"], g["W"], L, **kw)
+    theta = opf.init_theta(arch, seed, backflow_scale=0.05, scale_all=0.02)
+    model = ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, precision=precision, **kw)
+    return g, arch, theta, model
+
+
+def _configs(W, N, L, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, L, (W, N, 2)), rng.integers(0, 2, (W, N))
+
+
+@pytest.mark.parametrize("name", list(ARCHS))
+def test_O_matches_oracle_autodiff(name):
+    g, arch, theta, model = _pair(name)
+    R, S = _configs(5, 5, 7.0, 21)
+    O = nms.per_sample_O(model, theta.cuda(), None, R, S).cpu().numpy()
+    Oo = oms.per_sample_O(arch, theta, R, S).numpy()
+    assert O.shape == Oo.shape
+    scale = np.max(np.abs(Oo))
+    assert np.max(np.abs(O - Oo)) < 1e-9 * scale
+
+
+def test_O_matches_central_finite_differences():
+    g, arch, theta, model = _pair("kdet_jastrow_env")
+    R, S = _configs(2, 5, 7.0, 22)
+    O = nms.per_sample_O(model, theta.cuda(), None, R, S).cpu().numpy()
+    P = theta.shape[0]
+    h = 1e-5
+    for k in list(range(0, P, max(1, P // 40))) + [P - 1]:
+        e = torch.zeros_like(theta)
+        e[k] = h
+        fp = make_logpsi(model, (theta + e).cuda())(R, S).cpu().numpy()
+        fm = make_logpsi(model, (theta - e).cuda())(R, S).cpu().numpy()
+        fd = (fp - fm) / (2 * h)
+        assert np.max(np.abs(fd - O[:, k])) < 2e-6 * max(1.0, np.max(np.abs(fd)))
+
+
+def test_mixed_precision_O_close_to_fp64():
+    g, arch, theta, m64 = _pair("base")
+    _, _, _, m32 = _pair("base", precision="mixed")
+    R, S = _configs(6, 5, 7.0, 23)
+    O64 = nms.per_sample_O(m64, theta.cuda(), None, R, S).cpu().numpy()
+    O32 = nms.per_sample_O(m32, theta.cuda(), None, R, S).cpu().numpy()
+    assert np.max(np.abs(O32 - O64)) < 1e-4 * np.max(np.abs(O64))
+
+
+def _minsr_case(W=32, seed=4):
+    g, arch, theta, model = _pair("base", seed=seed)
+    R, S = _configs(W, 5, 7.0, seed)
+    O = nms.per_sample_O(model, theta.cuda(), None, R, S)
+    rng = np.random.default_rng(seed)
+    eps = torch.tensor(rng.normal(size=W) + 1j * rng.normal(size=W))
+    return O, eps
+
+
+@pytest.mark.parametrize("lam", [1e-2, 1e-4])
+def test_minsr_matches_oracle_and_paramsr(lam):
+    O, eps = _minsr_case()
+    d = nms.minsr_direction(O, eps, lam, gram="fp64").cpu()
+    assert int(nms.minsr_direction.last_info.item()) == 0
+    d_lu = oms.minsr_direction(O.cpu(), eps, lam)
+    d_ps = oms.paramsr_direction(O.cpu(), eps, lam)
+    den = float(torch.max(torch.abs(d_ps)))
+    assert float(torch.max(torch.abs(d - d_lu))) / den < 1e-6
+    assert float(torch.max(torch.abs(d - d_ps))) / den < 1e-6
+    d_gpu_ps = nms.paramsr_direction(O, eps, lam).cpu()
+    assert float(torch.max(torch.abs(d_gpu_ps - d_ps))) / den < 1e-6
+
+
+@pytest.mark.parametrize("gram,tol", [("fp32", 1e-3), ("bf16", 5e-2)])
+def test_low_precision_gram_close_to_fp64(gram, tol):
+    """bf16 / f32 MFMA Gram (f32 accumulate) + float64 Cholesky + one refinement step from the
+    float32 O: update vector within tol (relative, max norm) of the float64 MinSR."""
+    g, arch, theta, m64 = _pair("base", seed=5)
+    _, _, _, m32 = _pair("base", precision="mixed", seed=5)
+    R, S = _configs(64, 5, 7.0, 5)
+    eps = torch.tensor(np.random.default_rng(5).normal(size=64) + 0j)
+    O64, _ = nms.grad_params(make_logpsi(m64, theta.cuda()), R, S)
+    O32, _ = nms.grad_params(make_logpsi(m32, theta.cuda()), R, S)
+    d64 = nms.minsr_direction(O64, eps, 1e-3, gram="fp64").cpu()
+    d = nms.minsr_direction(O32, eps, 1e-3, gram=gram, refine=1).cpu()
+    rel = float(torch.max(torch.abs(d - d64)) / torch.max(torch.abs(d64)))
+    assert rel < tol, rel
+
+
+def test_clip_matches_oracle():
+    rng = np.random.default_rng(0)
+    for W in (7, 256, 1000):
+        e = rng.normal(size=W) + 1j * rng.normal(size=W)
+        e[::17] += 50.0
+        ec, st = nms.clip_energies(torch.tensor(e), 5.0)
+        ref = oms.clip_energies(e, 5.0)
+        assert np.array_equal(ec.cpu().numpy(), ref), float(np.max(np.abs(ec.cpu().numpy() - ref)))
+        s = st.cpu().numpy()
+        assert abs(s[0] - e.real.mean()) < 1e-12 and abs(s[1] - e.real.std() / np.sqrt(W)) < 1e-12
+
+
+def test_minsr_descends_to_Egs():
+    """tests/test_nqs_minsr.py:64-76 on the HIP path: perturbed backflow, 256 walkers, 80 steps."""
+    N, L, hz = 5, 7.0, 0.5
+    g = bands.rashba_gs(L, N, 1.0, 1.0, hz, n_cut=10, twist=(0.1, 0.17))
+    model = ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, embed_dim=16, depth=2, n_heads=4)
+    params = model.init(0)
+    rng = np.random.default_rng(7)
+    for leaf in ("kernel", "bias"):
+        x = params["params"]["backflow"][leaf]
+        params["params"]["backflow"][leaf] = x + 0.1 * torch.tensor(rng.standard_normal(tuple(x.shape))
+                                                                    + 1j * rng.standard_normal(tuple(x.shape)))
+    params, hist = nms.train_minsr(model, params, L, N, 3, m_star=1.0, lambda_r=1.0, h_z=hz, n_walkers=256,
+                                   n_steps=80, lr=0.1, lam=1e-3, n_sweeps=8, step_size=0.5, burn=40, log_every=20)
+    E_final = float(np.mean(hist["E"][-5:]))
+    assert hist["E"][0] > g["E_gs"] + 1e-2
+    assert g["E_gs"] - 2e-2 < E_final < g["E_gs"] + 1e-2
