@@ -92,9 +92,138 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs<T> g) {
       }
 }
 
+// v2: block tile 128 x 128 x BK, 4 waves in 2 x 2, 64 x 64 per wave (4 x 4 MFMA blocks), operands
+// moved in 16-byte vectors through registers into a double-buffered LDS whose row pitch
+// (BK + 16 bytes) keeps the 16 rows a fragment read touches on distinct banks.
+// Requires lda, ldb multiples of 16 bytes and 16-byte aligned A, B; rows/columns beyond M, N
+// and k beyond K are zero-filled in the tile.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  typedef double2 t;
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+  typedef float4 t;
+  static constexpr int n = 4;
+};
+
+template <typename T, int BK>
+__global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
+  constexpr int BM = 128, VN = Vec16<T>::n, PAD = VN, LD = BK + PAD;
+  constexpr int CH = BK / VN;         // 16-byte chunks per tile row
+  constexpr int PER = 128 * CH / 256; // chunks per thread per operand
+  typedef typename Vec16<T>::t vt;
+  __shared__ T As[2][BM][LD];
+  __shared__ T Bs[2][BM][LD];
+  const int bm = blockIdx.y, bn = blockIdx.x, bz = blockIdx.z;
+  if (g.tri == 1 && bn > bm) return;
+  const T* A = g.A + (int64_t)bz * g.strideA;
+  const T* B = g.B + (int64_t)bz * g.strideB;
+  T* C = g.C + (int64_t)bz * g.strideC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = (int64_t)bm * BM, n0 = (int64_t)bn * BM;
+  typedef typename Mma16<T>::acc_t acc_t;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
+  vt ra[PER], rb[PER];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;
+      const int64_t gk = k0 + ch * VN;
+      const int64_t gm = m0 + row, gn = n0 + row;
+      vt za;
+      vt zb;
+#pragma unroll
+      for (int e = 0; e < VN; ++e) {
+        reinterpret_cast<T*>(&za)[e] = 0;
+        reinterpret_cast<T*>(&zb)[e] = 0;
+      }
+      if (gk + VN <= g.K) {
+        if (gm < g.M) za = *reinterpret_cast<const vt*>(A + gm * g.lda + gk);
+        if (gn < g.N) zb = *reinterpret_cast<const vt*>(B + gn * g.ldb + gk);
+      } else if (gk < g.K) {
+        for (int e = 0; e < VN && gk + e < g.K; ++e) {
+          if (gm < g.M) reinterpret_cast<T*>(&za)[e] = A[gm * g.lda + gk + e];
+          if (gn < g.N) reinterpret_cast<T*>(&zb)[e] = B[gn * g.ldb + gk + e];
+        }
+      }
+      ra[q] = za;
+      rb[q] = zb;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;
+      *reinterpret_cast<vt*>(&As[buf][row][ch * VN]) = ra[q];
+      *reinterpret_cast<vt*>(&Bs[buf][row][ch * VN]) = rb[q];
+    }
+  };
+  const int64_t nk = cdiv_dev(g.K, BK);
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t kb = 0; kb < nk; ++kb) {
+    const int cur = kb & 1;
+    if (kb + 1 < nk) gload((kb + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[cur][wm * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][wn * 64 + j * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mma16<T>::mma(a[i], b[j], acc[i][j]);
+    }
+    if (kb + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r);
+        const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (row >= g.M || col >= g.N) continue;
+        if (g.tri == 1 && col > row) continue;
+        T v = g.alpha * acc[i][j][r];
+        if (g.beta != (T)0) v += g.beta * C[row * g.ldc + col];
+        if (g.bias && (row % g.bias_period) < g.bias_len) v += g.bias[col];
+        C[row * g.ldc + col] = v;
+        if (g.mirror && row != col) C[col * g.ldc + row] = v;
+      }
+}
+
 template <typename T>
 int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return NQS_OK;
+  constexpr int VN = Vec16<T>::n;
+  const bool vec_ok = (g.lda % VN == 0) && (g.ldb % VN == 0) && (((uintptr_t)g.A & 15) == 0) &&
+                      (((uintptr_t)g.B & 15) == 0) && (g.strideA % VN == 0) && (g.strideB % VN == 0);
+  if (vec_ok) {
+    dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
+    if (sizeof(T) == 8)
+      hipLaunchKernelGGL((k_gemm_nt_v2<T, 16>), grid, dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL((k_gemm_nt_v2<T, 32>), grid, dim3(256), 0, st, g);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  }
   dim3 grid((unsigned)cdiv(g.N, 64), (unsigned)cdiv(g.M, 64), (unsigned)batch);
   if (sizeof(T) == 8)
     hipLaunchKernelGGL((k_gemm_nt<T, 16>), grid, dim3(256), 0, st, g);

@@ -155,8 +155,117 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A,
       }
 }
 
+// 256 x 256 x 64 block tile on 8 waves (4 x 2), each owning a 64 x 128 sub-tile (2 x 4 blocks of
+// v_mfma_f32_32x32x16_bf16, 128 accumulators per lane); each thread stages 4 + 4 16-byte vectors
+// per stage, so accumulators, staging and fragments fit in registers at 2 waves per SIMD.
+// Workgroup b runs on XCD b % 8; tiles are dealt so that each XCD walks a contiguous stretch of
+// the row-major lower triangle, whose concurrently resident tiles share their row panel in that
+// XCD's L2.
+#define G2_BM 256
+#define G2_BK 64
+#define G2_LD (G2_BK + 8)
+__global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict__ A, int64_t lda, int64_t K,
+                                                       float* __restrict__ C, int64_t ldc, int64_t ntiles) {
+  extern __shared__ __bf16 g2s[];
+  __bf16(*As)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s);
+  __bf16(*Bs)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s + 2 * G2_BM * G2_LD);
+  const int64_t per_xcd = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  int bm = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(bm + 1) * (bm + 2) / 2 <= t) ++bm;
+  while ((int64_t)bm * (bm + 1) / 2 > t) --bm;
+  const int bn = (int)(t - (int64_t)bm * (bm + 1) / 2);
+  const int64_t m0 = (int64_t)bm * G2_BM, n0 = (int64_t)bn * G2_BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  f16v acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  // staging: thread covers row tid / 8 + 64 q (q = 0..3), 16-byte chunk tid % 8
+  uint4 ra[4], rb[4];
+  const int srow = tid >> 3, scol = (tid & 7) * 8;
+  const __bf16* pa = A + (m0 + srow) * lda + scol;
+  const __bf16* pb = A + (n0 + srow) * lda + scol;
+  const int64_t rstep = 64 * lda;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ra[q] = *reinterpret_cast<const uint4*>(pa + q * rstep);
+    rb[q] = *reinterpret_cast<const uint4*>(pb + q * rstep);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    *reinterpret_cast<uint4*>(&As[0][srow + 64 * q][scol]) = ra[q];
+    *reinterpret_cast<uint4*>(&Bs[0][srow + 64 * q][scol]) = rb[q];
+  }
+  __syncthreads();
+  const int64_t nk = K / G2_BK;
+  for (int64_t kb = 0; kb < nk; ++kb) {
+    const int cur = kb & 1;
+    // prefetch the next stage unconditionally (the last iteration re-reads its own stage into
+    // the idle buffer), so the staging arrays never sit under a branch and stay in registers
+    {
+      const int64_t k0 = (kb + 1 < nk ? kb + 1 : kb) * G2_BK;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ra[q] = *reinterpret_cast<const uint4*>(pa + q * rstep + k0);
+        rb[q] = *reinterpret_cast<const uint4*>(pb + q * rstep + k0);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < G2_BK; kk += 16) {
+      bf8v a[2], b[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf8v*>(&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        b[j] = *reinterpret_cast<const bf8v*>(&Bs[cur][wn * 128 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      *reinterpret_cast<uint4*>(&As[cur ^ 1][srow + 64 * q][scol]) = ra[q];
+      *reinterpret_cast<uint4*>(&Bs[cur ^ 1][srow + 64 * q][scol]) = rb[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = n0 + wn * 128 + j * 32 + (lane & 31);
+        if (row >= col) C[row * ldc + col] = acc[i][j][r];
+      }
+}
+
 int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st) {
   NQS_REQUIRE(n % GB_BM == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs 2W to be a multiple of 128");
+  if (n % G2_BM == 0 && K % G2_BK == 0 && lda % 8 == 0 && ((uintptr_t)A & 15) == 0) {
+    const int64_t nt = n / G2_BM;
+    const int64_t tiles = nt * (nt + 1) / 2;
+    const size_t lds = sizeof(__bf16) * 4 * G2_BM * G2_LD;
+    static bool attr = false;
+    if (!attr) {
+      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_syrk_bf16_256, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(512), lds, st, A, lda, K, C, ldc,
+                       tiles);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  }
   NQS_REQUIRE(K % GB_BK == 0 && lda % 8 == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs K % 64 == 0 and lda % 8 == 0");
   NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
   const int64_t nt = n / GB_BM;
@@ -222,19 +331,22 @@ __global__ __launch_bounds__(256) void k_defl_rows(const double* __restrict__ A,
     ts[n + r] = b * s;
   }
 }
-// ... plus the column part (r > c), one thread per column, coalesced over consecutive columns
+// ... plus the column part (r > c): one thread per column and a 256-row slab per blockIdx.y
+// (coalesced over consecutive columns), slabs combined with float64 atomics
 __global__ void k_defl_cols(const double* __restrict__ A, int64_t lda, int64_t W, double* __restrict__ ts) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = 2 * W;
-  if (c >= n) return;
+  const int64_t r0 = (int64_t)blockIdx.y * 256;
+  if (c >= n || r0 + 255 <= c) return;
   double su = 0, sv = 0;
-  for (int64_t r = c + 1; r < n; ++r) {
+  const int64_t r1 = r0 + 256 < n ? r0 + 256 : n;
+  for (int64_t r = (c + 1 > r0 ? c + 1 : r0); r < r1; ++r) {
     const double a = A[r * lda + c];
     if (r < W) su += a; else sv += a;
   }
   const double s = 1.0 / sqrt((double)W);
-  ts[c] += su * s;
-  ts[n + c] += sv * s;
+  if (su != 0.0) atomicAdd(&ts[c], su * s);
+  if (sv != 0.0) atomicAdd(&ts[n + c], sv * s);
 }
 // scalars: alpha = u.Tu, beta = u.Tv, gamma = v.Tv, c = mean diagonal (one workgroup)
 __global__ void k_defl_scalars(const double* __restrict__ A, int64_t lda, int64_t W, const double* __restrict__ ts,
@@ -295,7 +407,8 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
   const int64_t n = 2 * W;
   hipLaunchKernelGGL(k_defl_rows, dim3((unsigned)n), dim3(256), 0, st, A, lda, W, ts);
   NQS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_defl_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, A, lda, W, ts);
+  hipLaunchKernelGGL(k_defl_cols, dim3((unsigned)cdiv(n, 256), (unsigned)cdiv(n, 256)), dim3(256), 0, st, A, lda, W,
+                     ts);
   NQS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_defl_scalars, dim3(1), dim3(1024), 0, st, A, lda, W, ts, sc);
   NQS_LAUNCH_CHECK();

@@ -47,6 +47,7 @@ void set_error(const std::string& msg);
   } while (0)
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__device__ inline int64_t cdiv_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline int64_t round_up(int64_t a, int64_t b) { return cdiv(a, b) * b; }
 
 // ---- complex double arithmetic on a plain struct (no std::complex on device)
