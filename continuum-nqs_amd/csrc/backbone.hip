@@ -9,6 +9,7 @@
 //   a*b:       d(ab) = a db + b da,    Lap(ab) = a Lap b + b Lap a + 2 sum_c da_c db_c
 // Reference ops (src/nqs/psiformer.py): features 84-97, LayerNorm 76-77,136 (flax eps 1e-6,
 // fast variance), attention 50-64, tanh-GELU 79.
+#include <algorithm>
 #include "nqs_common.h"
 #include "kernels.h"
 
@@ -258,13 +259,25 @@ int launch_gelu_jet(const JetGeom& j, int width, const T* x, T* y, hipStream_t s
 //   Lap S = (LapQ K^T + Q LapK^T + 2 sum_c dQ_c dK_c^T) sc
 //   Lap P = P (Lap S - rowdot(P, Lap S) + sum_c [(dS_c - abar_c)^2 - rowdot(P, (dS_c - abar_c)^2)])
 //   dO_c = dP_c V + P dV_c,   Lap O = Lap P V + P Lap V + 2 sum_c dP_c dV_c
+// Wave-level variant: one 64-lane wave per (cfg, head), `wpg` waves per workgroup, each with its
+// own LDS region; phases inside a wave are ordered by wave-scope fences (LDS operations of one
+// wave complete in issue order), so no workgroup barrier is needed.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh, const T* __restrict__ qkv,
-                                                       T* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_attention_jet_wave(JetGeom j, int d, int nh, const T* __restrict__ qkv,
+                                                            T* __restrict__ out, int wpg, int64_t npairs) {
   extern __shared__ char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
   const int N = j.N, hd = d / nh;
   const int NN = N * N, NH = N * hd;
+  const int per_wave = 7 * NH + 5 * NN + 2 * N;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t pair = (int64_t)blockIdx.x * wpg + wave;
+  if (wave >= wpg || pair >= npairs) return;
+  T* sm = reinterpret_cast<T*>(smem_raw) + (size_t)wave * per_wave;
   T* Q0 = sm;
   T* K0 = Q0 + NH;
   T* V0 = K0 + NH;
@@ -276,39 +289,35 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
   T* accS = Sc + NN;
   T* accP = accS + NN;
   T* accPV = accP + NN;
-  T* ab = accPV + NH;  // N
-  T* ab2 = ab + N;     // N
-
-  const int64_t cfg = blockIdx.x / nh;
-  const int h = blockIdx.x % nh;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t rs = 3 * (int64_t)d;   // qkv row stride
+  T* ab = accPV + NH;
+  T* ab2 = ab + N;
+  const int64_t cfg = pair / nh;
+  const int h = (int)(pair % nh);
+  const int64_t rs = 3 * (int64_t)d;
   const int64_t chs_in = (int64_t)N * rs;
   const int64_t chs_out = (int64_t)N * d;
   const T* in0 = qkv + cfg * j.C * chs_in;
   T* out0 = out + cfg * j.C * chs_out;
   const T sc = (T)1 / sqrt((T)hd);
-
   auto load = [&](int c, T* q, T* k, T* v) {
     const T* b = in0 + c * chs_in;
-    for (int e = tid; e < NH; e += nthr) {
+    for (int e = lane; e < NH; e += 64) {
       const int t = e / hd, q_ = e % hd;
       q[e] = b[t * rs + h * hd + q_];
       k[e] = b[t * rs + d + h * hd + q_];
       v[e] = b[t * rs + 2 * d + h * hd + q_];
     }
   };
-  // value channel
   load(0, Q0, K0, V0);
-  __syncthreads();
-  for (int e = tid; e < NN; e += nthr) {
+  wave_sync();
+  for (int e = lane; e < NN; e += 64) {
     const int a = e / N, b = e % N;
     T s = 0;
     for (int q = 0; q < hd; ++q) s += Q0[a * hd + q] * K0[b * hd + q];
     P[e] = s * sc;
   }
-  __syncthreads();
-  for (int a = tid; a < N; a += nthr) {
+  wave_sync();
+  for (int a = lane; a < N; a += 64) {
     T mx = P[a * N];
     for (int b = 1; b < N; ++b) mx = P[a * N + b] > mx ? P[a * N + b] : mx;
     T z = 0;
@@ -320,24 +329,23 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
     const T iz = (T)1 / z;
     for (int b = 0; b < N; ++b) P[a * N + b] *= iz;
   }
-  __syncthreads();
-  for (int e = tid; e < NH; e += nthr) {
+  wave_sync();
+  for (int e = lane; e < NH; e += 64) {
     const int a = e / hd, q = e % hd;
     T o = 0;
     for (int b = 0; b < N; ++b) o += P[a * N + b] * V0[b * hd + q];
     out0[a * d + h * hd + q] = o;
   }
   if (j.nt == 0 && !j.has_lap) return;
-  for (int e = tid; e < NN; e += nthr) {
+  for (int e = lane; e < NN; e += 64) {
     accS[e] = 0;
     accP[e] = 0;
   }
-  for (int e = tid; e < NH; e += nthr) accPV[e] = 0;
-  __syncthreads();
+  for (int e = lane; e < NH; e += 64) accPV[e] = 0;
   for (int c = 1; c <= j.nt; ++c) {
     load(c, Qc, Kc, Vc);
-    __syncthreads();
-    for (int e = tid; e < NN; e += nthr) {
+    wave_sync();
+    for (int e = lane; e < NN; e += 64) {
       const int a = e / N, b = e % N;
       T s = 0, s2 = 0;
       for (int q = 0; q < hd; ++q) {
@@ -347,8 +355,8 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
       Sc[e] = s * sc;
       accS[e] += s2;
     }
-    __syncthreads();
-    for (int a = tid; a < N; a += nthr) {
+    wave_sync();
+    for (int a = lane; a < N; a += 64) {
       T m = 0;
       for (int b = 0; b < N; ++b) m += P[a * N + b] * Sc[a * N + b];
       T m2 = 0;
@@ -359,16 +367,16 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
       ab[a] = m;
       ab2[a] = m2;
     }
-    __syncthreads();
-    for (int e = tid; e < NN; e += nthr) {
+    wave_sync();
+    for (int e = lane; e < NN; e += 64) {
       const int a = e / N;
       const T u = Sc[e] - ab[a];
       accP[e] += u * u - ab2[a];
-      Sc[e] = P[e] * u;  // dP_c
+      Sc[e] = P[e] * u;
     }
-    __syncthreads();
+    wave_sync();
     T* oc = out0 + c * chs_out;
-    for (int e = tid; e < NH; e += nthr) {
+    for (int e = lane; e < NH; e += 64) {
       const int a = e / hd, q = e % hd;
       T o = 0, o2 = 0;
       for (int b = 0; b < N; ++b) {
@@ -378,32 +386,32 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
       oc[a * d + h * hd + q] = o;
       accPV[e] += o2;
     }
-    __syncthreads();
+    wave_sync();
   }
   if (!j.has_lap) return;
   const int cl = j.nt + 1;
   load(cl, Qc, Kc, Vc);
-  __syncthreads();
-  for (int e = tid; e < NN; e += nthr) {
+  wave_sync();
+  for (int e = lane; e < NN; e += 64) {
     const int a = e / N, b = e % N;
     T s = 0;
     for (int q = 0; q < hd; ++q) s += Qc[a * hd + q] * K0[b * hd + q] + Q0[a * hd + q] * Kc[b * hd + q];
-    Sc[e] = (s + (T)2 * accS[e]) * sc;  // Lap S
+    Sc[e] = (s + (T)2 * accS[e]) * sc;
   }
-  __syncthreads();
-  for (int a = tid; a < N; a += nthr) {
+  wave_sync();
+  for (int a = lane; a < N; a += 64) {
     T m = 0;
     for (int b = 0; b < N; ++b) m += P[a * N + b] * Sc[a * N + b];
     ab[a] = m;
   }
-  __syncthreads();
-  for (int e = tid; e < NN; e += nthr) {
+  wave_sync();
+  for (int e = lane; e < NN; e += 64) {
     const int a = e / N;
-    Sc[e] = P[e] * (Sc[e] - ab[a] + accP[e]);  // Lap P
+    Sc[e] = P[e] * (Sc[e] - ab[a] + accP[e]);
   }
-  __syncthreads();
+  wave_sync();
   T* oc = out0 + cl * chs_out;
-  for (int e = tid; e < NH; e += nthr) {
+  for (int e = lane; e < NH; e += 64) {
     const int a = e / hd, q = e % hd;
     T o = 0;
     for (int b = 0; b < N; ++b) o += Sc[a * N + b] * V0[b * hd + q] + P[a * N + b] * Vc[b * hd + q];
@@ -413,12 +421,24 @@ __global__ __launch_bounds__(256) void k_attention_jet(JetGeom j, int d, int nh,
 
 template <typename T>
 int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* out, hipStream_t st) {
-  const int64_t blocks = j.n_cfg * n_heads;
-  if (blocks == 0) return NQS_OK;
+  const int64_t npairs = j.n_cfg * n_heads;
+  if (npairs == 0) return NQS_OK;
   const int N = j.N, hd = d / n_heads;
-  const size_t lds = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
-  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");
-  hipLaunchKernelGGL(k_attention_jet<T>, dim3((unsigned)blocks), dim3(256), lds, st, j, d, n_heads, qkv, out);
+  const size_t per_wave = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
+  NQS_REQUIRE(per_wave <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");
+  int wpg = (int)std::min<size_t>(4, (64 * 1024) / per_wave);
+  if (wpg < 1) wpg = 1;
+  const size_t lds = per_wave * wpg;
+  if (lds > 64 * 1024) {
+    static bool attr[2] = {false, false};
+    if (!attr[sizeof(T) == 8]) {
+      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_attention_jet_wave<T>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      attr[sizeof(T) == 8] = true;
+    }
+  }
+  hipLaunchKernelGGL(k_attention_jet_wave<T>, dim3((unsigned)cdiv(npairs, wpg)), dim3(64 * wpg), lds, st, j, d,
+                     n_heads, qkv, out, wpg, npairs);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

@@ -101,12 +101,12 @@ template <typename T>
 struct Vec16;
 template <>
 struct Vec16<double> {
-  typedef double2 t;
+  typedef double t __attribute__((ext_vector_type(2)));
   static constexpr int n = 2;
 };
 template <>
 struct Vec16<float> {
-  typedef float4 t;
+  typedef float t __attribute__((ext_vector_type(4)));
   static constexpr int n = 4;
 };
 
@@ -134,48 +134,37 @@ __global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
+  // Branch-free staging: addresses are clamped into the matrices and out-of-range vectors are
+  // replaced by zeros with a select, so the loads of a stage issue back to back and the
+  // staging arrays stay in registers.  K is a multiple of VN (checked by the launcher).
   vt ra[PER], rb[PER];
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;
-      const int64_t gk = k0 + ch * VN;
-      const int64_t gm = m0 + row, gn = n0 + row;
-      vt za;
-      vt zb;
-#pragma unroll
-      for (int e = 0; e < VN; ++e) {
-        reinterpret_cast<T*>(&za)[e] = 0;
-        reinterpret_cast<T*>(&zb)[e] = 0;
-      }
-      if (gk + VN <= g.K) {
-        if (gm < g.M) za = *reinterpret_cast<const vt*>(A + gm * g.lda + gk);
-        if (gn < g.N) zb = *reinterpret_cast<const vt*>(B + gn * g.ldb + gk);
-      } else if (gk < g.K) {
-        for (int e = 0; e < VN && gk + e < g.K; ++e) {
-          if (gm < g.M) reinterpret_cast<T*>(&za)[e] = A[gm * g.lda + gk + e];
-          if (gn < g.N) reinterpret_cast<T*>(&zb)[e] = B[gn * g.ldb + gk + e];
-        }
-      }
-      ra[q] = za;
-      rb[q] = zb;
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;
-      *reinterpret_cast<vt*>(&As[buf][row][ch * VN]) = ra[q];
-      *reinterpret_cast<vt*>(&Bs[buf][row][ch * VN]) = rb[q];
-    }
-  };
+  const vt zero = (vt)(T)0;
   const int64_t nk = cdiv_dev(g.K, BK);
-  gload(0);
-  sstore(0);
+  // raw loads here; the out-of-range zeroing happens at the LDS store, so no wait is forced
+  // between issuing a stage's loads and the MFMAs of the current stage
+#define NQS_GEMM_LOAD(k0)                                                                         \
+  _Pragma("unroll") for (int q = 0; q < PER; ++q) {                                               \
+    const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;                                 \
+    const int64_t gk = (k0) + ch * VN;                                                            \
+    const int64_t gm = m0 + row, gn = n0 + row;                                                   \
+    const int64_t ck = gk < g.K ? gk : g.K - VN;                                                  \
+    ra[q] = *reinterpret_cast<const vt*>(A + (gm < g.M ? gm : g.M - 1) * g.lda + ck);             \
+    rb[q] = *reinterpret_cast<const vt*>(B + (gn < g.N ? gn : g.N - 1) * g.ldb + ck);             \
+  }
+#define NQS_GEMM_STORE(buf, k0)                                                                   \
+  _Pragma("unroll") for (int q = 0; q < PER; ++q) {                                               \
+    const int idx = tid + 256 * q, row = idx / CH, ch = idx % CH;                                 \
+    const bool okk = (k0) + ch * VN < g.K;                                                        \
+    *reinterpret_cast<vt*>(&As[buf][row][ch * VN]) = (okk && m0 + row < g.M) ? ra[q] : zero;      \
+    *reinterpret_cast<vt*>(&Bs[buf][row][ch * VN]) = (okk && n0 + row < g.N) ? rb[q] : zero;      \
+  }
+  NQS_GEMM_LOAD(0)
+  NQS_GEMM_STORE(0, 0)
   __syncthreads();
   for (int64_t kb = 0; kb < nk; ++kb) {
     const int cur = kb & 1;
-    if (kb + 1 < nk) gload((kb + 1) * BK);
+    const int64_t knext = (kb + 1 < nk ? kb + 1 : kb) * BK;
+    NQS_GEMM_LOAD(knext)
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T a[4], b[4];
@@ -188,8 +177,47 @@ __global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Mma16<T>::mma(a[i], b[j], acc[i][j]);
     }
-    if (kb + 1 < nk) sstore(cur ^ 1);
+    NQS_GEMM_STORE(cur ^ 1, knext)
     __syncthreads();
+  }
+#undef NQS_GEMM_LOAD
+#undef NQS_GEMM_STORE
+  // Epilogue in two phases: every C / bias operand is read first (clamped addresses, uniform
+  // branches only), then the tile is combined and stored under the per-element predicate.
+  const bool has_beta = g.beta != (T)0, has_bias = g.bias != nullptr;
+  T addv[4][4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) addv[i][j][r] = 0;
+  if (has_beta) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r);
+          const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+          const int64_t rr = row < g.M ? row : g.M - 1, cc = col < g.N ? col : g.N - 1;
+          addv[i][j][r] = g.beta * C[rr * g.ldc + cc];
+        }
+  }
+  if (has_bias) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+      const T bv = g.bias[col < g.N ? col : g.N - 1];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t row = (uint32_t)(m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r));
+          addv[i][j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv : (T)0;
+        }
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -199,13 +227,12 @@ __global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r);
         const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (row >= g.M || col >= g.N) continue;
-        if (g.tri == 1 && col > row) continue;
-        T v = g.alpha * acc[i][j][r];
-        if (g.beta != (T)0) v += g.beta * C[row * g.ldc + col];
-        if (g.bias && (row % g.bias_period) < g.bias_len) v += g.bias[col];
-        C[row * g.ldc + col] = v;
-        if (g.mirror && row != col) C[col * g.ldc + row] = v;
+        const bool in = row < g.M && col < g.N && !(g.tri == 1 && col > row);
+        const T v = g.alpha * acc[i][j][r] + addv[i][j][r];
+        if (in) {
+          C[row * g.ldc + col] = v;
+          if (g.mirror && row != col) C[col * g.ldc + row] = v;
+        }
       }
 }
 
@@ -213,7 +240,7 @@ template <typename T>
 int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return NQS_OK;
   constexpr int VN = Vec16<T>::n;
-  const bool vec_ok = (g.lda % VN == 0) && (g.ldb % VN == 0) && (((uintptr_t)g.A & 15) == 0) &&
+  const bool vec_ok = (g.K % VN == 0) && (g.lda % VN == 0) && (g.ldb % VN == 0) && (((uintptr_t)g.A & 15) == 0) &&
                       (((uintptr_t)g.B & 15) == 0) && (g.strideA % VN == 0) && (g.strideB % VN == 0);
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
