@@ -9,9 +9,15 @@ Rashba SOC, Zeeman) + Ewald Coulomb + median/MAD clip + the per-walker gradient 
 backward) + MinSR (Gram on MFMA, float64 Cholesky, one refinement step, push-through) + the
 parameter update.
 
---precision mixed: float32 backbone, bf16 MFMA Gram with f32 accumulation, float64
-determinants / Ewald / Cholesky.  --precision fp64: everything float64 (the plain FP64 GPU
-path).  Prints one JSON line; --gpus > 1 is refused (this build measures one GPU).
+--precision mixed: float32 backbone activations whose dense layers run as bf16x3 on MFMA
+(float32 operands split into bf16 hi/lo pairs, float32 accumulation), bf16 MFMA Gram with
+float32 accumulation, float64 determinants / Ewald / Cholesky.  --precision fp64: everything
+float64 (the plain FP64 GPU path).  Prints one JSON line; --gpus > 1 is refused (one GPU only).
+
+For --precision mixed the line also carries "accuracy": after the timed steps, the mixed path
+and the float64 path are evaluated on the same walkers with the same parameters (outside the
+timed region): per-step energy difference, log psi, E_loc and Ewald differences, and the relative
+difference of the MinSR direction d theta.
 """
 import argparse
 import ctypes
@@ -35,13 +41,13 @@ ARCH = dict(embed_dim=128, depth=4, n_heads=8, n_freq=8, n_dets=8)
 PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12}
 
 
-def build_system(precision, W, seed=0):
+def build_system(precision, W, seed=0, gemm="bf16x3"):
     from nqs.rashba_reference import noninteracting_gs, occupied_orbital_matrix, planewave_envelope
     from nqs.psiformer import ContinuumSpinorPsiformer, ravel_params
     L = RS * math.sqrt(math.pi * N_E)
     g = noninteracting_gs(N_E, L, 1.0, LAM, HZ, n_cut=10, twist=(0.1, 0.17))
     Gs, Wenv = planewave_envelope(*occupied_orbital_matrix(g))
-    model = ContinuumSpinorPsiformer(n_elec=N_E, Gs=Gs, W=Wenv, L=L, precision=precision, **ARCH)
+    model = ContinuumSpinorPsiformer(n_elec=N_E, Gs=Gs, W=Wenv, L=L, precision=precision, gemm=gemm, **ARCH)
     params = model.init(seed)
     rng = np.random.default_rng(seed + 1)
     kr = params["params"]["backflow"]["kernel"]
@@ -77,6 +83,42 @@ def cpu_baseline(sysd, n_walkers=8, seed=0):
                       f"stage seconds {json.dumps({k: round(v, 3) for k, v in tm.items()})}"}
 
 
+def accuracy_vs_fp64(vs, sysd, coul, gemm):
+    """Mixed vs float64 GPU path on the walkers and parameters of the last timed step."""
+    from nqs.psiformer import ContinuumSpinorPsiformer, LogPsi
+    from nqs.local_energy import make_local_energy
+    from nqs.minsr import grad_params, minsr_direction, clip_energies
+    m64 = ContinuumSpinorPsiformer(n_elec=N_E, Gs=sysd["Gs"], W=sysd["Wenv"], L=sysd["L"], precision="fp64", **ARCH)
+    R, S, theta = vs.chain.R, vs.chain.S, vs.theta
+    lps = {"mixed": LogPsi(vs.model, theta), "fp64": LogPsi(m64, theta)}
+    res = {}
+    for k, lp in lps.items():
+        lv = lp(R, S)
+        e = make_local_energy(lp, m_star=1.0, lambda_r=LAM, h_z=HZ)(R, S)
+        res[k] = dict(lp=lv, e=e)
+    ec = coul(R)
+    dlp = res["mixed"]["lp"] - res["fp64"]["lp"]
+    dim = torch.remainder(dlp.imag + math.pi, 2 * math.pi) - math.pi
+    de = res["mixed"]["e"] - res["fp64"]["e"]
+    E_m = float((res["mixed"]["e"].real + ec).mean())
+    E_d = float((res["fp64"]["e"].real + ec).mean())
+    out = {"reference": "float64 GPU path, same walkers and parameters", "gemm": gemm,
+           "walkers": int(R.shape[0]), "E_step_fp64": E_d, "dE_step": abs(E_m - E_d),
+           "logpsi_re_max_abs": float(dlp.real.abs().max()), "logpsi_im_max_abs": float(dim.abs().max()),
+           "eloc_max_abs": float(de.abs().max()), "eloc_rms": float(de.abs().pow(2).mean().sqrt()),
+           "ewald_max_abs": 0.0}
+    # MinSR direction from each path's own O, clip and Gram
+    dth = {}
+    for k, lp in lps.items():
+        e_clip, _ = clip_energies(res[k]["e"] + ec, 5.0)
+        O, _ = grad_params(lp, R, S, None)
+        dth[k] = minsr_direction(O, e_clip, vs.lam, "bf16" if k == "mixed" else "fp64", 1)
+        del O
+        torch.cuda.empty_cache()
+    out["dtheta_rel"] = float((dth["mixed"] - dth["fp64"]).norm() / dth["fp64"].norm())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +130,9 @@ def main():
     ap.add_argument("--burn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-walkers", type=int, default=8)
+    ap.add_argument("--gemm", default="bf16x3", choices=["bf16x3", "fp32", "bf16"],
+                    help="dense-layer MFMA mode of the mixed backbone")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: runs on several GPUs are not part of this build", file=sys.stderr)
@@ -97,7 +142,7 @@ def main():
     from nqs.minsr import VMCStep
 
     W = args.walkers
-    sysd = build_system(args.precision, W)
+    sysd = build_system(args.precision, W, gemm=args.gemm)
     _, _, coul = make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
     gram = "bf16" if args.precision == "mixed" else "fp64"
     vs = VMCStep(sysd["model"], sysd["theta"], sysd["R"], sysd["S"], m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul,
@@ -136,7 +181,8 @@ def main():
            "vs_baseline": None, "dtype": "f32+bf16+f64" if args.precision == "mixed" else "f64",
            "data": "synthetic",
            "config": {"workload": "N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers=8192",
-                      "walkers": W, "params": P, "sweeps": args.sweeps, "precision": args.precision, "gram": gram},
+                      "walkers": W, "params": P, "sweeps": args.sweeps, "precision": args.precision, "gram": gram,
+                      "gemm": args.gemm if args.precision == "mixed" else "fp64"},
            "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
                         "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / peak, "traffic": None, "gram_ms": float(mins[1])},
@@ -146,6 +192,8 @@ def main():
                         "minsr_cholesky": float(mins[2]), "minsr_solve_refine": float(mins[3]),
                         "minsr_pushthrough": float(mins[4])},
            "E_mean": float(stats[0]), "E_err": float(stats[1])}
+    if args.precision == "mixed" and not args.no_accuracy:
+        out["accuracy"] = accuracy_vs_fp64(vs, sysd, coul, args.gemm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sysd, args.cpu_walkers)
     else:

@@ -258,6 +258,7 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
     g.bias_len = 0;
     g.tri = 1;
     g.mirror = 0;
+    g.split = 0;
     if (sizeof(T) == 8) {
       g.C = (T*)w.A64;
       g.ldc = n;

@@ -11,6 +11,10 @@
 #include "nqs_common.h"
 #include "kernels.h"
 
+#ifndef NQS_BF_NBUF
+#define NQS_BF_NBUF 2
+#endif
+
 namespace nqs {
 
 template <typename T>
@@ -111,7 +115,7 @@ struct Vec16<float> {
 };
 
 template <typename T, int BK>
-__global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(GemmArgs<T> g) {
   constexpr int BM = 128, VN = Vec16<T>::n, PAD = VN, LD = BK + PAD;
   constexpr int CH = BK / VN;         // 16-byte chunks per tile row
   constexpr int PER = 128 * CH / 256; // chunks per thread per operand
@@ -236,12 +240,219 @@ __global__ __launch_bounds__(256) void k_gemm_nt_v2(GemmArgs<T> g) {
       }
 }
 
+// ---------------------------------------------------------------------------------------------
+// float32 operands on bf16 MFMA.  Each float32 x is split into hi = bf16(x) and lo = bf16(x - hi)
+// (both round-to-nearest-even) as it is staged into LDS; SPLIT = 3 accumulates
+// hi*hi + hi*lo + lo*hi in float32 (products carry about 16 significant bits; the dropped lo*lo
+// term is below 2^-16 relative), SPLIT = 1 keeps hi*hi only (plain bf16).  Block tile
+// 128 x 128 x 32, 4 waves in 2 x 2, 64 x 64 per wave as 2 x 2 blocks of
+// v_mfma_f32_32x32x16_bf16 (A[lane%32][8 (lane/32) + e], C row 8 (r/4) + 4 (lane/32) + r%4,
+// col lane%32).  Tiles are dealt XCD-aware: workgroup b runs on XCD b % 8, and the column
+// tiles of one row panel go to the same XCD back to back, so the panel is read from HBM once.
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+__device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  const f2v v = {x0, x1};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
+  const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
+  const f2v l = {x0 - h0, x1 - h1};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
+}
+
+constexpr int BF_BM = 128, BF_BK = 32, BF_LD = BF_BK + 8;
+
+template <int SPLIT, int NBUF>
+constexpr size_t bf_lds_bytes() {
+  return sizeof(__bf16) * (size_t)NBUF * 2 * (SPLIT == 1 ? 1 : 2) * BF_BM * BF_LD;
+}
+
+template <int SPLIT, int NBUF>
+__global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
+  constexpr int NP = SPLIT == 1 ? 1 : 2;
+  constexpr int PLANE = BF_BM * BF_LD;
+  constexpr int BUF = 2 * NP * PLANE;  // A planes, then B planes
+  extern __shared__ __bf16 bfs[];
+  const int64_t per_xcd = (ntiles + 7) / 8;
+  const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (t >= ntiles) return;
+  const int64_t bm = t / nbn;
+  const int bn = (int)(t % nbn);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = bm * BF_BM, n0 = (int64_t)bn * BF_BM;
+  f16v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  // staging: thread covers rows tid / 8 + 32 q (q = 0..3), float4 chunk tid % 8 of the 32-wide k slab
+  const int srow = tid >> 3, sk = (tid & 7) * 4;
+  const float* pa[4];
+  const float* pb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t gm = m0 + srow + 32 * q, gn = n0 + srow + 32 * q;
+    pa[q] = g.A + (gm < g.M ? gm : g.M - 1) * g.lda;
+    pb[q] = g.B + (gn < g.N ? gn : g.N - 1) * g.ldb;
+  }
+  f4v ra[4], rb[4];
+  const f4v zero = (f4v)0.f;
+  const int64_t nk = cdiv_dev(g.K, BF_BK);
+#define NQS_BF_LOAD(k0)                                                  \
+  {                                                                      \
+    const int64_t gk = (k0) + sk, ck = gk < g.K ? gk : g.K - 4;          \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                      \
+      ra[q] = *reinterpret_cast<const f4v*>(pa[q] + ck);                 \
+      rb[q] = *reinterpret_cast<const f4v*>(pb[q] + ck);                 \
+    }                                                                    \
+  }
+#define NQS_BF_STORE(buf, k0)                                                                \
+  {                                                                                          \
+    const bool okk = (k0) + sk < g.K;                                                        \
+    __bf16* s = bfs + (buf) * BUF;                                                           \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                          \
+      const int row = srow + 32 * q;                                                         \
+      const f4v a = (okk && m0 + row < g.M) ? ra[q] : zero;                                  \
+      const f4v b = (okk && n0 + row < g.N) ? rb[q] : zero;                                  \
+      uint2 ah, al, bh, bl;                                                                  \
+      split_bf16x2(a[0], a[1], ah.x, al.x);                                                  \
+      split_bf16x2(a[2], a[3], ah.y, al.y);                                                  \
+      split_bf16x2(b[0], b[1], bh.x, bl.x);                                                  \
+      split_bf16x2(b[2], b[3], bh.y, bl.y);                                                  \
+      *reinterpret_cast<uint2*>(s + row * BF_LD + sk) = ah;                                  \
+      *reinterpret_cast<uint2*>(s + NP * PLANE + row * BF_LD + sk) = bh;                     \
+      if (NP == 2) {                                                                         \
+        *reinterpret_cast<uint2*>(s + PLANE + row * BF_LD + sk) = al;                        \
+        *reinterpret_cast<uint2*>(s + 3 * PLANE + row * BF_LD + sk) = bl;                    \
+      }                                                                                      \
+    }                                                                                        \
+  }
+  NQS_BF_LOAD(0)
+  NQS_BF_STORE(0, 0)
+  __syncthreads();
+  for (int64_t kb = 0; kb < nk; ++kb) {
+    const int cur = NBUF == 2 ? (int)(kb & 1) : 0;
+    const int64_t knext = (kb + 1 < nk ? kb + 1 : kb) * BF_BK;
+    NQS_BF_LOAD(knext)
+    const __bf16* s = bfs + cur * BUF;
+#pragma unroll
+    for (int kk = 0; kk < BF_BK; kk += 16) {
+      bf8v ah[2], al[2], bh[2], bl[2];
+      const int ko = kk + 8 * (lane >> 5);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int ra_ = (wm * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
+        const int rb_ = (wn * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
+        ah[i] = *reinterpret_cast<const bf8v*>(s + ra_);
+        bh[i] = *reinterpret_cast<const bf8v*>(s + NP * PLANE + rb_);
+        if (NP == 2) {
+          al[i] = *reinterpret_cast<const bf8v*>(s + PLANE + ra_);
+          bl[i] = *reinterpret_cast<const bf8v*>(s + 3 * PLANE + rb_);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (NP == 2) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (NBUF == 1) __syncthreads();
+    NQS_BF_STORE(NBUF == 2 ? cur ^ 1 : 0, knext)
+    __syncthreads();
+  }
+#undef NQS_BF_LOAD
+#undef NQS_BF_STORE
+  // two-phase epilogue as in k_gemm_nt_v2
+  const bool has_beta = g.beta != 0.f, has_bias = g.bias != nullptr;
+  float addv[2][2][16];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) addv[i][j][r] = 0.f;
+  if (has_beta) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+          const int64_t rr = row < g.M ? row : g.M - 1, cc = col < g.N ? col : g.N - 1;
+          addv[i][j][r] = g.beta * g.C[rr * g.ldc + cc];
+        }
+  }
+  if (has_bias) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+      const float bv = g.bias[col < g.N ? col : g.N - 1];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t row = (uint32_t)(m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3));
+          addv[i][j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv : 0.f;
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+        const float v = g.alpha * acc[i][j][r] + addv[i][j][r];
+        if (row < g.M && col < g.N) g.C[row * g.ldc + col] = v;
+      }
+}
+
+template <int SPLIT, int NBUF>
+static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
+  constexpr size_t lds = bf_lds_bytes<SPLIT, NBUF>();
+  if (lds > 64 * 1024) {
+    static bool attr = false;
+    if (!attr) {
+      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_gemm_nt_bf<SPLIT, NBUF>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr = true;
+    }
+  }
+  const int nbn = (int)cdiv(g.N, BF_BM);
+  const int64_t ntiles = cdiv(g.M, BF_BM) * nbn;
+  NQS_REQUIRE(ntiles < (int64_t)1 << 31, NQS_ERR_UNSUPPORTED, "GEMM too large for one launch");
+  hipLaunchKernelGGL((k_gemm_nt_bf<SPLIT, NBUF>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(256), lds, st, g,
+                     nbn, ntiles);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
 template <typename T>
 int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return NQS_OK;
   constexpr int VN = Vec16<T>::n;
   const bool vec_ok = (g.K % VN == 0) && (g.lda % VN == 0) && (g.ldb % VN == 0) && (((uintptr_t)g.A & 15) == 0) &&
                       (((uintptr_t)g.B & 15) == 0) && (g.strideA % VN == 0) && (g.strideB % VN == 0);
+  if constexpr (sizeof(T) == 4) {
+    if (g.split != 0) {
+      NQS_REQUIRE(g.split == 1 || g.split == 3, NQS_ERR_ARG, "bf16 GEMM split must be 1 or 3");
+      NQS_REQUIRE(vec_ok && batch == 1 && g.tri == 0 && !g.mirror && g.K % 4 == 0, NQS_ERR_UNSUPPORTED,
+                  "bf16 GEMM needs K, lda, ldb multiples of 4, 16-byte aligned operands and no batch/triangle");
+      return g.split == 3 ? launch_gemm_bf<3, NQS_BF_NBUF>(g, st) : launch_gemm_bf<1, NQS_BF_NBUF>(g, st);
+    }
+  }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
     if (sizeof(T) == 8)

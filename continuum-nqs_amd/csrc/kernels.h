@@ -17,6 +17,7 @@ struct GemmArgs {
   int64_t bias_len;
   int tri;     // 0 full, 1 lower-triangular tiles only (col <= row)
   int mirror;  // also store C[col][row]
+  int split;   // float only: 0 float32 MFMA, 1 bf16 MFMA, 3 bf16x3 (hi/lo split) MFMA
 };
 
 template <typename T>

@@ -495,6 +495,7 @@ static GemmArgs<double> g64(const double* A, const double* B, double* C, int64_t
   g.bias_len = 0;
   g.tri = tri;
   g.mirror = 0;
+  g.split = 0;
   return g;
 }
 

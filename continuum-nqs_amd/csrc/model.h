@@ -36,6 +36,7 @@ struct Model {
   int64_t gs_off, wenv_off, dlw_off, jas_off, n_dbl;  // offsets in doubles from dbl_off
   int64_t packed_bytes;
   int tsize;           // sizeof(T)
+  int split;           // GemmArgs::split of the backbone dense layers (float32 models)
 };
 
 int build_model(const nqs_model_desc* m, Model& md);

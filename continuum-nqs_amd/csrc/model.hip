@@ -43,6 +43,9 @@ int build_model(const nqs_model_desc* m, Model& md) {
   NQS_REQUIRE(md.d <= 512 && md.ff * md.d <= 512, NQS_ERR_UNSUPPORTED, "embed_dim*ff_mult must be <= 512");
   NQS_REQUIRE(md.K >= 1 && md.n_g >= 1 && md.F >= 1 && md.depth >= 0 && md.ff >= 1, NQS_ERR_ARG, "bad architecture");
   NQS_REQUIRE(m->precision == NQS_FP64 || m->precision == NQS_MIXED, NQS_ERR_ARG, "bad precision");
+  NQS_REQUIRE(m->precision == NQS_FP64 || (m->gemm >= NQS_GEMM_BF16X3 && m->gemm <= NQS_GEMM_BF16), NQS_ERR_ARG,
+              "bad gemm mode");
+  md.split = m->precision == NQS_FP64 ? 0 : (m->gemm == NQS_GEMM_BF16X3 ? 3 : m->gemm == NQS_GEMM_BF16 ? 1 : 0);
   md.kin = 4 * md.F + 2;
   md.kpad = (int)round_up(md.kin, 8);
   md.KN = md.K * md.N;
@@ -335,8 +338,8 @@ FwdBufs<T> carve_fwd(const Model& md, char*& p, int64_t n_cfg, int C, int nt) {
 }
 
 template <typename T>
-static GemmArgs<T> gargs(const T* A, const T* B, T* C, int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb,
-                         int64_t ldc, const T* bias, int64_t period, int64_t blen, T beta) {
+static GemmArgs<T> gargs(const Model& md, const T* A, const T* B, T* C, int64_t M, int64_t N, int64_t K, int64_t lda,
+                         int64_t ldb, int64_t ldc, const T* bias, int64_t period, int64_t blen, T beta) {
   GemmArgs<T> g;
   g.A = A;
   g.B = B;
@@ -355,6 +358,7 @@ static GemmArgs<T> gargs(const T* A, const T* B, T* C, int64_t M, int64_t N, int
   g.bias_len = blen;
   g.tri = 0;
   g.mirror = 0;
+  g.split = sizeof(T) == 4 ? md.split : 0;
   return g;
 }
 
@@ -366,24 +370,24 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
   const int64_t rows = jg.n_cfg * jg.C * md.N;
   const int64_t per = (int64_t)jg.C * md.N, blen = md.N;
   NQS_TRY(launch_features<T>(jg, R, S, md.Ainv, md.F, md.kpad, w0, b.X0, st));
-  NQS_TRY(gemm_nt<T>(gargs<T>(b.X0, pk + md.embT, b.x, rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per,
+  NQS_TRY(gemm_nt<T>(gargs<T>(md, b.X0, pk + md.embT, b.x, rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per,
                               blen, (T)0),
                      1, st));
   for (int l = 0; l < md.depth; ++l) {
     const LayerPack& P = md.layers[l];
     NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + P.ln0_s, pk + P.ln0_b, b.h, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(b.h, pk + P.qkvT, b.big, rows, 3 * d, d, d, d, 3 * d, pk + P.qkv_b, per, blen, (T)0),
+    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + P.qkvT, b.big, rows, 3 * d, d, d, d, 3 * d, pk + P.qkv_b, per, blen, (T)0),
                        1, st));
     NQS_TRY(launch_attention_jet<T>(jg, d, md.nh, b.big, b.att, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(b.att, pk + P.oT, b.x, rows, d, d, d, d, d, pk + P.o_b, per, blen, (T)1), 1, st));
+    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.att, pk + P.oT, b.x, rows, d, d, d, d, d, pk + P.o_b, per, blen, (T)1), 1, st));
     NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + P.ln1_s, pk + P.ln1_b, b.h, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(b.h, pk + P.w0T, b.big, rows, ffd, d, d, d, ffd, pk + P.b0, per, blen, (T)0), 1, st));
+    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + P.w0T, b.big, rows, ffd, d, d, d, ffd, pk + P.b0, per, blen, (T)0), 1, st));
     NQS_TRY(launch_gelu_jet<T>(jg, ffd, b.big, b.att, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1, per, blen, (T)1), 1,
+    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1, per, blen, (T)1), 1,
                        st));
   }
   NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + md.lnf_s, pk + md.lnf_b, b.h, st));
-  NQS_TRY(gemm_nt<T>(gargs<T>(b.h, pk + md.bfT, b.big, rows, 2 * KN, d, d, d, 2 * KN, pk + md.bf_b, per, blen, (T)0),
+  NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + md.bfT, b.big, rows, 2 * KN, d, d, d, 2 * KN, pk + md.bf_b, per, blen, (T)0),
                      1, st));
   const DetParams dp = det_params(md, packed);
   NQS_TRY(launch_det_jet<T>(jg, dp, R, S, w0, b.big, b.lp, b.tan, b.lap, st));
@@ -483,7 +487,7 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
     return launch_ln_bwd<T>(a, st);
   };
   auto gemm = [&](const T* A, const T* B, T* C, int64_t M, int64_t Nn, int64_t K, const T* bias, T beta) {
-    return gemm_nt<T>(gargs<T>(A, B, C, M, Nn, K, K, K, Nn, bias, per, blen, beta), 1, st);
+    return gemm_nt<T>(gargs<T>(md, A, B, C, M, Nn, K, K, K, Nn, bias, per, blen, beta), 1, st);
   };
   // block forward from x_in into x_out (x_out = x_in + attn + mlp), keeping hA, qkv, attA, xmid, hB, U, Gv
   auto block_fwd = [&](int l, const T* x_in, T* x_out) -> int {
@@ -504,7 +508,7 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
   };
   // ---- forward with block-input checkpoints
   NQS_TRY(launch_features<T>(jg, R, S, md.Ainv, md.F, md.kpad, w0, X0, st));
-  NQS_TRY(gemm_nt<T>(gargs<T>(X0, pk + md.embT, xs[0], rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per, blen,
+  NQS_TRY(gemm_nt<T>(gargs<T>(md, X0, pk + md.embT, xs[0], rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per, blen,
                               (T)0),
                      1, st));
   for (int l = 0; l < depth; ++l) NQS_TRY(block_fwd(l, xs[l], xs[l + 1]));

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libnqs_hip.so")
 
 NQS_OK, NQS_ERR_ARG, NQS_ERR_HIP, NQS_ERR_WORKSPACE, NQS_ERR_NUMERIC, NQS_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 NQS_FP64, NQS_MIXED = 0, 1
+NQS_GEMM_BF16X3, NQS_GEMM_FP32, NQS_GEMM_BF16 = 0, 1, 2
 OP_LOGPSI, OP_ELOC, OP_MCMC, OP_GRAD = 0, 1, 2, 3
 
 c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -23,7 +24,7 @@ class ModelDesc(ctypes.Structure):
     _fields_ = [("n_elec", c_i32), ("n_g", c_i32), ("embed_dim", c_i32), ("depth", c_i32),
                 ("n_heads", c_i32), ("ff_mult", c_i32), ("n_freq", c_i32), ("n_dets", c_i32),
                 ("learnable_env", c_i32), ("use_jastrow", c_i32), ("jastrow_kind", c_i32),
-                ("n_knots", c_i32), ("precision", c_i32), ("reserved", c_i32),
+                ("n_knots", c_i32), ("precision", c_i32), ("gemm", c_i32),
                 ("cell", c_dbl * 4), ("cell_inv", c_dbl * 4), ("cusp", c_dbl)]
 
 

@@ -8,7 +8,10 @@ R is (N, 2) or (W, N, 2) float64, S is (N,) or (W, N) integer spins in {0, 1}; b
 vmap.  The log psi apply, its derivatives and everything downstream run in libnqs_hip.so; the
 parameters live on the GPU as the flat real_ravel vector theta plus a packed kernel layout.
 `precision="mixed"` runs the backbone (features, attention, MLP, LayerNorm, backflow) in
-float32; the envelope, determinants, Jastrow and log-sum-exp stay float64 either way.
+float32; the envelope, determinants, Jastrow and log-sum-exp stay float64 either way.  `gemm`
+picks how a mixed model's dense layers use the matrix cores: "bf16x3" (default; each float32
+operand split into a bf16 hi/lo pair, three bf16 products summed in float32), "fp32" (float32
+MFMA) or "bf16" (operands rounded to bf16 once).
 """
 import ctypes
 import math
@@ -23,7 +26,7 @@ from .cell import as_cell
 class ContinuumSpinorPsiformer:
     def __init__(self, n_elec, Gs, W, L, embed_dim=32, depth=3, n_heads=4, ff_mult=2, n_freq=4, n_dets=1,
                  learnable_env=False, use_jastrow=False, jastrow_kind="rbf", n_knots=10, cusp=1.0,
-                 precision="fp64"):
+                 precision="fp64", gemm="bf16x3"):
         self.n_elec = int(n_elec)
         self.Gs = np.asarray(Gs, dtype=float)
         self.W = np.asarray(W, dtype=complex)
@@ -38,6 +41,9 @@ class ContinuumSpinorPsiformer:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
+        if gemm not in ("bf16x3", "fp32", "bf16"):
+            raise ValueError("gemm must be 'bf16x3', 'fp32' or 'bf16'")
+        self.gemm = gemm
         if self.W.shape != (self.n_elec, self.Gs.shape[0], 2):
             raise ValueError(f"W must have shape (n_elec, n_G, 2) = {(self.n_elec, self.Gs.shape[0], 2)}")
         self._dev = None
@@ -51,6 +57,7 @@ class ContinuumSpinorPsiformer:
         d.jastrow_kind = 0 if self.jastrow_kind == "rbf" else 1
         d.n_knots = self.n_knots
         d.precision = _lib.NQS_FP64 if self.precision == "fp64" else _lib.NQS_MIXED
+        d.gemm = {"bf16x3": _lib.NQS_GEMM_BF16X3, "fp32": _lib.NQS_GEMM_FP32, "bf16": _lib.NQS_GEMM_BF16}[self.gemm]
         for i, v in enumerate(self.A.reshape(-1)):
             d.cell[i] = float(v)
         for i, v in enumerate(np.linalg.inv(self.A).reshape(-1)):

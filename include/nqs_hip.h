@@ -41,6 +41,12 @@ enum {
  * envelope, Jastrow, Ewald and all linear algebra of MinSR's solve stay float64 in both. */
 enum { NQS_FP64 = 0, NQS_MIXED = 1 };
 
+/* how the backbone's dense layers use MFMA in an NQS_MIXED model (activations are float32 in
+ * all three, accumulation is float32): BF16X3 splits each float32 operand into a bf16 hi/lo pair
+ * and sums three bf16 products; FP32 uses float32 MFMA; BF16 rounds the operands to bf16 once.
+ * Ignored for NQS_FP64 models (float64 MFMA). */
+enum { NQS_GEMM_BF16X3 = 0, NQS_GEMM_FP32 = 1, NQS_GEMM_BF16 = 2 };
+
 /* ops for nqs_workspace_bytes */
 enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3 };
 
@@ -59,7 +65,7 @@ typedef struct nqs_model_desc {
   int32_t jastrow_kind;  /* 0 = rbf, 1 = cusp */
   int32_t n_knots;
   int32_t precision;     /* NQS_FP64 or NQS_MIXED */
-  int32_t reserved;
+  int32_t gemm;          /* NQS_GEMM_* for NQS_MIXED */
   double cell[4];        /* A row-major; columns are the lattice vectors a1, a2 */
   double cell_inv[4];    /* A^-1 row-major, as the caller computes it (positions are wrapped
                             with exactly these numbers) */
