@@ -419,11 +419,215 @@ __global__ __launch_bounds__(256) void k_attention_jet_wave(JetGeom j, int d, in
   }
 }
 
+// Register-resident variant on 16x16x4 MFMA for N <= 16 tokens and head width hd <= 16 NQ
+// (NQ = 1, 2).
+// One wave per (cfg, head), no LDS.  Tokens are padded to 16 and features to 16 NQ with zeros
+// (padded keys get score -inf, so their softmax weight is exactly 0).
+//
+// Scores are formed transposed, S^T = K Q^T, so a lane owns query a = lane%16 and the four keys
+// b = row(lane, r) of its accumulator slots; a softmax row is then 4 slots of 16 lanes' worth
+// reduced across the lane groups lane/16 (shuffles by 16 and 32).  The feature index of k-slot
+// g in MFMA step s is q = 16 (s/4) + 4 g + s%4, so each lane reads 4 consecutive features of
+// its token.  For P V the k index (the key) of slot g in step s is b = row(lane, s), i.e. the
+// lane's own softmax slot s, so P enters the MFMA straight from registers and V is read as
+// V[row(lane, s)][16 t + lane%16].
+template <typename T, int NQ>
+struct AttnRegs {
+  T q[4 * NQ], k[4 * NQ], v[4 * NQ];
+};
+
+template <typename T, int NQ>
+__device__ __forceinline__ void attn_load(const T* __restrict__ base, int lane, int N, int hd, int d, int h,
+                                          int64_t rs, AttnRegs<T, NQ>& x) {
+  const int li = lane & 15, g = lane >> 4;
+  const int tq = li < N ? li : N - 1;
+#pragma unroll
+  for (int s = 0; s < 4 * NQ; ++s) {
+    const int q = 16 * (s >> 2) + 4 * g + (s & 3);
+    const int qc = q < hd ? q : hd - 1;
+    const bool ok = li < N && q < hd;
+    const T vq = base[tq * rs + h * hd + qc];
+    const T vk = base[tq * rs + d + h * hd + qc];
+    x.q[s] = ok ? vq : (T)0;
+    x.k[s] = ok ? vk : (T)0;
+    const int b = Mma16<T>::row(lane, s & 3), qv = 16 * (s >> 2) + li;
+    const int bc = b < N ? b : N - 1, qvc = qv < hd ? qv : hd - 1;
+    const T vv = base[bc * rs + 2 * d + h * hd + qvc];
+    x.v[s] = (b < N && qv < hd) ? vv : (T)0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T xsum16(T v) {
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+template <typename T, int NQ>
+__device__ __forceinline__ void attn_store(T* __restrict__ o, int lane, int N, int hd, int d, int h,
+                                           const typename Mma16<T>::acc_t (&acc)[NQ], T scale2,
+                                           const typename Mma16<T>::acc_t (&add)[NQ]) {
+  const int li = lane & 15;
+#pragma unroll
+  for (int t = 0; t < NQ; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int a = Mma16<T>::row(lane, r), q = 16 * t + li;
+      if (a < N && q < hd) o[a * d + h * hd + q] = acc[t][r] + scale2 * add[t][r];
+    }
+}
+
+template <typename T, int NQ>
+__global__ __launch_bounds__(256) void k_attention_jet_mfma(JetGeom j, int d, int nh, const T* __restrict__ qkv,
+                                                            T* __restrict__ out, int64_t npairs) {
+  typedef typename Mma16<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63;
+  const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= npairs) return;
+  const int N = j.N, hd = d / nh;
+  const int64_t cfg = pair / nh;
+  const int h = (int)(pair % nh);
+  const int64_t rs = 3 * (int64_t)d;
+  const int64_t chs_in = (int64_t)N * rs, chs_out = (int64_t)N * d;
+  const T* in0 = qkv + cfg * j.C * chs_in;
+  T* out0 = out + cfg * j.C * chs_out;
+  const T sc = (T)1 / sqrt((T)hd);
+  const acc_t zacc = (acc_t)(T)0;
+  acc_t zq[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) zq[t] = zacc;
+  // ---- value channel
+  AttnRegs<T, NQ> x0;
+  attn_load<T, NQ>(in0, lane, N, hd, d, h, rs, x0);
+  acc_t s0 = zacc;
+#pragma unroll
+  for (int s = 0; s < 4 * NQ; ++s) s0 = Mma16<T>::mma(x0.k[s], x0.q[s], s0);
+  T P[4];
+  {
+    T sv[4], mx;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sv[r] = Mma16<T>::row(lane, r) < N ? s0[r] * sc : -INFINITY;
+    mx = sv[0];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) mx = sv[r] > mx ? sv[r] : mx;
+    T o = __shfl_xor(mx, 16);
+    mx = o > mx ? o : mx;
+    o = __shfl_xor(mx, 32);
+    mx = o > mx ? o : mx;
+    T z = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      P[r] = exp(sv[r] - mx);
+      z += P[r];
+    }
+    const T iz = (T)1 / xsum16(z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) P[r] *= iz;
+  }
+  {
+    acc_t o[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+      o[t] = zacc;
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) o[t] = Mma16<T>::mma(P[kb], x0.v[4 * t + kb], o[t]);
+    }
+    attn_store<T, NQ>(out0, lane, N, hd, d, h, o, (T)0, zq);
+  }
+  if (j.nt == 0 && !j.has_lap) return;
+  // ---- tangent channels
+  acc_t accS = zacc, accPV[NQ];
+  T accP[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) accPV[t] = zacc;
+  for (int c = 1; c <= j.nt; ++c) {
+    AttnRegs<T, NQ> xc;
+    attn_load<T, NQ>(in0 + c * chs_in, lane, N, hd, d, h, rs, xc);
+    acc_t sa = zacc;
+#pragma unroll
+    for (int s = 0; s < 4 * NQ; ++s) {
+      sa = Mma16<T>::mma(x0.k[s], xc.q[s], sa);
+      sa = Mma16<T>::mma(xc.k[s], x0.q[s], sa);
+      accS = Mma16<T>::mma(xc.k[s], xc.q[s], accS);
+    }
+    T u[4], Pc[4], m = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m += P[r] * (sa[r] * sc);
+    m = xsum16(m);
+    T m2 = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      u[r] = sa[r] * sc - m;
+      m2 += P[r] * u[r] * u[r];
+    }
+    m2 = xsum16(m2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      accP[r] += u[r] * u[r] - m2;
+      Pc[r] = P[r] * u[r];
+    }
+    acc_t o[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+      o[t] = zacc;
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        o[t] = Mma16<T>::mma(Pc[kb], x0.v[4 * t + kb], o[t]);
+        o[t] = Mma16<T>::mma(P[kb], xc.v[4 * t + kb], o[t]);
+        accPV[t] = Mma16<T>::mma(Pc[kb], xc.v[4 * t + kb], accPV[t]);
+      }
+    }
+    attn_store<T, NQ>(out0 + c * chs_out, lane, N, hd, d, h, o, (T)0, zq);
+  }
+  if (!j.has_lap) return;
+  // ---- Laplacian channel
+  const int cl = j.nt + 1;
+  AttnRegs<T, NQ> xl;
+  attn_load<T, NQ>(in0 + cl * chs_in, lane, N, hd, d, h, rs, xl);
+  acc_t sl = zacc;
+#pragma unroll
+  for (int s = 0; s < 4 * NQ; ++s) {
+    sl = Mma16<T>::mma(x0.k[s], xl.q[s], sl);
+    sl = Mma16<T>::mma(xl.k[s], x0.q[s], sl);
+  }
+  T w[4], ml = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    w[r] = (sl[r] + (T)2 * accS[r]) * sc;
+    ml += P[r] * w[r];
+  }
+  ml = xsum16(ml);
+  T Pl[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Pl[r] = P[r] * (w[r] - ml + accP[r]);
+  acc_t o[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) {
+    o[t] = zacc;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      o[t] = Mma16<T>::mma(Pl[kb], x0.v[4 * t + kb], o[t]);
+      o[t] = Mma16<T>::mma(P[kb], xl.v[4 * t + kb], o[t]);
+    }
+  }
+  attn_store<T, NQ>(out0 + cl * chs_out, lane, N, hd, d, h, o, (T)2, accPV);
+}
+
 template <typename T>
 int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* out, hipStream_t st) {
   const int64_t npairs = j.n_cfg * n_heads;
   if (npairs == 0) return NQS_OK;
   const int N = j.N, hd = d / n_heads;
+  if (N <= 16 && hd <= 32) {
+    const dim3 grid((unsigned)cdiv(npairs, 4)), block(256);
+    if (hd <= 16)
+      hipLaunchKernelGGL((k_attention_jet_mfma<T, 1>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
+    else
+      hipLaunchKernelGGL((k_attention_jet_mfma<T, 2>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  }
   const size_t per_wave = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
   NQS_REQUIRE(per_wave <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");
   int wpg = (int)std::min<size_t>(4, (64 * 1024) / per_wave);

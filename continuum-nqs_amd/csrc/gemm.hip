@@ -17,25 +17,6 @@
 
 namespace nqs {
 
-template <typename T>
-struct Mma16;
-template <>
-struct Mma16<double> {
-  typedef d4v acc_t;
-  static __device__ inline acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <>
-struct Mma16<float> {
-  typedef f4v acc_t;
-  static __device__ inline acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-
 // Block tile 64 x 64, 4 waves in a 2 x 2 arrangement, 32 x 32 per wave (2 x 2 MFMA blocks).
 template <typename T, int BK>
 __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs<T> g) {
@@ -446,12 +427,11 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   const bool vec_ok = (g.K % VN == 0) && (g.lda % VN == 0) && (g.ldb % VN == 0) && (((uintptr_t)g.A & 15) == 0) &&
                       (((uintptr_t)g.B & 15) == 0) && (g.strideA % VN == 0) && (g.strideB % VN == 0);
   if constexpr (sizeof(T) == 4) {
-    if (g.split != 0) {
-      NQS_REQUIRE(g.split == 1 || g.split == 3, NQS_ERR_ARG, "bf16 GEMM split must be 1 or 3");
-      NQS_REQUIRE(vec_ok && batch == 1 && g.tri == 0 && !g.mirror && g.K % 4 == 0, NQS_ERR_UNSUPPORTED,
-                  "bf16 GEMM needs K, lda, ldb multiples of 4, 16-byte aligned operands and no batch/triangle");
+    // shapes the bf16 kernel cannot take (K or a leading dimension not a multiple of 4, batched or
+    // triangular calls) run on the float32 kernels below
+    NQS_REQUIRE(g.split == 0 || g.split == 1 || g.split == 3, NQS_ERR_ARG, "bf16 GEMM split must be 0, 1 or 3");
+    if (g.split != 0 && vec_ok && batch == 1 && g.tri == 0 && !g.mirror)
       return g.split == 3 ? launch_gemm_bf<3, NQS_BF_NBUF>(g, st) : launch_gemm_bf<1, NQS_BF_NBUF>(g, st);
-    }
   }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);

@@ -4,6 +4,27 @@
 
 namespace nqs {
 
+// v_mfma_*_16x16x4 wrappers: A[lane%16][lane/16], B[lane/16][lane%16]; accumulator slot r of
+// a lane holds C[row(lane, r)][lane%16], with row = lane/16 + 4 r (f64) or 4 (lane/16) + r (f32).
+template <typename T>
+struct Mma16;
+template <>
+struct Mma16<double> {
+  typedef d4v acc_t;
+  static __device__ inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mma16<float> {
+  typedef f4v acc_t;
+  static __device__ inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 template <typename T>
 struct GemmArgs {
   const T* A;  // M x K, row stride lda
