@@ -399,12 +399,387 @@ __global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const d
   }
 }
 
+// ------------------------------------------------------------------ wave-level head for N <= 16
+// One workgroup of 4 waves per configuration; the envelope jet is built once in LDS, then each
+// wave takes determinants k = wave, wave + 4, ... entirely in registers.  Matrices are padded to
+// 16 x 16 with an identity block (same determinant, inverse and traces).
+//
+// Two register layouts of a 16 x 16 complex matrix X, lane = 16 g + l (l = lane % 16):
+//   row chunk  ("A"): slot s holds X[l][4 g + s]
+//   column chunk ("B"): slot s holds X[4 g + s][l]
+// M and its inverse live in the row-chunk layout, which is also the A operand of
+// v_mfma_f64_16x16x4f64 with k index 4 g + s in step s; the tangent matrices dM live in the
+// column-chunk layout, the matching B operand.  Hence tr(Mi dM) is a lane-local dot product
+// summed over the wave, and tr((Mi dM)^2) = sum_ij A_ij A_ji pairs each accumulator slot of
+// A = Mi dM with the same slot of (dM^T Mi^T) = A^T, obtained from the same registers with the
+// MFMA operands swapped.
+//
+// LU with partial pivoting (largest |re| + |im|, first index on ties, as the block kernel
+// above) moves rows with cross-lane shuffles; `orig` follows the original index of the row at
+// each position, so P I is known without replaying the swaps.
+__device__ __forceinline__ cd shfl_cd(cd v, int src) { return mk(__shfl(v.re, src), __shfl(v.im, src)); }
+__device__ __forceinline__ cd wave_sum_cd(cd v) { return mk(wave_sum(v.re), wave_sum(v.im)); }
+
+struct Lu16 {
+  cd m[4];      // row-chunk layout; after factor(): L below the diagonal (unit), U on and above
+  int orig;     // original row index of the row now at this lane's position
+  int parity;   // parity of the row swaps
+  bool zero_pivot;
+};
+
+// slot s (0..3, wave-uniform at run time) of the 4 slots: an extract from a vector value, so the
+// slots stay in registers (an array indexed at run time would be moved to scratch)
+typedef double d8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ cd pick4(const cd (&m)[4], int s) {
+  d8v v;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    v[2 * t] = m[t].re;
+    v[2 * t + 1] = m[t].im;
+  }
+  return mk(v[2 * s], v[2 * s + 1]);
+}
+
+__device__ __forceinline__ void lu16_factor(Lu16& f, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  f.orig = r;
+  f.parity = 0;
+  f.zero_pivot = false;
+#pragma unroll 1
+  for (int q = 0; q < 16; ++q) {
+    const int src_col = r + 16 * (q >> 2);
+    cd cq = shfl_cd(pick4(f.m, q & 3), src_col);
+    double v = r >= q ? fabs(cq.re) + fabs(cq.im) : -1.0;
+    int br = r;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const double ov = __shfl_xor(v, off);
+      const int obr = __shfl_xor(br, off);
+      const bool take = ov > v || (ov == v && obr < br);
+      v = take ? ov : v;
+      br = take ? obr : br;
+    }
+    const int p = __builtin_amdgcn_readfirstlane(br);
+    if (__builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0)) f.zero_pivot = true;
+    if (p != q) {
+      f.parity ^= 1;
+      const int src = (r == q ? p : (r == p ? q : r)) + 16 * g;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) f.m[s] = shfl_cd(f.m[s], src);
+      f.orig = __shfl(f.orig, src);
+      cq = shfl_cd(cq, src);
+    }
+    const cd pv = shfl_cd(cq, q);
+    const bool zero = pv.re == 0.0 && pv.im == 0.0;
+    const cd l = (r > q && !zero) ? cdiv_(cq, pv) : mk(0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cd rq = shfl_cd(f.m[s], q + 16 * g);
+      const int col = 4 * g + s;
+      if (r > q) {
+        if (col == q) f.m[s] = l;
+        else if (col > q) f.m[s] = f.m[s] - l * rq;
+      }
+    }
+  }
+}
+
+// log det from the factors (principal branch), as lu_logdet for the first N pivots.
+__device__ __forceinline__ cd lu16_logdet(const Lu16& f, int lane, int N) {
+  const int r = lane & 15, g = lane >> 4;
+  const bool mine = (r >> 2) == g && r < N;
+  const cd u = pick4(f.m, r & 3);
+  const double au = mine ? cabs_(u) : 1.0;
+  double la = wave_sum(log(au));
+  cd ph = mine && au > 0 ? mk(u.re / au, u.im / au) : mk(1, 0);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) ph = ph * shfl_cd(ph, lane ^ off);
+  if (f.parity) ph = mk(-ph.re, -ph.im);
+  return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
+}
+
+// Mi = U^-1 L^-1 P in the row-chunk layout.
+__device__ __forceinline__ void lu16_inverse(const Lu16& f, int lane, cd (&y)[4]) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) y[s] = mk(4 * g + s == f.orig ? 1.0 : 0.0, 0);
+#pragma unroll 1
+  for (int q = 0; q < 16; ++q) {
+    const cd lq = shfl_cd(pick4(f.m, q & 3), r + 16 * (q >> 2));
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cd yq = shfl_cd(y[s], q + 16 * g);
+      if (r > q) y[s] = y[s] - lq * yq;
+    }
+  }
+#pragma unroll 1
+  for (int q = 15; q >= 0; --q) {
+    const cd mq = pick4(f.m, q & 3);
+    const cd uqq = shfl_cd(mq, q + 16 * (q >> 2));
+    if (r == q)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) y[s] = cdiv_(y[s], uqq);
+    const cd urq = shfl_cd(mq, r + 16 * (q >> 2));
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cd zq = shfl_cd(y[s], q + 16 * g);
+      if (r < q) y[s] = y[s] - urq * zq;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cd bf_at(const T* __restrict__ bfc, int64_t rowlen, int64_t KN, int N, int k, int a, int i) {
+  const bool ok = a < N && i < N;
+  const int ac = a < N ? a : N - 1, ic = i < N ? i : N - 1;
+  const T* rp = bfc + (int64_t)ic * rowlen + (int64_t)k * N + ac;
+  const double re = (double)rp[0], im = (double)rp[KN];
+  return ok ? mk(re, im) : mk(0, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
+                                                   const int32_t* __restrict__ S, int64_t w0,
+                                                   const T* __restrict__ bf, double* __restrict__ out_lp,
+                                                   double* __restrict__ out_tan, double* __restrict__ out_lap) {
+  __shared__ cd env[256], dex[256], dey[256], lenv[256];
+  __shared__ double pos[32];
+  __shared__ int spin[16];
+  extern __shared__ cd res[];  // [K][nt + 2]: logdet, tangents, Laplacian
+  const int N = j.N, K = p.K, nt = j.nt;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t cfg = blockIdx.x;
+  int64_t w;
+  int flip = -1;
+  if (j.mode == 2) {
+    w = w0 + cfg / N;
+    flip = (int)(cfg % N);
+  } else {
+    w = w0 + cfg;
+  }
+  if (tid < N) {
+    pos[2 * tid] = R[(w * N + tid) * 2];
+    pos[2 * tid + 1] = R[(w * N + tid) * 2 + 1];
+    const int s = S[w * N + tid];
+    spin[tid] = (tid == flip) ? 1 - s : s;
+  }
+  __syncthreads();
+  {  // envelope jet, entry (a, i) = (tid / 16, tid % 16)
+    const int a = tid >> 4, i = tid & 15;
+    cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0), lp = mk(0, 0);
+    if (a < N && i < N) {
+      for (int gg = 0; gg < p.n_g; ++gg) {
+        const double Gx = p.gs[2 * gg], Gy = p.gs[2 * gg + 1];
+        double sn, cs;
+        sincos(Gx * pos[2 * i] + Gy * pos[2 * i + 1], &sn, &cs);
+        const double* wp = p.wenv + ((size_t)(a * p.n_g + gg) * 2 + spin[i]) * 2;
+        const cd t = mk(wp[0], wp[1]) * mk(cs, sn);
+        v = v + t;
+        gx = gx + t * mk(0, Gx);
+        gy = gy + t * mk(0, Gy);
+        lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
+      }
+    }
+    env[tid] = v;
+    dex[tid] = gx;
+    dey[tid] = gy;
+    lenv[tid] = lp;
+  }
+  __syncthreads();
+  const int C = j.C;
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
+  const T* bfc0 = bf + cfg * C * chs;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int rs_ = nt + 2;
+  for (int k = wave; k < K; k += 4) {
+    Lu16 f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int a = l16, i = 4 * g + s;
+      const cd b = bf_at(bfc0, rowlen, KN, N, k, a, i);
+      const cd e = env[a * 16 + i];
+      f.m[s] = (a < N && i < N) ? e * (mk(1, 0) + b) : mk(a == i ? 1.0 : 0.0, 0);
+    }
+    lu16_factor(f, lane);
+    const cd ld = lu16_logdet(f, lane, N);
+    if (lane == 0) res[k * rs_] = ld;
+    if (nt == 0 && !j.has_lap) continue;
+    cd mi[4];
+    lu16_inverse(f, lane, mi);
+    // (1 + bf) in the column-chunk layout
+    cd ob[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) ob[s] = mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, 4 * g + s, l16);
+    cd hsum = mk(0, 0);
+    for (int c = 1; c <= nt; ++c) {
+      int ec, dc;
+      if (j.mode == 2) {
+        ec = flip;
+        dc = c - 1;
+      } else {
+        ec = (c - 1) >> 1;
+        dc = (c - 1) & 1;
+      }
+      const T* bfc = bfc0 + c * chs;
+      cd dm[4];
+      cd tr = mk(0, 0);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int a = 4 * g + s, i = l16, e_ = a * 16 + i;
+        cd v = env[e_] * bf_at(bfc, rowlen, KN, N, k, a, i);
+        if (i == ec && a < N) v = v + (dc == 0 ? dex[e_] : dey[e_]) * ob[s];
+        dm[s] = v;
+        tr = tr + mi[s] * v;
+      }
+      tr = wave_sum_cd(tr);
+      if (lane == 0) res[k * rs_ + c] = tr;
+      if (j.has_lap) {
+        d4v are = (d4v)0.0, aim = (d4v)0.0, tre = (d4v)0.0, tim = (d4v)0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          are = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[s].re, dm[s].re, are, 0, 0, 0);
+          are = __builtin_amdgcn_mfma_f64_16x16x4f64(-mi[s].im, dm[s].im, are, 0, 0, 0);
+          aim = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[s].re, dm[s].im, aim, 0, 0, 0);
+          aim = __builtin_amdgcn_mfma_f64_16x16x4f64(mi[s].im, dm[s].re, aim, 0, 0, 0);
+          tre = __builtin_amdgcn_mfma_f64_16x16x4f64(dm[s].re, mi[s].re, tre, 0, 0, 0);
+          tre = __builtin_amdgcn_mfma_f64_16x16x4f64(-dm[s].im, mi[s].im, tre, 0, 0, 0);
+          tim = __builtin_amdgcn_mfma_f64_16x16x4f64(dm[s].re, mi[s].im, tim, 0, 0, 0);
+          tim = __builtin_amdgcn_mfma_f64_16x16x4f64(dm[s].im, mi[s].re, tim, 0, 0, 0);
+        }
+        cd h = mk(0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h = h + mk(are[r], aim[r]) * mk(tre[r], tim[r]);
+        hsum = hsum + wave_sum_cd(h);
+      }
+    }
+    if (j.has_lap) {
+      const int cl = nt + 1;
+      const T* bfl = bfc0 + cl * chs;
+      cd tr = mk(0, 0);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int a = 4 * g + s, i = l16, e_ = a * 16 + i;
+        const int ic = i < N ? i : N - 1;
+        cd v = lenv[e_] * ob[s] + env[e_] * bf_at(bfl, rowlen, KN, N, k, a, i);
+        v = v + 2.0 * (dex[e_] * bf_at(bfc0 + (1 + 2 * ic) * chs, rowlen, KN, N, k, a, i) +
+                       dey[e_] * bf_at(bfc0 + (2 + 2 * ic) * chs, rowlen, KN, N, k, a, i));
+        tr = tr + mi[s] * v;
+      }
+      tr = wave_sum_cd(tr);
+      if (lane == 0) res[k * rs_ + cl] = tr - hsum;
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  // ---- log-sum-exp over determinants, tangents and Laplacian of log psi, Jastrow
+  cd L;
+  double amax = -INFINITY;
+  for (int k = 0; k < K; ++k) {
+    cd z = res[k * rs_];
+    if (K > 1) z = z + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]);
+    amax = z.re > amax ? z.re : amax;
+  }
+  if (K == 1) {
+    L = res[0];
+  } else {
+    if (!isfinite(amax)) amax = 0.0;
+    cd s = mk(0, 0);
+    for (int k = 0; k < K; ++k) s = s + cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - mk(amax, 0));
+    L = clog_(s) + mk(amax, 0);
+  }
+  auto pk_of = [&](int k) -> cd {
+    return K == 1 ? mk(1, 0) : cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - L);
+  };
+  // tangent c = lane + 1 (nt <= 2N <= 32)
+  const int c = lane + 1;
+  const bool has_c = c <= nt;
+  cd gsum = mk(0, 0), lsum = mk(0, 0);
+  if (has_c || j.has_lap)
+    for (int k = 0; k < K; ++k) {
+      const cd pk = pk_of(k);
+      if (has_c) {
+        const cd gk = res[k * rs_ + c];
+        gsum = gsum + pk * gk;
+        if (j.has_lap) lsum = lsum + pk * (gk * gk);
+      }
+    }
+  cd lap = mk(0, 0);
+  if (j.has_lap) {
+    cd lk = mk(0, 0);
+    for (int k = 0; k < K; ++k) lk = lk + pk_of(k) * res[k * rs_ + nt + 1];
+    const cd g2 = wave_sum_cd(has_c ? gsum * gsum : mk(0, 0));
+    lap = wave_sum_cd(lsum) + lk - g2;
+  }
+  if (p.use_jastrow) {
+    // lane a (< N) sums its pairs: gradient of J wrt r_a over all partners, J and its
+    // Laplacian over partners b > a
+    const int a = lane;
+    double J = 0, lJ = 0, gxa = 0, gya = 0;
+    if (a < N) {
+      for (int b = 0; b < N; ++b) {
+        if (b == a) continue;
+        const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+        double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
+        fx -= rint(fx);
+        fy -= rint(fy);
+        const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
+        const double d2 = mx * mx + my * my;
+        const double r = sqrt(d2 + 1e-12);
+        double u, u1, u2;
+        jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
+        gxa += mx * u1 / r;
+        gya += my * u1 / r;
+        if (b > a) {
+          J += u;
+          lJ += 2.0 * (u2 * d2 / (r * r) + u1 * (2.0 / r - d2 / (r * r * r)));
+        }
+      }
+    }
+    J = wave_sum(J);
+    lJ = wave_sum(lJ);
+    int ec = 0, dc = 0;
+    if (has_c) {
+      if (j.mode == 2) {
+        ec = flip;
+        dc = c - 1;
+      } else {
+        ec = (c - 1) >> 1;
+        dc = (c - 1) & 1;
+      }
+    }
+    const double gxe = __shfl(gxa, ec), gye = __shfl(gya, ec);
+    gsum = gsum + mk(dc == 0 ? gxe : gye, 0);
+    L = L + mk(J, 0);
+    lap = lap + mk(lJ, 0);
+  }
+  if (lane == 0) {
+    out_lp[2 * cfg] = L.re;
+    out_lp[2 * cfg + 1] = L.im;
+    if (j.has_lap && out_lap) {
+      out_lap[2 * cfg] = lap.re;
+      out_lap[2 * cfg + 1] = lap.im;
+    }
+  }
+  if (has_c) {
+    out_tan[(cfg * nt + (c - 1)) * 2] = gsum.re;
+    out_tan[(cfg * nt + (c - 1)) * 2 + 1] = gsum.im;
+  }
+}
+
 // Note on rint vs jnp.round: both round half to even; minimum image is exact either way.
 template <typename T>
 int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const int32_t* S, int64_t w0,
                    const T* bf, double* out_lp, double* out_tan, double* out_lap, hipStream_t st) {
   if (j.n_cfg == 0) return NQS_OK;
   const int N = j.N, NN = N * N;
+  if (N <= 16 && j.nt <= 64) {
+    const size_t lds = sizeof(cd) * (size_t)p.K * (j.nt + 2);
+    NQS_REQUIRE(lds <= 96 * 1024, NQS_ERR_UNSUPPORTED, "determinant head exceeds LDS");
+    hipLaunchKernelGGL(k_det_jet16<T>, dim3((unsigned)j.n_cfg), dim3(256), lds, st, j, p, R, S, w0, bf, out_lp,
+                       out_tan, out_lap);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  }
   NQS_REQUIRE(NN <= 4 * 256, NQS_ERR_UNSUPPORTED, "determinant head supports N <= 32");
   size_t lds = sizeof(cd) * (size_t)(7 * NN + (size_t)p.K * (j.nt + 2)) + sizeof(double) * 2 * N +
                sizeof(int) * (2 * N + 4) + sizeof(double) * (size_t)(8 + 2 + 2 * (j.nt + 1) + (j.nt + 1) + 8);
