@@ -186,6 +186,8 @@ struct MinsrWs {
   double* Linv;
   double* panel;
   double *et, *y, *r, *u, *v, *ts, *sc;
+  double* part;  // partial column sums (colsum_part_elems)
+  double* red;   // ysum[2], mdot[2]
   int64_t bytes;
 };
 static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
@@ -210,6 +212,8 @@ static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
   w.v = (double*)grab(n * 8);
   w.ts = (double*)grab(2 * n * 8);
   w.sc = (double*)grab(8 * 8);
+  w.part = (double*)grab(colsum_part_elems(d->ldo) * 8);
+  w.red = (double*)grab(4 * 8);
   w.bytes = p - base;
   return w;
 }
@@ -236,7 +240,11 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   MinsrWs w = minsr_ws(d, ws);
   NQS_TRY(minsr_mark(0, st));
   NQS_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), st));
-  NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, st));
+  // the bf16 Gram reads only the centred bf16 copy, so O stays as given and the GEMVs below
+  // apply the centring on the fly; the float64 / float32 Gram reads O, which is centred in place
+  const bool keep = d->gram == NQS_GRAM_BF16;
+  const double* mc = keep ? w.mean : nullptr;
+  NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, keep ? 0 : 1, w.part, st));
   NQS_TRY(make_etil(eps, W, w.et, st));
   NQS_TRY(minsr_mark(1, st));
   if (d->gram == NQS_GRAM_BF16) {
@@ -279,15 +287,15 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.y, st));
   for (int it = 0; it < d->refine; ++it) {
     // r = etil - Otil (Otil^T y) - lam y ;  y += (L L^T)^-1 r
-    NQS_TRY(gemv_cols<T>(O, ldo, n, P, w.y, w.u, st));
-    NQS_TRY(gemv_rows<T>(O, ldo, n, P, w.u, w.v, 1.0, 0.0, 0, st));
+    NQS_TRY(gemv_cols_c<T>(O, ldo, W, P, w.y, mc, w.part, w.red, w.u, ldo, st));
+    NQS_TRY(gemv_rows_c<T>(O, ldo, W, P, w.u, mc, w.red + 2, w.v, st));
     NQS_TRY(axpby(n, 1.0, w.et, -1.0, w.v, w.r, st));
     NQS_TRY(axpby(n, 1.0, w.r, -d->lam, w.y, w.r, st));
     NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.r, st));
     NQS_TRY(axpby(n, 1.0, w.y, 1.0, w.r, w.y, st));
   }
   NQS_TRY(minsr_mark(4, st));
-  NQS_TRY(gemv_cols<T>(O, ldo, n, P, w.y, dth, st));
+  NQS_TRY(gemv_cols_c<T>(O, ldo, W, P, w.y, mc, w.part, w.red, dth, P, st));
   NQS_TRY(minsr_mark(5, st));
   return NQS_OK;
 }

@@ -12,39 +12,227 @@
 
 namespace nqs {
 
-// ------------------------------------------------------------------ centring
-template <typename T>
-__global__ void k_colmean(const T* __restrict__ O, int64_t ldo, int64_t W, int64_t P, double* __restrict__ mean) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int half = blockIdx.y;
-  if (k >= P) return;
-  const T* base = O + (int64_t)half * W * ldo + k;
-  double s = 0;
-  for (int64_t w = 0; w < W; ++w) s += (double)base[w * ldo];
-  mean[(int64_t)half * P + k] = s / (double)W;
-}
+// ------------------------------------------------------------------ column sums, centring
+// Streaming passes over O (2W x ldo) read 4 consecutive columns per thread (16 or 32 bytes) and
+// split the rows into kSplit chunks per half (rows [0, W) hold Re O, [W, 2W) Im O); each chunk
+// writes its partial column sums, which a second kernel adds up in a fixed order.
+constexpr int kSplit = 8;  // row chunks per half
 
+// part[s][c] = sum over rows r of chunk s of x(r) O[r][c]  (x = 1 when x is null)
 template <typename T>
-__global__ void k_center(T* __restrict__ O, int64_t ldo, int64_t W, int64_t P, const double* __restrict__ mean,
-                         __bf16* __restrict__ Ob) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t r = blockIdx.y;
-  if (k >= ldo) return;
-  const int half = r >= W ? 1 : 0;
-  T v = (T)0;
-  if (k < P) {
-    v = (T)((double)O[r * ldo + k] - mean[(int64_t)half * P + k]);
-    O[r * ldo + k] = v;
+__global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ O, int64_t ldo, int64_t W,
+                                                     const double* __restrict__ x, double* __restrict__ part) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  const int64_t c0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (c0 >= ldo) return;
+  const int s = blockIdx.y, half = s / kSplit, chunk = s % kSplit;
+  const int64_t len = (W + kSplit - 1) / kSplit;
+  const int64_t r0 = half * W + chunk * len;
+  const int64_t r1 = min(half * W + W, r0 + len);
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  const T* p = O + r0 * ldo + c0;
+  for (int64_t r = r0; r < r1; ++r, p += ldo) {
+    const v4 v = *reinterpret_cast<const v4*>(p);
+    const double w = x ? x[r] : 1.0;
+    a0 += w * (double)v[0];
+    a1 += w * (double)v[1];
+    a2 += w * (double)v[2];
+    a3 += w * (double)v[3];
   }
-  if (Ob) Ob[r * ldo + k] = (__bf16)(float)v;
+  double* q = part + (int64_t)s * ldo + c0;
+  q[0] = a0;
+  q[1] = a1;
+  q[2] = a2;
+  q[3] = a3;
 }
 
+// mode 0: mean[h][c] = scale * sum_{chunks of half h} part   (mean is [2][P])
+// mode 1: out[c] = sum_{all chunks} part - (mean ? mean[0][c] ysum[0] + mean[1][c] ysum[1] : 0),
+//         i.e. (O - 1 mean^T)^T x for the row-wise centred O without forming it
+//         written for c < nout; entries P <= c < nout are set to zero (a padded vector)
+__global__ void k_colsum_final(const double* __restrict__ part, int64_t ldo, int64_t P, int mode, double scale,
+                               const double* __restrict__ mean, const double* __restrict__ ysum,
+                               double* __restrict__ out, int64_t nout) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 1 && c >= P && c < nout) out[c] = 0.0;
+  if (c >= P) return;
+  double h0 = 0, h1 = 0;
+  for (int s = 0; s < kSplit; ++s) h0 += part[(int64_t)s * ldo + c];
+  for (int s = kSplit; s < 2 * kSplit; ++s) h1 += part[(int64_t)s * ldo + c];
+  if (mode == 0) {
+    out[c] = h0 * scale;
+    out[P + c] = h1 * scale;
+  } else {
+    double v = h0 + h1;
+    if (mean) v -= mean[c] * ysum[0] + mean[P + c] * ysum[1];
+    out[c] = v;
+  }
+}
+
+// ysum = [sum_{r < W} y_r, sum_{r >= W} y_r]; two workgroups
+__global__ __launch_bounds__(256) void k_half_sums(const double* __restrict__ y, int64_t W, double* __restrict__ ysum) {
+  __shared__ double red[4];
+  const int h = blockIdx.x;
+  double s = 0;
+  for (int64_t r = threadIdx.x; r < W; r += 256) s += y[h * W + r];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) ysum[h] = red[0] + red[1] + red[2] + red[3];
+}
+
+// mdot[h] = mean[h] . u over P columns; two workgroups of 1024 threads
+__global__ __launch_bounds__(1024) void k_mean_dot(const double* __restrict__ mean, int64_t P,
+                                                   const double* __restrict__ u, double* __restrict__ mdot) {
+  __shared__ double red[16];
+  const int h = blockIdx.x;
+  double s = 0;
+  for (int64_t c = threadIdx.x; c < P; c += 1024) s += mean[h * P + c] * u[c];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int q = 0; q < 16; ++q) t += red[q];
+    mdot[h] = t;
+  }
+}
+
+// Centring.  In place (O -= 1 mean^T per half) for the float64 / float32 Gram, which reads O;
+// for the bf16 Gram only the centred bf16 copy is written and O is left as it was (the
+// refinement and push-through apply the mean correction on the fly).  A thread takes 8
+// consecutive columns of one row (32 rows x 64 columns per workgroup).  With `blocked`, the
+// bf16 copy is laid out in 256-row x 64-column blocks, [row panel][column block][256][64], each
+// 32 KB contiguous, so the Gram streams a panel slice as one contiguous read instead of 256
+// scattered 128-byte pieces.
 template <typename T>
-int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, hipStream_t st) {
-  hipLaunchKernelGGL(k_colmean<T>, dim3((unsigned)cdiv(P, 256), 2), dim3(256), 0, st, O, ldo, W, P, mean);
+__global__ __launch_bounds__(256) void k_center(T* __restrict__ O, int64_t ldo, int64_t W, int64_t P,
+                                                const double* __restrict__ mean, __bf16* __restrict__ Ob,
+                                                int blocked, int inplace) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  const int64_t r = (int64_t)blockIdx.y * 32 + (threadIdx.x >> 3);
+  const int64_t k0 = (int64_t)blockIdx.x * 64 + (threadIdx.x & 7) * 8;
+  if (r >= 2 * W || k0 >= ldo) return;
+  const int half = r >= W ? 1 : 0;
+  const double* mrow = mean + (int64_t)half * P;
+  T* orow = O + r * ldo + k0;
+  v4 x0 = *reinterpret_cast<const v4*>(orow), x1 = *reinterpret_cast<const v4*>(orow + 4);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = k0 + e;
+    const T raw = e < 4 ? x0[e & 3] : x1[e & 3];
+    const T c = k < P ? (T)((double)raw - mrow[k < P ? k : 0]) : (T)0;
+    v[e] = (float)c;
+    if (e < 4) x0[e & 3] = c;
+    else x1[e & 3] = c;
+  }
+  if (inplace) {
+    *reinterpret_cast<v4*>(orow) = x0;
+    *reinterpret_cast<v4*>(orow + 4) = x1;
+  }
+  if (!Ob) return;
+  bf8v b;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) b[e] = (__bf16)v[e];
+  int64_t off;
+  if (blocked) {
+    const int64_t nkb = ldo / 64;
+    off = (((r >> 8) * nkb + (k0 >> 6)) * 256 + (r & 255)) * 64 + (k0 & 63);
+  } else {
+    off = r * ldo + k0;
+  }
+  *reinterpret_cast<bf8v*>(Ob + off) = b;
+}
+
+bool gram_bf16_blocked(int64_t n, int64_t K) { return n % 256 == 0 && K % 64 == 0; }
+
+int64_t colsum_part_elems(int64_t ldo) { return 2LL * kSplit * ldo; }
+
+template <typename T>
+int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, int inplace, double* part,
+                 hipStream_t st) {
+  NQS_REQUIRE(ldo % 64 == 0, NQS_ERR_ARG, "ldo must be a multiple of 64");
+  hipLaunchKernelGGL(k_colsum_part<T>, dim3((unsigned)cdiv(ldo, 1024), 2 * kSplit), dim3(256), 0, st, O, ldo, W,
+                     nullptr, part);
   NQS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_center<T>, dim3((unsigned)cdiv(ldo, 256), (unsigned)(2 * W)), dim3(256), 0, st, O, ldo, W, P,
-                     mean, Ob);
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, st, part, ldo, P, 0,
+                     1.0 / (double)W, nullptr, nullptr, mean, P);
+  NQS_LAUNCH_CHECK();
+  if (!Ob && !inplace) return NQS_OK;
+  const int blocked = gram_bf16_blocked(2 * W, ldo) ? 1 : 0;
+  hipLaunchKernelGGL(k_center<T>, dim3((unsigned)cdiv(ldo, 64), (unsigned)cdiv(2 * W, 32)), dim3(256), 0, st, O, ldo,
+                     W, P, mean, Ob, blocked, inplace);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// out = (O - 1 mean^T)^T y over the 2W rows (mean null: O already centred).  ysum: 2 doubles.
+// out gets nout entries, zero beyond P.
+template <typename T>
+int gemv_cols_c(const T* O, int64_t ldo, int64_t W, int64_t P, const double* y, const double* mean, double* part,
+                double* ysum, double* out, int64_t nout, hipStream_t st) {
+  hipLaunchKernelGGL(k_colsum_part<T>, dim3((unsigned)cdiv(ldo, 1024), 2 * kSplit), dim3(256), 0, st, O, ldo, W, y,
+                     part);
+  NQS_LAUNCH_CHECK();
+  if (mean) {
+    hipLaunchKernelGGL(k_half_sums, dim3(2), dim3(256), 0, st, y, W, ysum);
+    NQS_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(nout > P ? nout : P, 256)), dim3(256), 0, st, part, ldo, P, 1,
+                     1.0, mean, ysum, out, nout);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// v[r] = sum_{c < P} O[r][c] u[c] - (mean ? mdot[half(r)] : 0), 8 rows per workgroup so that u
+// is read once per 8 rows; columns in 4-wide vectors (pad columns masked out).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_rows8(const T* __restrict__ O, int64_t ldo, int64_t nrow, int64_t W,
+                                                    int64_t P, const double* __restrict__ u,
+                                                    const double* __restrict__ mdot, double* __restrict__ v) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  __shared__ double red[4][8];
+  const int64_t r0 = (int64_t)blockIdx.x * 8;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t c = (int64_t)threadIdx.x * 4; c < ldo; c += 1024) {
+    const d4 uu = *reinterpret_cast<const d4*>(u + c);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t r = r0 + q < nrow ? r0 + q : nrow - 1;
+      const v4 x = *reinterpret_cast<const v4*>(O + r * ldo + c);
+      const double x0 = c < P ? (double)x[0] : 0.0, x1 = c + 1 < P ? (double)x[1] : 0.0;
+      const double x2 = c + 2 < P ? (double)x[2] : 0.0, x3 = c + 3 < P ? (double)x[3] : 0.0;
+      acc[q] += x0 * uu[0] + x1 * uu[1] + x2 * uu[2] + x3 * uu[3];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double s = wave_sum(acc[q]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int64_t r = r0 + threadIdx.x;
+    if (r < nrow) {
+      double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+      if (mdot) t -= mdot[r >= W ? 1 : 0];
+      v[r] = t;
+    }
+  }
+}
+
+// v = (O - 1 mean^T) u; u must hold ldo entries (zero beyond P) and be 32-byte aligned.
+template <typename T>
+int gemv_rows_c(const T* O, int64_t ldo, int64_t W, int64_t P, const double* u, const double* mean, double* mdot,
+                double* v, hipStream_t st) {
+  if (mean) {
+    hipLaunchKernelGGL(k_mean_dot, dim3(2), dim3(1024), 0, st, mean, P, u, mdot);
+    NQS_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_gemv_rows8<T>, dim3((unsigned)cdiv(2 * W, 8)), dim3(256), 0, st, O, ldo, 2 * W, W, P, u,
+                     mean ? mdot : nullptr, v);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }
@@ -158,24 +346,54 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A,
 // 256 x 256 x 64 block tile on 8 waves (4 x 2), each owning a 64 x 128 sub-tile (2 x 4 blocks of
 // v_mfma_f32_32x32x16_bf16, 128 accumulators per lane); each thread stages 4 + 4 16-byte vectors
 // per stage, so accumulators, staging and fragments fit in registers at 2 waves per SIMD.
-// Workgroup b runs on XCD b % 8; tiles are dealt so that each XCD walks a contiguous stretch of
-// the row-major lower triangle, whose concurrently resident tiles share their row panel in that
-// XCD's L2.
+//
+// Tile order.  Every tile streams two 256-row panels over all of K, so HBM traffic is set by how
+// many concurrently running tiles share panels in an XCD's L2.  Tiles are grouped in 8 x 8
+// super-tiles (lower triangle of super-tiles, row-major; a diagonal super-tile keeps its lower
+// triangle), row-major inside a super-tile, and each XCD (workgroup b runs on XCD b % 8) takes a
+// contiguous stretch of that sequence: the 32 workgroups an XCD runs at once then cover a 4 x 8
+// block reading 12 panels, where a row-major walk of the triangle reads 33.
 #define G2_BM 256
 #define G2_BK 64
 #define G2_LD (G2_BK + 8)
+// Position t of the super-tile sequence -> tile (bm, bn) of the padded grid; positions whose tile
+// lies outside the lower triangle of the nt x nt grid are skipped by the caller.
+constexpr int SUPER = 8;
+__device__ inline void supertile_of(int64_t t, int& bm, int& bn) {
+  // super row I holds I full super-tiles (SUPER^2 tiles each) and one diagonal one
+  // (SUPER (SUPER + 1) / 2 tiles): prefix(I) = SUPER^2 I (I - 1) / 2 + SUPER (SUPER + 1) / 2 I
+  constexpr int64_t F = SUPER * SUPER, D = SUPER * (SUPER + 1) / 2;
+  int I = 0;
+  while (F * (int64_t)(I + 1) * I / 2 + D * (I + 1) <= t) ++I;
+  const int64_t o = t - (F * (int64_t)I * (I - 1) / 2 + D * I);
+  if (o < F * I) {
+    const int J = (int)(o / F), u = (int)(o % F);
+    bm = SUPER * I + u / SUPER;
+    bn = SUPER * J + u % SUPER;
+  } else {
+    const int u = (int)(o - F * I);  // lower triangle of the diagonal super-tile, row-major
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= u) ++r;
+    bm = SUPER * I + r;
+    bn = SUPER * I + (u - r * (r + 1) / 2);
+  }
+}
+static int64_t supertile_count(int64_t nt) {
+  const int64_t ns = (nt + SUPER - 1) / SUPER;
+  return (int64_t)SUPER * SUPER * ns * (ns - 1) / 2 + (int64_t)SUPER * (SUPER + 1) / 2 * ns;
+}
+
 __global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict__ A, int64_t lda, int64_t K,
-                                                       float* __restrict__ C, int64_t ldc, int64_t ntiles) {
+                                                       float* __restrict__ C, int64_t ldc, int64_t ntiles, int nt) {
   extern __shared__ __bf16 g2s[];
   __bf16(*As)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s);
   __bf16(*Bs)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s + 2 * G2_BM * G2_LD);
   const int64_t per_xcd = (ntiles + 7) / 8;
   const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (t >= ntiles) return;
-  int bm = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((int64_t)(bm + 1) * (bm + 2) / 2 <= t) ++bm;
-  while ((int64_t)bm * (bm + 1) / 2 > t) --bm;
-  const int bn = (int)(t - (int64_t)bm * (bm + 1) / 2);
+  int bm, bn;
+  supertile_of(t, bm, bn);
+  if (bm >= nt || bn > bm) return;
   const int64_t m0 = (int64_t)bm * G2_BM, n0 = (int64_t)bn * G2_BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -186,36 +404,38 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict_
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  // staging: thread covers row tid / 8 + 64 q (q = 0..3), 16-byte chunk tid % 8
-  uint4 ra[4], rb[4];
+  // staging: thread covers row tid / 8 + 64 q (q = 0..3), 16-byte chunk tid % 8; A is in the
+  // blocked layout of k_center, so the k-step kb of a panel is the contiguous block
+  // A + (panel * nkb + kb) * 256 * 64
+  u4v ra[4], rb[4];  // ext vectors: HIP's uint4 struct would be demoted to scratch here
   const int srow = tid >> 3, scol = (tid & 7) * 8;
-  const __bf16* pa = A + (m0 + srow) * lda + scol;
-  const __bf16* pb = A + (n0 + srow) * lda + scol;
-  const int64_t rstep = 64 * lda;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    ra[q] = *reinterpret_cast<const uint4*>(pa + q * rstep);
-    rb[q] = *reinterpret_cast<const uint4*>(pb + q * rstep);
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    *reinterpret_cast<uint4*>(&As[0][srow + 64 * q][scol]) = ra[q];
-    *reinterpret_cast<uint4*>(&Bs[0][srow + 64 * q][scol]) = rb[q];
-  }
-  __syncthreads();
+  const int64_t nkb = K / G2_BK;
+  const __bf16* pa = A + (int64_t)bm * nkb * (G2_BM * G2_BK) + srow * G2_BK + scol;
+  const __bf16* pb = A + (int64_t)bn * nkb * (G2_BM * G2_BK) + srow * G2_BK + scol;
+  const int64_t rstep = 64 * G2_BK;
   const int64_t nk = K / G2_BK;
+  // Software pipeline over k-steps: the registers carry stage kb + 1 across the loop back edge,
+  // so its loads are issued a whole stage of MFMA work before they are stored to LDS (loads
+  // consumed within one iteration get sunk next to their stores by the compiler).
+#define NQS_G2_LOAD(kb_)                                                  \
+  {                                                                       \
+    const int64_t k0 = (int64_t)(kb_) * (G2_BM * G2_BK);                  \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                       \
+      ra[q] = *reinterpret_cast<const u4v*>(pa + q * rstep + k0);       \
+      rb[q] = *reinterpret_cast<const u4v*>(pb + q * rstep + k0);       \
+    }                                                                     \
+  }
+#define NQS_G2_STORE(buf)                                                        \
+  _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                \
+    *reinterpret_cast<u4v*>(&As[buf][srow + 64 * q][scol]) = ra[q];            \
+    *reinterpret_cast<u4v*>(&Bs[buf][srow + 64 * q][scol]) = rb[q];            \
+  }
+  NQS_G2_LOAD(0)
+  NQS_G2_STORE(0)
+  NQS_G2_LOAD(nk > 1 ? 1 : 0)
+  __syncthreads();
   for (int64_t kb = 0; kb < nk; ++kb) {
     const int cur = kb & 1;
-    // prefetch the next stage unconditionally (the last iteration re-reads its own stage into
-    // the idle buffer), so the staging arrays never sit under a branch and stay in registers
-    {
-      const int64_t k0 = (kb + 1 < nk ? kb + 1 : kb) * G2_BK;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        ra[q] = *reinterpret_cast<const uint4*>(pa + q * rstep + k0);
-        rb[q] = *reinterpret_cast<const uint4*>(pb + q * rstep + k0);
-      }
-    }
 #pragma unroll
     for (int kk = 0; kk < G2_BK; kk += 16) {
       bf8v a[2], b[4];
@@ -230,13 +450,14 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict_
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      *reinterpret_cast<uint4*>(&As[cur ^ 1][srow + 64 * q][scol]) = ra[q];
-      *reinterpret_cast<uint4*>(&Bs[cur ^ 1][srow + 64 * q][scol]) = rb[q];
-    }
+    // stage kb + 1 (loaded one iteration ago) into the idle buffer, then issue stage kb + 2
+    // (clamped: the tail re-reads the last stage, which is never consumed)
+    NQS_G2_STORE(cur ^ 1)
+    NQS_G2_LOAD(kb + 2 < nk ? kb + 2 : nk - 1)
     __syncthreads();
   }
+#undef NQS_G2_LOAD
+#undef NQS_G2_STORE
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -251,9 +472,10 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict_
 
 int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st) {
   NQS_REQUIRE(n % GB_BM == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs 2W to be a multiple of 128");
-  if (n % G2_BM == 0 && K % G2_BK == 0 && lda % 8 == 0 && ((uintptr_t)A & 15) == 0) {
+  NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
+  if (gram_bf16_blocked(n, K)) {  // A in the blocked layout written by k_center
     const int64_t nt = n / G2_BM;
-    const int64_t tiles = nt * (nt + 1) / 2;
+    const int64_t tiles = supertile_count(nt);
     const size_t lds = sizeof(__bf16) * 4 * G2_BM * G2_LD;
     static bool attr = false;
     if (!attr) {
@@ -262,7 +484,7 @@ int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int6
       attr = true;
     }
     hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(512), lds, st, A, lda, K, C, ldc,
-                       tiles);
+                       tiles, (int)nt);
     NQS_LAUNCH_CHECK();
     return NQS_OK;
   }
@@ -572,56 +794,6 @@ int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, doub
 }
 
 // ------------------------------------------------------------------ GEMVs over O and T
-// y[r] = alpha sum_c A[r][c] x[c] + beta y[r]; one workgroup per row
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_rows(const T* __restrict__ A, int64_t lda, int64_t ncol,
-                                                   const double* __restrict__ x, double* __restrict__ y, double alpha,
-                                                   double beta, int sym_lower) {
-  __shared__ double red[8];
-  const int64_t r = blockIdx.x;
-  double s = 0;
-  if (!sym_lower) {
-    for (int64_t c = threadIdx.x; c < ncol; c += blockDim.x) s += (double)A[r * lda + c] * x[c];
-  } else {
-    // symmetric matrix stored in the lower triangle: row part c <= r, column part c > r
-    for (int64_t c = threadIdx.x; c < ncol; c += blockDim.x)
-      s += (double)(c <= r ? A[r * lda + c] : A[c * lda + r]) * x[c];
-  }
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
-    y[r] = alpha * t + (beta != 0.0 ? beta * y[r] : 0.0);
-  }
-}
-// y[c] = sum_r A[r][c] x[r]; one thread per column
-template <typename T>
-__global__ void k_gemv_cols(const T* __restrict__ A, int64_t lda, int64_t nrow, int64_t ncol,
-                            const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  double s = 0;
-  for (int64_t r = 0; r < nrow; ++r) s += (double)A[r * lda + c] * x[r];
-  y[c] = s;
-}
-
-template <typename T>
-int gemv_rows(const T* A, int64_t lda, int64_t nrow, int64_t ncol, const double* x, double* y, double alpha,
-              double beta, int sym_lower, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_rows<T>, dim3((unsigned)nrow), dim3(256), 0, st, A, lda, ncol, x, y, alpha, beta,
-                     sym_lower);
-  NQS_LAUNCH_CHECK();
-  return NQS_OK;
-}
-template <typename T>
-int gemv_cols(const T* A, int64_t lda, int64_t nrow, int64_t ncol, const double* x, double* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_cols<T>, dim3((unsigned)cdiv(ncol, 256)), dim3(256), 0, st, A, lda, nrow, ncol, x, y);
-  NQS_LAUNCH_CHECK();
-  return NQS_OK;
-}
-
 __global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, const double* __restrict__ y,
                        double b, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -709,15 +881,17 @@ int make_etil(const double* e, int64_t W, double* et, hipStream_t st) {
   return NQS_OK;
 }
 
-template int minsr_center<double>(double*, int64_t, int64_t, int64_t, double*, __bf16*, hipStream_t);
-template int minsr_center<float>(float*, int64_t, int64_t, int64_t, double*, __bf16*, hipStream_t);
+template int minsr_center<double>(double*, int64_t, int64_t, int64_t, double*, __bf16*, int, double*, hipStream_t);
+template int minsr_center<float>(float*, int64_t, int64_t, int64_t, double*, __bf16*, int, double*, hipStream_t);
+template int gemv_cols_c<double>(const double*, int64_t, int64_t, int64_t, const double*, const double*, double*,
+                                 double*, double*, int64_t, hipStream_t);
+template int gemv_cols_c<float>(const float*, int64_t, int64_t, int64_t, const double*, const double*, double*,
+                                double*, double*, int64_t, hipStream_t);
+template int gemv_rows_c<double>(const double*, int64_t, int64_t, int64_t, const double*, const double*, double*,
+                                 double*, hipStream_t);
+template int gemv_rows_c<float>(const float*, int64_t, int64_t, int64_t, const double*, const double*, double*,
+                                double*, hipStream_t);
 template int to_f64_shift<double>(const double*, int64_t, int64_t, double, double*, int64_t, hipStream_t);
 template int to_f64_shift<float>(const float*, int64_t, int64_t, double, double*, int64_t, hipStream_t);
-template int gemv_rows<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, double, double, int,
-                               hipStream_t);
-template int gemv_rows<float>(const float*, int64_t, int64_t, int64_t, const double*, double*, double, double, int,
-                              hipStream_t);
-template int gemv_cols<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, hipStream_t);
-template int gemv_cols<float>(const float*, int64_t, int64_t, int64_t, const double*, double*, hipStream_t);
 
 }  // namespace nqs

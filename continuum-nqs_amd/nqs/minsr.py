@@ -7,8 +7,8 @@
   train_minsr(model, params, L, n_elec, key, ...)                  (the VMC loop, one step = VMCStep)
 
 Hot-path layout: the O matrix is kept as the real (2W, ldo) array [Re O; Im O] that
-nqs_psiformer_grad writes (float64 for precision="fp64" models, float32 for "mixed"), centred
-in place and consumed by nqs_minsr_direction.  Gram precision: "bf16" (MFMA bf16 in, f32
+nqs_psiformer_grad writes (float64 for precision="fp64" models, float32 for "mixed") and
+nqs_minsr_direction consumes (read only with the bf16 Gram, centred in place otherwise).  Gram precision: "bf16" (MFMA bf16 in, f32
 accumulate), "fp32" or "fp64"; the Cholesky, the solves and the refinement step are float64.
 """
 import ctypes
@@ -103,8 +103,9 @@ _GRAM = {"fp64": 0, "fp32": 1, "bf16": 2}
 
 
 def minsr_direction(O, eps, lam, gram="auto", refine=1):
-    """dtheta = Otil^T (Otil Otil^T + lam I)^-1 etil (minsr.py:76-81).  O: an OMatrix (centred in
-    place) or a (W, P) complex tensor."""
+    """dtheta = Otil^T (Otil Otil^T + lam I)^-1 etil (minsr.py:76-81).  O: an OMatrix or a (W, P)
+    complex tensor; an OMatrix is left unchanged by the bf16 Gram and centred in place by the
+    fp32 / fp64 Gram."""
     lib = _lib.load()
     dev = _lib.device()
     if not isinstance(O, OMatrix):

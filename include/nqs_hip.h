@@ -169,7 +169,9 @@ typedef struct nqs_minsr_desc {
   double lam;        /* diagonal shift added to the unnormalised Gram */
 } nqs_minsr_desc;
 int64_t nqs_minsr_workspace_bytes(const nqs_minsr_desc* d);
-/* O_dev (2W, ldo) is centred in place.  eps_dev (W) complex local energies (already clipped);
+/* O_dev (2W, ldo): with the bf16 Gram it is read only (the centring is applied on the fly); with
+ * the float64 / float32 Gram it is centred in place.  Pad columns P..ldo-1 are ignored.
+ * eps_dev (W) complex local energies (already clipped);
  * dtheta_dev (P) float64; info_dev (int32, device): 0, or 1 + the first non-positive pivot. */
 int nqs_minsr_direction(const nqs_minsr_desc* d, void* O_dev, const double* eps_dev, double* dtheta_dev,
                         int32_t* info_dev, void* ws_dev, int64_t ws_bytes, void* stream);

@@ -96,14 +96,16 @@ def test_minsr_matches_oracle_and_paramsr(lam):
     assert float(torch.max(torch.abs(d_gpu_ps - d_ps))) / den < 1e-6
 
 
-@pytest.mark.parametrize("gram,tol", [("fp32", 1e-3), ("bf16", 5e-2)])
-def test_low_precision_gram_close_to_fp64(gram, tol):
+@pytest.mark.parametrize("gram,tol,W", [("fp32", 1e-3, 64), ("bf16", 5e-2, 64), ("fp32", 1e-3, 384),
+                                        ("bf16", 5e-2, 384)])
+def test_low_precision_gram_close_to_fp64(gram, tol, W):
     """bf16 / f32 MFMA Gram (f32 accumulate) + float64 Cholesky + one refinement step from the
-    float32 O: update vector within tol (relative, max norm) of the float64 MinSR."""
+    float32 O: update vector within tol (relative, max norm) of the float64 MinSR.  W = 384
+    (2W = 768 = 3 x 256) takes the 256-tile bf16 Gram on the blocked operand layout."""
     g, arch, theta, m64 = _pair("base", seed=5)
     _, _, _, m32 = _pair("base", precision="mixed", seed=5)
-    R, S = _configs(64, 5, 7.0, 5)
-    eps = torch.tensor(np.random.default_rng(5).normal(size=64) + 0j)
+    R, S = _configs(W, 5, 7.0, 5)
+    eps = torch.tensor(np.random.default_rng(5).normal(size=W) + 0j)
     O64, _ = nms.grad_params(make_logpsi(m64, theta.cuda()), R, S)
     O32, _ = nms.grad_params(make_logpsi(m32, theta.cuda()), R, S)
     d64 = nms.minsr_direction(O64, eps, 1e-3, gram="fp64").cpu()
