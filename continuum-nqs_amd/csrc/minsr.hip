@@ -639,54 +639,76 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
   return NQS_OK;
 }
 
-// ------------------------------------------------------------------ Cholesky (lower, float64, nb = 64)
+// ------------------------------------------------------------------ Cholesky (lower, float64)
+// Right-looking over outer panels of CH_NBO = 256 columns (rank-256 trailing updates on the f64
+// MFMA GEMM, lower tiles only); inside an outer panel, left-looking over CH_NB = 64 columns:
+// factor the 64 x 64 diagonal block, solve the rows below it with its inverse, update the rest
+// of the outer panel.
 #define CH_NB 64
-// Factor the nb x nb diagonal block at (j0, j0) in place (lower), store its inverse (lower)
-// in Linv (nb x nb).  info[0] = first non-positive pivot + 1 (0 when the block is SPD).
-__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0, int nb,
-                                                    double* __restrict__ Linv, int* __restrict__ info) {
-  __shared__ double L[CH_NB][CH_NB + 1];
-  __shared__ double X[CH_NB][CH_NB + 1];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    L[r][c] = (c <= r) ? A[(j0 + r) * lda + j0 + c] : 0.0;
-  }
-  __syncthreads();
-  for (int c = 0; c < nb; ++c) {
-    if (tid == 0) {
-      const double d = L[c][c];
-      if (!(d > 0.0)) {
-        if (info[0] == 0) info[0] = (int)(j0 + c + 1);
-        L[c][c] = 1.0;
-      } else {
-        L[c][c] = sqrt(d);
-      }
+#define CH_NBO 256
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Factor the 64 x 64 diagonal block at (j0, j0) in place (lower) and store its inverse (lower,
+// row-major) in Linv, with one wave: lane i keeps row i in registers; column k's multipliers
+// reach the other lanes by v_readlane, so no step needs a barrier.  The inverse is formed by
+// forward substitution, column c in lane c, reading L from LDS (the same address on every lane).
+// info[0] = first non-positive pivot + 1 (the pivot is then taken as 1 and the factor goes on).
+__global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0,
+                                                   double* __restrict__ Linv, int* __restrict__ info) {
+  __shared__ double Ls[CH_NB][CH_NB + 1];
+  const int i = threadIdx.x;
+  double a[CH_NB];
+  double* row = A + (j0 + i) * lda + j0;
+#pragma unroll
+  for (int k = 0; k < CH_NB; ++k) a[k] = row[k];
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < CH_NB; ++k) {
+    double d = readlane_d(a[k], k);
+    if (!(d > 0.0)) {
+      if (!bad) bad = k + 1;
+      d = 1.0;
     }
-    __syncthreads();
-    for (int r = c + 1 + tid; r < nb; r += blockDim.x) L[r][c] /= L[c][c];
-    __syncthreads();
-    const int w = nb - c - 1;
-    for (int e = tid; e < w * w; e += blockDim.x) {
-      const int r = c + 1 + e / w, k = c + 1 + e % w;
-      if (k <= r) L[r][k] -= L[r][c] * L[k][c];
-    }
-    __syncthreads();
-  }
-  // X = L^-1 by forward substitution, column c by thread c
-  for (int c = tid; c < nb; c += blockDim.x) {
-    for (int r = 0; r < nb; ++r) {
-      double s = (r == c) ? 1.0 : 0.0;
-      for (int k = c; k < r; ++k) s -= L[r][k] * X[k][c];
-      X[r][c] = (r >= c) ? s / L[r][r] : 0.0;
+    const double lkk = sqrt(d), inv = 1.0 / lkk;
+    a[k] = i == k ? lkk : (i > k ? a[k] * inv : a[k]);
+#pragma unroll
+    for (int j = k + 1; j < CH_NB; ++j) {
+      const double ljk = readlane_d(a[k], j);
+      a[j] = i >= j ? a[j] - a[k] * ljk : a[j];
     }
   }
-  __syncthreads();
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    A[(j0 + r) * lda + j0 + c] = (c <= r) ? L[r][c] : 0.0;
-    Linv[e] = X[r][c];
+  if (bad && i == 0 && info[0] == 0) info[0] = (int)(j0 + bad);
+#pragma unroll
+  for (int k = 0; k < CH_NB; ++k) {
+    const double v = k <= i ? a[k] : 0.0;
+    row[k] = v;
+    Ls[i][k] = v;
   }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int c = i;
+  double x[CH_NB];
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r) {
+    double s0 = r == c ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int k = 0; k + 3 < r; k += 4) {
+      s0 -= Ls[r][k] * x[k];
+      s1 -= Ls[r][k + 1] * x[k + 1];
+      s2 -= Ls[r][k + 2] * x[k + 2];
+      s3 -= Ls[r][k + 3] * x[k + 3];
+    }
+#pragma unroll
+    for (int k = r & ~3; k < r; ++k) s0 -= Ls[r][k] * x[k];
+    x[r] = ((s0 + s1) + (s2 + s3)) / Ls[r][r];
+  }
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r) Linv[r * CH_NB + c] = x[r];
 }
 
 __global__ void k_copy_panel(const double* __restrict__ P, int64_t m, int nb, double* __restrict__ A, int64_t lda,
@@ -725,22 +747,34 @@ static GemmArgs<double> g64(const double* A, const double* B, double* C, int64_t
 // panel: n x nb scratch.
 int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st) {
   NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "Cholesky size must be a multiple of 64");
-  for (int64_t j0 = 0; j0 < n; j0 += CH_NB) {
-    double* Li = Linv + (j0 / CH_NB) * CH_NB * CH_NB;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, st, A, lda, j0, CH_NB, Li, info);
-    NQS_LAUNCH_CHECK();
-    const int64_t m = n - j0 - CH_NB;
-    if (m == 0) break;
-    // panel = A21 Linv^T  (NT GEMM with B = Linv, K = nb)
-    NQS_TRY(gemm_nt<double>(g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0),
-                            1, st));
-    // A22 -= panel panel^T (lower tiles)
-    NQS_TRY(gemm_nt<double>(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, m, CH_NB, CH_NB, CH_NB, lda, -1.0,
-                                1.0, 1),
-                            1, st));
-    hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256)), dim3(256), 0, st, panel, m, CH_NB, A, lda,
-                       j0 + CH_NB, j0);
-    NQS_LAUNCH_CHECK();
+  for (int64_t J0 = 0; J0 < n; J0 += CH_NBO) {
+    const int64_t nbo = n - J0 < CH_NBO ? n - J0 : CH_NBO;
+    for (int64_t j0 = J0; j0 < J0 + nbo; j0 += CH_NB) {
+      double* Li = Linv + (j0 / CH_NB) * CH_NB * CH_NB;
+      hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, A, lda, j0, Li, info);
+      NQS_LAUNCH_CHECK();
+      const int64_t m = n - j0 - CH_NB;
+      if (m == 0) break;
+      // panel = A21 Linv^T  (NT GEMM with B = Linv, K = 64)
+      NQS_TRY(gemm_nt<double>(
+          g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0), 1, st));
+      // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= panel panel[0:w]^T (lower part)
+      const int64_t w = J0 + nbo - (j0 + CH_NB);
+      if (w > 0)
+        NQS_TRY(gemm_nt<double>(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NB, CH_NB, lda,
+                                    -1.0, 1.0, 1),
+                                1, st));
+      hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256)), dim3(256), 0, st, panel, m, CH_NB, A,
+                         lda, j0 + CH_NB, j0);
+      NQS_LAUNCH_CHECK();
+    }
+    // trailing update with the whole outer panel: A22 -= Lp Lp^T (lower tiles), K = nbo
+    const int64_t M = n - J0 - nbo;
+    if (M > 0) {
+      const double* Lp = A + (J0 + nbo) * lda + J0;
+      NQS_TRY(gemm_nt<double>(g64(Lp, Lp, A + (J0 + nbo) * lda + J0 + nbo, M, M, nbo, lda, lda, lda, -1.0, 1.0, 1), 1,
+                              st));
+    }
   }
   return NQS_OK;
 }
