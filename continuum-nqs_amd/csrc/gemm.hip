@@ -11,9 +11,6 @@
 #include "nqs_common.h"
 #include "kernels.h"
 
-#ifndef NQS_BF_NBUF
-#define NQS_BF_NBUF 2
-#endif
 
 namespace nqs {
 
@@ -226,7 +223,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(Gemm
 // (both round-to-nearest-even) as it is staged into LDS; SPLIT = 3 accumulates
 // hi*hi + hi*lo + lo*hi in float32 (products carry about 16 significant bits; the dropped lo*lo
 // term is below 2^-16 relative), SPLIT = 1 keeps hi*hi only (plain bf16).  Block tile
-// 128 x 128 x 32, 4 waves in 2 x 2, 64 x 64 per wave as 2 x 2 blocks of
+// 128 x 128 x 64, 4 waves in 2 x 2, 64 x 64 per wave as 2 x 2 blocks of
 // v_mfma_f32_32x32x16_bf16 (A[lane%32][8 (lane/32) + e], C row 8 (r/4) + 4 (lane/32) + r%4,
 // col lane%32).  Tiles are dealt XCD-aware: workgroup b runs on XCD b % 8, and the column
 // tiles of one row panel go to the same XCD back to back, so the panel is read from HBM once.
@@ -241,18 +238,22 @@ __device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& 
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
 }
 
-constexpr int BF_BM = 128, BF_BK = 32, BF_LD = BF_BK + 8;
+constexpr int BF_BM = 128, BF_BK = 64, BF_LD = BF_BK + 8;
 
-template <int SPLIT, int NBUF>
+template <int SPLIT>
 constexpr size_t bf_lds_bytes() {
-  return sizeof(__bf16) * (size_t)NBUF * 2 * (SPLIT == 1 ? 1 : 2) * BF_BM * BF_LD;
+  return sizeof(__bf16) * (size_t)2 * (SPLIT == 1 ? 1 : 2) * BF_BM * BF_LD;
 }
 
-template <int SPLIT, int NBUF>
+// Staging: a thread moves rows tid / 8 + 32 q (q = 0..3), k-chunk tid % 8 (8 floats, two 16-byte
+// loads) and writes its 8 hi and 8 lo bf16 as 16-byte LDS stores; 8 lanes fill one 128-byte row,
+// and the 144-byte row pitch keeps both the stores and the fragment reads free of bank conflicts.
+// One LDS buffer per operand and plane; the registers carry stage kb + 1 across the loop back
+// edge (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
+template <int SPLIT>
 __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
   constexpr int NP = SPLIT == 1 ? 1 : 2;
-  constexpr int PLANE = BF_BM * BF_LD;
-  constexpr int BUF = 2 * NP * PLANE;  // A planes, then B planes
+  constexpr int PLANE = BF_BM * BF_LD;  // A planes (hi, lo), then B planes
   extern __shared__ __bf16 bfs[];
   const int64_t per_xcd = (ntiles + 7) / 8;
   const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -269,8 +270,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  // staging: thread covers rows tid / 8 + 32 q (q = 0..3), float4 chunk tid % 8 of the 32-wide k slab
-  const int srow = tid >> 3, sk = (tid & 7) * 4;
+  const int srow = tid >> 3, sk = (tid & 7) * 8;
   const float* pa[4];
   const float* pb[4];
 #pragma unroll
@@ -279,46 +279,56 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
     pa[q] = g.A + (gm < g.M ? gm : g.M - 1) * g.lda;
     pb[q] = g.B + (gn < g.N ? gn : g.N - 1) * g.ldb;
   }
-  f4v ra[4], rb[4];
+  f4v ra[4][2], rb[4][2];
   const f4v zero = (f4v)0.f;
   const int64_t nk = cdiv_dev(g.K, BF_BK);
-#define NQS_BF_LOAD(k0)                                                  \
-  {                                                                      \
-    const int64_t gk = (k0) + sk, ck = gk < g.K ? gk : g.K - 4;          \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                      \
-      ra[q] = *reinterpret_cast<const f4v*>(pa[q] + ck);                 \
-      rb[q] = *reinterpret_cast<const f4v*>(pb[q] + ck);                 \
-    }                                                                    \
+  // K is a multiple of 4: each 4-wide half of a chunk is either wholly inside K or wholly out
+#define NQS_BF_LOAD(kb_)                                                        \
+  {                                                                             \
+    const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                             \
+    const int64_t c0 = g0 < g.K ? g0 : g.K - 4, c1 = g0 + 4 < g.K ? g0 + 4 : g.K - 4; \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                             \
+      ra[q][0] = *reinterpret_cast<const f4v*>(pa[q] + c0);                     \
+      ra[q][1] = *reinterpret_cast<const f4v*>(pa[q] + c1);                     \
+      rb[q][0] = *reinterpret_cast<const f4v*>(pb[q] + c0);                     \
+      rb[q][1] = *reinterpret_cast<const f4v*>(pb[q] + c1);                     \
+    }                                                                           \
   }
-#define NQS_BF_STORE(buf, k0)                                                                \
-  {                                                                                          \
-    const bool okk = (k0) + sk < g.K;                                                        \
-    __bf16* s = bfs + (buf) * BUF;                                                           \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                          \
-      const int row = srow + 32 * q;                                                         \
-      const f4v a = (okk && m0 + row < g.M) ? ra[q] : zero;                                  \
-      const f4v b = (okk && n0 + row < g.N) ? rb[q] : zero;                                  \
-      uint2 ah, al, bh, bl;                                                                  \
-      split_bf16x2(a[0], a[1], ah.x, al.x);                                                  \
-      split_bf16x2(a[2], a[3], ah.y, al.y);                                                  \
-      split_bf16x2(b[0], b[1], bh.x, bl.x);                                                  \
-      split_bf16x2(b[2], b[3], bh.y, bl.y);                                                  \
-      *reinterpret_cast<uint2*>(s + row * BF_LD + sk) = ah;                                  \
-      *reinterpret_cast<uint2*>(s + NP * PLANE + row * BF_LD + sk) = bh;                     \
-      if (NP == 2) {                                                                         \
-        *reinterpret_cast<uint2*>(s + PLANE + row * BF_LD + sk) = al;                        \
-        *reinterpret_cast<uint2*>(s + 3 * PLANE + row * BF_LD + sk) = bl;                    \
-      }                                                                                      \
-    }                                                                                        \
+#define NQS_BF_SPLIT8(x0, x1, hi, lo)                                             \
+  {                                                                               \
+    uint32_t h_[4], l_[4];                                                        \
+    split_bf16x2(x0[0], x0[1], h_[0], l_[0]);                                     \
+    split_bf16x2(x0[2], x0[3], h_[1], l_[1]);                                     \
+    split_bf16x2(x1[0], x1[1], h_[2], l_[2]);                                     \
+    split_bf16x2(x1[2], x1[3], h_[3], l_[3]);                                     \
+    hi = (u4v){h_[0], h_[1], h_[2], h_[3]};                                       \
+    lo = (u4v){l_[0], l_[1], l_[2], l_[3]};                                       \
+  }
+#define NQS_BF_STORE(kb_)                                                                     \
+  {                                                                                           \
+    const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                                           \
+    const bool ok0 = g0 < g.K, ok1 = g0 + 4 < g.K;                                            \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
+      const int row = srow + 32 * q;                                                          \
+      const bool am = m0 + row < g.M, bnn = n0 + row < g.N;                                   \
+      const f4v a0 = (ok0 && am) ? ra[q][0] : zero, a1 = (ok1 && am) ? ra[q][1] : zero;       \
+      const f4v b0 = (ok0 && bnn) ? rb[q][0] : zero, b1 = (ok1 && bnn) ? rb[q][1] : zero;     \
+      u4v ah, al, bh, bl;                                                                     \
+      NQS_BF_SPLIT8(a0, a1, ah, al)                                                           \
+      NQS_BF_SPLIT8(b0, b1, bh, bl)                                                           \
+      *reinterpret_cast<u4v*>(bfs + row * BF_LD + sk) = ah;                                   \
+      *reinterpret_cast<u4v*>(bfs + NP * PLANE + row * BF_LD + sk) = bh;                      \
+      if (NP == 2) {                                                                          \
+        *reinterpret_cast<u4v*>(bfs + PLANE + row * BF_LD + sk) = al;                         \
+        *reinterpret_cast<u4v*>(bfs + 3 * PLANE + row * BF_LD + sk) = bl;                     \
+      }                                                                                       \
+    }                                                                                         \
   }
   NQS_BF_LOAD(0)
-  NQS_BF_STORE(0, 0)
+  NQS_BF_STORE(0)
+  if (nk > 1) NQS_BF_LOAD(1)
   __syncthreads();
   for (int64_t kb = 0; kb < nk; ++kb) {
-    const int cur = NBUF == 2 ? (int)(kb & 1) : 0;
-    const int64_t knext = (kb + 1 < nk ? kb + 1 : kb) * BF_BK;
-    NQS_BF_LOAD(knext)
-    const __bf16* s = bfs + cur * BUF;
 #pragma unroll
     for (int kk = 0; kk < BF_BK; kk += 16) {
       bf8v ah[2], al[2], bh[2], bl[2];
@@ -327,11 +337,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
       for (int i = 0; i < 2; ++i) {
         const int ra_ = (wm * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
         const int rb_ = (wn * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
-        ah[i] = *reinterpret_cast<const bf8v*>(s + ra_);
-        bh[i] = *reinterpret_cast<const bf8v*>(s + NP * PLANE + rb_);
+        ah[i] = *reinterpret_cast<const bf8v*>(bfs + ra_);
+        bh[i] = *reinterpret_cast<const bf8v*>(bfs + NP * PLANE + rb_);
         if (NP == 2) {
-          al[i] = *reinterpret_cast<const bf8v*>(s + PLANE + ra_);
-          bl[i] = *reinterpret_cast<const bf8v*>(s + 3 * PLANE + rb_);
+          al[i] = *reinterpret_cast<const bf8v*>(bfs + PLANE + ra_);
+          bl[i] = *reinterpret_cast<const bf8v*>(bfs + 3 * PLANE + rb_);
         }
       }
 #pragma unroll
@@ -345,12 +355,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
-    if (NBUF == 1) __syncthreads();
-    NQS_BF_STORE(NBUF == 2 ? cur ^ 1 : 0, knext)
-    __syncthreads();
+    if (kb + 1 < nk) {  // uniform
+      __syncthreads();
+      NQS_BF_STORE(kb + 1)
+      if (kb + 2 < nk) NQS_BF_LOAD(kb + 2)
+      __syncthreads();
+    }
   }
 #undef NQS_BF_LOAD
 #undef NQS_BF_STORE
+#undef NQS_BF_SPLIT8
   // two-phase epilogue as in k_gemm_nt_v2
   const bool has_beta = g.beta != 0.f, has_bias = g.bias != nullptr;
   float addv[2][2][16];
@@ -400,22 +414,22 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
       }
 }
 
-template <int SPLIT, int NBUF>
+template <int SPLIT>
 static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
-  constexpr size_t lds = bf_lds_bytes<SPLIT, NBUF>();
+  constexpr size_t lds = bf_lds_bytes<SPLIT>();
   if (lds > 64 * 1024) {
     static bool attr = false;
     if (!attr) {
-      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_gemm_nt_bf<SPLIT, NBUF>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_gemm_nt_bf<SPLIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
       attr = true;
     }
   }
   const int nbn = (int)cdiv(g.N, BF_BM);
   const int64_t ntiles = cdiv(g.M, BF_BM) * nbn;
   NQS_REQUIRE(ntiles < (int64_t)1 << 31, NQS_ERR_UNSUPPORTED, "GEMM too large for one launch");
-  hipLaunchKernelGGL((k_gemm_nt_bf<SPLIT, NBUF>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(256), lds, st, g,
-                     nbn, ntiles);
+  hipLaunchKernelGGL((k_gemm_nt_bf<SPLIT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(256), lds, st, g, nbn,
+                     ntiles);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }
@@ -431,7 +445,7 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
     // triangular calls) run on the float32 kernels below
     NQS_REQUIRE(g.split == 0 || g.split == 1 || g.split == 3, NQS_ERR_ARG, "bf16 GEMM split must be 0, 1 or 3");
     if (g.split != 0 && vec_ok && batch == 1 && g.tri == 0 && !g.mirror)
-      return g.split == 3 ? launch_gemm_bf<3, NQS_BF_NBUF>(g, st) : launch_gemm_bf<1, NQS_BF_NBUF>(g, st);
+      return g.split == 3 ? launch_gemm_bf<3>(g, st) : launch_gemm_bf<1>(g, st);
   }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
