@@ -83,6 +83,17 @@ def cpu_baseline(sysd, n_walkers=8, seed=0):
                       f"stage seconds {json.dumps({k: round(v, 3) for k, v in tm.items()})}"}
 
 
+def gram_traffic(precision):
+    """HBM bytes per launch of the Gram kernel, measured with rocprofv3 PMC counters on this workload
+    (FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel); None when no measurement is present."""
+    f = os.path.join(ROOT, "profiles", "gram_hbm_traffic_n16_w8192.json")
+    try:
+        with open(f) as fh:
+            return float(json.load(fh)[precision]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def accuracy_vs_fp64(vs, sysd, coul, gemm):
     """Mixed vs float64 GPU path on the walkers and parameters of the last timed step."""
     from nqs.psiformer import ContinuumSpinorPsiformer, LogPsi
@@ -133,10 +144,21 @@ def main():
     ap.add_argument("--gemm", default="bf16x3", choices=["bf16x3", "fp32", "bf16"],
                     help="dense-layer MFMA mode of the mixed backbone")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (the calibration kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: runs on several GPUs are not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 << 28, "copies": 5}))
+        return
     from nqs import _lib
     from nqs.ewald import make_ewald
     from nqs.minsr import VMCStep
@@ -185,7 +207,8 @@ def main():
                       "gemm": args.gemm if args.precision == "mixed" else "fp64"},
            "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
                         "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / peak, "traffic": None, "gram_ms": float(mins[1])},
+                        "frac": achieved / peak,
+                        "traffic": gram_traffic(args.precision) if W == 8192 else None, "gram_ms": float(mins[1])},
            "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
                         "minsr": float(stages[3]), "update_repack": float(stages[4]),
                         "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
