@@ -150,6 +150,42 @@ __device__ inline void jastrow_u(const DetParams& p, bool same, double r, double
   }
 }
 
+// Pair distance of the Jastrow and its derivatives with respect to r_a (r = r_a - r_b), 2D:
+//   kinds 0, 1 (psiformer.py:164-170): the minimum-image distance d = |r - A round(A^-1 r)|,
+//     grad d = m / d, Lap d = 2/d - |m|^2/d^3 (with the 1e-12 regularisation inside d);
+//   kind 2 (no reference counterpart): the periodic chord distance of a rectangular cell,
+//     d^2 = sum_i (L_i/pi)^2 sin^2(pi x_i / L_i), equal to |r| near r = 0 (the Kato cusp is kept)
+//     and smooth everywhere else on the torus, so the local energy has no hidden delta terms on
+//     the cell boundary (the minimum image has a slope jump there).
+__device__ inline void pair_dist(const DetParams& p, double dx, double dy, double& d, double& gx, double& gy,
+                                 double& lapd) {
+  if (p.jastrow_kind != 2) {
+    double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
+    fx -= rint(fx);
+    fy -= rint(fy);
+    const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
+    const double d2 = mx * mx + my * my;
+    d = sqrt(d2 + 1e-12);
+    gx = mx / d;
+    gy = my / d;
+    lapd = 2.0 / d - d2 / (d * d * d);
+    return;
+  }
+  const double Lx = p.A[0], Ly = p.A[3];
+  double sx, cx, sy, cy, s2x, c2x, s2y, c2y;
+  sincos(M_PI * dx / Lx, &sx, &cx);
+  sincos(M_PI * dy / Ly, &sy, &cy);
+  sincos(2.0 * M_PI * dx / Lx, &s2x, &c2x);
+  sincos(2.0 * M_PI * dy / Ly, &s2y, &c2y);
+  const double D = (Lx / M_PI) * (Lx / M_PI) * sx * sx + (Ly / M_PI) * (Ly / M_PI) * sy * sy;
+  const double gDx = (Lx / M_PI) * s2x, gDy = (Ly / M_PI) * s2y;
+  const double lapD = 2.0 * c2x + 2.0 * c2y;
+  d = sqrt(D + 1e-12);
+  gx = gDx / (2.0 * d);
+  gy = gDy / (2.0 * d);
+  lapd = lapD / (2.0 * d) - (gDx * gDx + gDy * gDy) / (4.0 * d * d * d);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const double* __restrict__ R,
                                                  const int32_t* __restrict__ S, int64_t w0,
@@ -364,20 +400,16 @@ __global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const d
       for (int a = 0; a < N; ++a)
         for (int b = a + 1; b < N; ++b) {
           const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
-          fx -= rint(fx);
-          fy -= rint(fy);
-          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
-          const double d2 = mx * mx + my * my;
-          const double r = sqrt(d2 + 1e-12);
+          double r, grx, gry, lapr;
+          pair_dist(p, dx, dy, r, grx, gry, lapr);
           double u, u1, u2;
           jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
           J += u;
-          lJ += 2.0 * (u2 * d2 / (r * r) + u1 * (2.0 / r - d2 / (r * r * r)));
+          lJ += 2.0 * (u2 * (grx * grx + gry * gry) + u1 * lapr);
           for (int c = 1; c <= nt; ++c) {
             int ec, bc;
             tan_elec(c, ec, bc);
-            const double comp = (bc == 0 ? mx : my) * u1 / r;
+            const double comp = (bc == 0 ? grx : gry) * u1;
             if (ec == a) red[2 + 2 * (nt + 1) + c] += comp;
             if (ec == b) red[2 + 2 * (nt + 1) + c] -= comp;
           }
@@ -719,19 +751,15 @@ __global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, co
       for (int b = 0; b < N; ++b) {
         if (b == a) continue;
         const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-        double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
-        fx -= rint(fx);
-        fy -= rint(fy);
-        const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
-        const double d2 = mx * mx + my * my;
-        const double r = sqrt(d2 + 1e-12);
+        double r, grx, gry, lapr;
+        pair_dist(p, dx, dy, r, grx, gry, lapr);
         double u, u1, u2;
         jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
-        gxa += mx * u1 / r;
-        gya += my * u1 / r;
+        gxa += grx * u1;
+        gya += gry * u1;
         if (b > a) {
           J += u;
-          lJ += 2.0 * (u2 * d2 / (r * r) + u1 * (2.0 / r - d2 / (r * r * r)));
+          lJ += 2.0 * (u2 * (grx * grx + gry * gry) + u1 * lapr);
         }
       }
     }
@@ -877,11 +905,8 @@ __global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __re
       for (int a = 0; a < N; ++a)
         for (int b = a + 1; b < N; ++b) {
           const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
-          fx -= rint(fx);
-          fy -= rint(fy);
-          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
-          const double r = sqrt(mx * mx + my * my + 1e-12);
+          double r, grx, gry, lapr;
+          pair_dist(p, dx, dy, r, grx, gry, lapr);
           double u, u1, u2;
           jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
           J += u;
@@ -954,11 +979,8 @@ __global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __re
         for (int b = a + 1; b < N; ++b) {
           const bool same = spin[a] == spin[b];
           const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-          double fx = p.Ainv[0] * dx + p.Ainv[1] * dy, fy = p.Ainv[2] * dx + p.Ainv[3] * dy;
-          fx -= rint(fx);
-          fy -= rint(fy);
-          const double mx = p.A[0] * fx + p.A[1] * fy, my = p.A[2] * fx + p.A[3] * fy;
-          const double r = sqrt(mx * mx + my * my + 1e-12);
+          double r, grx, gry, lapr;
+          pair_dist(p, dx, dy, r, grx, gry, lapr);
           if (p.jastrow_kind == 0) {
             if (same != same_leaf) continue;
             const double mu = (p.n_knots > 1) ? p.rbf_rcut * (double)kk / (double)(p.n_knots - 1) : 0.0;

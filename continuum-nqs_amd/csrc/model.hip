@@ -56,6 +56,9 @@ int build_model(const nqs_model_desc* m, Model& md) {
   const double e00 = md.A[0] * md.Ainv[0] + md.A[1] * md.Ainv[2] - 1.0, e01 = md.A[0] * md.Ainv[1] + md.A[1] * md.Ainv[3];
   const double e10 = md.A[2] * md.Ainv[0] + md.A[3] * md.Ainv[2], e11 = md.A[2] * md.Ainv[1] + md.A[3] * md.Ainv[3] - 1.0;
   NQS_REQUIRE(fabs(e00) + fabs(e01) + fabs(e10) + fabs(e11) < 1e-10, NQS_ERR_ARG, "cell_inv is not the inverse of cell");
+  NQS_REQUIRE(!m->use_jastrow || (m->jastrow_kind >= 0 && m->jastrow_kind <= 2), NQS_ERR_ARG, "bad jastrow_kind");
+  NQS_REQUIRE(!(m->use_jastrow && m->jastrow_kind == 2) || (md.A[1] == 0.0 && md.A[2] == 0.0), NQS_ERR_UNSUPPORTED,
+              "the periodic cusp Jastrow needs a rectangular cell");
 
   // ---- leaves of the flax tree (psiformer.py:50-188)
   const int d = md.d, ffd = md.ff * md.d, N = md.N, K = md.K;

@@ -35,8 +35,8 @@ class ContinuumSpinorPsiformer:
         self.embed_dim, self.depth, self.n_heads, self.ff_mult = int(embed_dim), int(depth), int(n_heads), int(ff_mult)
         self.n_freq, self.n_dets = int(n_freq), int(n_dets)
         self.learnable_env, self.use_jastrow = bool(learnable_env), bool(use_jastrow)
-        if jastrow_kind not in ("rbf", "cusp"):
-            raise ValueError("jastrow_kind must be 'rbf' or 'cusp'")
+        if jastrow_kind not in ("rbf", "cusp", "cusp_periodic"):
+            raise ValueError("jastrow_kind must be 'rbf', 'cusp' or 'cusp_periodic'")
         self.jastrow_kind, self.n_knots, self.cusp = jastrow_kind, int(n_knots), float(cusp)
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -54,7 +54,7 @@ class ContinuumSpinorPsiformer:
         d.n_elec, d.n_g, d.embed_dim, d.depth = self.n_elec, self.Gs.shape[0], self.embed_dim, self.depth
         d.n_heads, d.ff_mult, d.n_freq, d.n_dets = self.n_heads, self.ff_mult, self.n_freq, self.n_dets
         d.learnable_env, d.use_jastrow = int(self.learnable_env), int(self.use_jastrow)
-        d.jastrow_kind = 0 if self.jastrow_kind == "rbf" else 1
+        d.jastrow_kind = {"rbf": 0, "cusp": 1, "cusp_periodic": 2}[self.jastrow_kind]
         d.n_knots = self.n_knots
         d.precision = _lib.NQS_FP64 if self.precision == "fp64" else _lib.NQS_MIXED
         d.gemm = {"bf16x3": _lib.NQS_GEMM_BF16X3, "fp32": _lib.NQS_GEMM_FP32, "bf16": _lib.NQS_GEMM_BF16}[self.gemm]

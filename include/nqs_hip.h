@@ -62,7 +62,9 @@ typedef struct nqs_model_desc {
   int32_t n_dets;        /* K generalized spinor determinants */
   int32_t learnable_env; /* adds the complex leaf dW (N, n_g, 2) */
   int32_t use_jastrow;
-  int32_t jastrow_kind;  /* 0 = rbf, 1 = cusp */
+  int32_t jastrow_kind;  /* 0 = rbf, 1 = cusp (both on the minimum-image distance, as the reference);
+                          2 = cusp on the smooth periodic chord distance (rectangular cells; no
+                          reference counterpart) */
   int32_t n_knots;
   int32_t precision;     /* NQS_FP64 or NQS_MIXED */
   int32_t gemm;          /* NQS_GEMM_* for NQS_MIXED */

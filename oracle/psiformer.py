@@ -204,13 +204,20 @@ def logpsi_from_params(a, p, R, S):
 
 
 def jastrow(a, p, R, S):
-    """psiformer.py:162-188: minimum-image pair distance (round), RBF or cusp pair term, i<j."""
+    """psiformer.py:162-188: minimum-image pair distance (round), RBF or cusp pair term, i<j.
+    jastrow_kind "cusp_periodic" (no reference counterpart): the cusp form on the smooth periodic
+    chord distance d^2 = sum_i (L_i/pi)^2 sin^2(pi x_i/L_i) of a rectangular cell."""
     A = torch.tensor(a.A, dtype=torch.float64)
     Ainv = torch.tensor(np.linalg.inv(a.A), dtype=torch.float64)
     d = R[:, None, :] - R[None, :, :]
-    dfrac = d @ Ainv.T
-    dmin = (dfrac - torch.round(dfrac).detach()) @ A.T
-    r = torch.sqrt((dmin * dmin).sum(-1) + 1e-12)
+    if a.jastrow_kind == "cusp_periodic":
+        Ls = torch.tensor([a.A[0, 0], a.A[1, 1]], dtype=torch.float64)
+        D = ((Ls / math.pi) ** 2 * torch.sin(math.pi * d / Ls) ** 2).sum(-1)
+        r = torch.sqrt(D + 1e-12)
+    else:
+        dfrac = d @ Ainv.T
+        dmin = (dfrac - torch.round(dfrac).detach()) @ A.T
+        r = torch.sqrt((dmin * dmin).sum(-1) + 1e-12)
     same = S[:, None] == S[None, :]
     if a.jastrow_kind == "rbf":
         area = abs(float(np.linalg.det(a.A)))

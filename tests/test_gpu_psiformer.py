@@ -24,6 +24,8 @@ ARCHS = {
     "kdet_jastrow_env": dict(embed_dim=16, depth=2, n_heads=2, n_freq=3, n_dets=3, use_jastrow=True,
                              learnable_env=True, n_knots=6),
     "cusp": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=2, use_jastrow=True, jastrow_kind="cusp"),
+    "cusp_periodic": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=1, use_jastrow=True,
+                          jastrow_kind="cusp_periodic"),
 }
 
 

@@ -176,3 +176,92 @@ def test_oracle_accept_rule():
     _, _, acc, _, _ = osa.sweep(logpsi_b, R, S, np.eye(2), 0.1, st["xi"][0], u, st["flip_i"][0], st["u_flip"][0],
                                 lp_cur=2 * L0)
     assert acc.tolist() == [True, True, False, False]
+
+
+# ------------------------------------------------------------------ two-electron exact diagonalisation
+def _relative_problem(L, n_cut, m_star, alpha):
+    """The lambda = h_z = 0 singlet reduced to one particle in the relative coordinate: plane waves
+    k = 2 pi n / L (n in the same disc), H[n, n'] = k^2/m* delta + alpha 2 pi/(L^2 |k - k'|)."""
+    from oracle import ed
+    v = ed.disc_vectors(n_cut)
+    k = 2.0 * np.pi * v / L
+    H = np.diag((k * k).sum(1) / m_star).astype(float)
+    q = np.linalg.norm(k[:, None, :] - k[None, :, :], axis=-1)
+    with np.errstate(divide="ignore"):
+        H += np.where(q > 0, alpha * 2.0 * np.pi / (L * L * np.where(q > 0, q, 1.0)), 0.0)
+    return float(np.linalg.eigvalsh(H)[0])
+
+
+@pytest.mark.parametrize("lr,hz,twist", [(0.5, 0.0, (0.0, 0.0)), (1.0, 0.5, (0.1, 0.17)), (0.3, 0.2, (0.4, -0.2))])
+def test_ed_noninteracting_limit_is_the_band_sum(lr, hz, twist):
+    """alpha = 0: the two-electron ground energy over all momentum sectors equals the sum of the two
+    lowest closed-form band energies (rashba_reference.py:49-54)."""
+    from oracle import ed
+    L, cut = 5.0, 4
+    g = bands.rashba_gs(L, 2, 1.0, lr, hz, n_cut=cut, twist=twist)
+    sectors = [(i, j) for i in range(-2, 3) for j in range(-2, 3)]
+    e = ed.ground_energy(L, cut, sectors=sectors, lambda_r=lr, h_z=hz, alpha=0.0, twist=twist)
+    assert abs(e - g["E_closed"]) < 1e-10
+
+
+@pytest.mark.parametrize("alpha", [0.3, 1.0])
+def test_ed_spin_free_singlet_equals_relative_coordinate_problem(alpha):
+    """lambda = h_z = 0: the antisymmetrised spinful ground state in the K = 0 sector is the singlet,
+    whose spatial part depends on r1 - r2 only; its energy must equal the lowest eigenvalue of the
+    one-body relative-coordinate problem in the same plane-wave set (independent construction:
+    no spins, no antisymmetrisation, reduced mass m*/2)."""
+    from oracle import ed
+    L = 2.0 * math.sqrt(2.0 * math.pi)
+    for cut in (4, 7):
+        HA, _ = ed.sector_hamiltonian(L, cut, (0, 0), 1.0, 0.0, 0.0, alpha)
+        assert abs(float(np.linalg.eigvalsh(HA)[0]) - _relative_problem(L, cut, 1.0, alpha)) < 1e-11
+
+
+def test_ed_rashba_sign_convention_invariance_and_variational_convergence():
+    """The spectrum cannot depend on the sign convention of the Rashba term (a global spin rotation);
+    the ground energy decreases monotonically with the cutoff (variational in the basis)."""
+    from oracle import ed
+    L = 2.0 * math.sqrt(2.0 * math.pi)
+    e_p = ed.ground_energy(L, 5, lambda_r=0.5, alpha=1.0)
+    e_m = ed.ground_energy(L, 5, lambda_r=-0.5, alpha=1.0)
+    assert abs(e_p - e_m) < 1e-11
+    es = [ed.ground_energy(L, c, lambda_r=0.5, alpha=1.0) for c in (3, 5, 7, 9)]
+    assert all(b < a for a, b in zip(es, es[1:]))
+    assert es[-2] - es[-1] < 1e-4
+
+
+def test_ed_madelung_matches_the_ewald_self_energy():
+    """zeta_M of the ED (converged sums) equals the Ewald restatement's (default sums) to 1e-12 and
+    is independent of the splitting parameter."""
+    from oracle import ed
+    L = 5.0
+    A = np.diag([L, L])
+    z = ed.madelung(A)
+    _, z0, _ = oew.make_ewald_lattice(A, 1.0)
+    _, z1, _ = oew.make_ewald_lattice(A, 1.0, ewald_alpha=0.7 * math.sqrt(math.pi) / L, n_real=9, n_recip=12)
+    assert abs(z - z0) < 1e-12 and abs(z - z1) < 1e-12
+
+
+def test_periodic_cusp_jastrow_is_smooth_periodic_and_cusped():
+    """The chord-distance cusp Jastrow (jastrow_kind "cusp_periodic"): periodic, |r| at short range,
+    and (unlike the minimum-image form) with a gradient that is continuous across the cell edge."""
+    L = 6.0
+    a = opf.Arch(2, np.zeros((2, 2)), np.zeros((2, 2, 2), complex), L, use_jastrow=True, jastrow_kind="cusp_periodic")
+    am = opf.Arch(2, np.zeros((2, 2)), np.zeros((2, 2, 2), complex), L, use_jastrow=True, jastrow_kind="cusp")
+    p = {"params/jastrow_loga": torch.tensor(0.3, dtype=torch.float64)}
+    S = torch.tensor([0, 1])
+
+    def J(arch, x, y):
+        R = torch.tensor([[0.0, 0.0], [x, y]], dtype=torch.float64)
+        return float(opf.jastrow(arch, p, R, S))
+
+    # periodicity and short-range agreement with the minimum-image form
+    assert abs(J(a, 1.3, 0.4) - J(a, 1.3 + L, 0.4 - L)) < 1e-12
+    assert abs(J(a, 0.05, 0.02) - J(am, 0.05, 0.02)) < 1e-5
+    # gradient across the edge x = L/2 (pair displacement crosses the Wigner-Seitz boundary)
+    def gx(arch, x):
+        h = 1e-6
+        return (J(arch, x + h, 0.7) - J(arch, x - h, 0.7)) / (2 * h)
+    jump_p = abs(gx(a, L / 2 + 1e-4) - gx(a, L / 2 - 1e-4))
+    jump_m = abs(gx(am, L / 2 + 1e-4) - gx(am, L / 2 - 1e-4))
+    assert jump_p < 1e-3 and jump_m > 1e-2
