@@ -144,6 +144,7 @@ def main():
     ap.add_argument("--gemm", default="bf16x3", choices=["bf16x3", "fp32", "bf16"],
                     help="dense-layer MFMA mode of the mixed backbone")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
+    ap.add_argument("--graph", action="store_true", help="replay the sweeps from a captured HIP graph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the calibration kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -168,7 +169,8 @@ def main():
     _, _, coul = make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
     gram = "bf16" if args.precision == "mixed" else "fp64"
     vs = VMCStep(sysd["model"], sysd["theta"], sysd["R"], sysd["S"], m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul,
-                 lr=0.05, lam=1e-4, n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1)
+                 lr=0.05, lam=1e-4, n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1,
+                 graph=args.graph)
     if args.burn:
         vs.chain.run(args.burn, 0.3 * RS, 2, key=7)
     lib = _lib.load()
@@ -204,7 +206,7 @@ def main():
            "data": "synthetic",
            "config": {"workload": "N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers=8192",
                       "walkers": W, "params": P, "sweeps": args.sweeps, "precision": args.precision, "gram": gram,
-                      "gemm": args.gemm if args.precision == "mixed" else "fp64"},
+                      "gemm": args.gemm if args.precision == "mixed" else "fp64", "graph_sweeps": bool(args.graph)},
            "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
                         "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / peak,

@@ -177,7 +177,7 @@ class VMCStep:
 
     def __init__(self, model, theta, R, S, m_star=1.0, lambda_r=1.0, h_z=0.0, ext_field=None, coulomb=None,
                  lr=0.05, lam=1e-3, n_sweeps=10, step_size=0.4, clip_k=5.0, n_spin_flips=2, gram="auto",
-                 refine=1):
+                 refine=1, graph=False):
         self.model = model
         self.theta = theta.to(_lib.device(), torch.float64).contiguous()
         self.lp = LogPsi(model, self.theta)
@@ -188,6 +188,8 @@ class VMCStep:
         self.n_spin_flips, self.gram, self.refine = n_spin_flips, gram, refine
         self.O = None
         self.timers = None
+        self.graph = graph      # replay the sweeps from a captured HIP graph (nqs.graphs.GraphSweeps)
+        self._gs = None
 
     def _ev(self):
         e = torch.cuda.Event(enable_timing=True)
@@ -203,7 +205,13 @@ class VMCStep:
 
     def step(self, key=0):
         t = [self._ev()] if self.timers is not None else None
-        ap, _ = self.chain.run(self.n_sweeps, self.step_size, self.n_spin_flips, key=key)
+        if self.graph:
+            if self._gs is None:
+                from .graphs import GraphSweeps
+                self._gs = GraphSweeps(self.chain, self.n_sweeps, self.step_size, self.n_spin_flips)
+            ap, _ = self._gs.run(key=key)
+        else:
+            ap, _ = self.chain.run(self.n_sweeps, self.step_size, self.n_spin_flips, key=key)
         if t: t.append(self._ev())
         e = self.energies()
         if t: t.append(self._ev())
@@ -218,9 +226,14 @@ class VMCStep:
             self.failed_steps = getattr(self, "failed_steps", 0) + 1
             d = torch.zeros_like(d)
         self.theta -= self.lr * d
-        self.lp = LogPsi(self.model, self.theta)
-        self.chain.logpsi = self.lp
-        self.chain.logp = (2.0 * self.lp(self.chain.R, self.chain.S).real).contiguous()
+        if self.graph:
+            # same buffers: the captured sweeps keep reading them
+            self.lp.update(self.theta)
+            self.chain.logp.copy_(2.0 * self.lp(self.chain.R, self.chain.S).real)
+        else:
+            self.lp = LogPsi(self.model, self.theta)
+            self.chain.logpsi = self.lp
+            self.chain.logp = (2.0 * self.lp(self.chain.R, self.chain.S).real).contiguous()
         if t:
             t.append(self._ev())
             self.timers.append(t)

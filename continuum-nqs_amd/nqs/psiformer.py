@@ -123,14 +123,20 @@ class ContinuumSpinorPsiformer:
             self._dev = (dev, gs, w)
         return self._dev
 
-    def pack(self, theta):
-        """theta (P,) float64 on the GPU -> packed kernel layout (uint8 device buffer)."""
+    def pack(self, theta, out=None):
+        """theta (P,) float64 on the GPU -> packed kernel layout (uint8 device buffer; written into
+        `out` when given, which must have the packed size)."""
         lib = _lib.load()
         dev, gs, w = self._device_consts()
         theta = theta.to(device=dev, dtype=torch.float64).contiguous()
         d = self.desc()
         nb = lib.nqs_packed_bytes(ctypes.byref(d))
-        packed = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+        if out is not None:
+            if out.numel() != nb or out.dtype != torch.uint8:
+                raise ValueError("packed buffer of the wrong size")
+            packed = out
+        else:
+            packed = torch.empty(int(nb), dtype=torch.uint8, device=dev)
         _lib.check(lib.nqs_pack_params(ctypes.byref(d), _lib.ptr(theta), _lib.ptr(gs), _lib.ptr(w),
                                        _lib.ptr(packed), _lib.stream()), "nqs_pack_params")
         return packed
@@ -180,6 +186,11 @@ class LogPsi:
         self.model = model
         self.theta = theta.to(device=_lib.device(), dtype=torch.float64).contiguous()
         self.packed = model.pack(self.theta)
+
+    def update(self, theta):
+        """New parameters in the same device buffers (keeps addresses bound by a captured graph)."""
+        self.theta.copy_(theta)
+        self.model.pack(self.theta, out=self.packed)
 
     def __call__(self, R, S):
         lib = _lib.load()
