@@ -12,6 +12,7 @@
 #include <algorithm>
 #include "nqs_common.h"
 #include "kernels.h"
+#include "tune.h"
 
 namespace nqs {
 
@@ -479,11 +480,11 @@ __device__ __forceinline__ void attn_store(T* __restrict__ o, int lane, int N, i
 }
 
 template <typename T, int NQ>
-__global__ __launch_bounds__(256) void k_attention_jet_mfma(JetGeom j, int d, int nh, const T* __restrict__ qkv,
+__global__ __launch_bounds__(512) void k_attention_jet_mfma(JetGeom j, int d, int nh, const T* __restrict__ qkv,
                                                             T* __restrict__ out, int64_t npairs) {
   typedef typename Mma16<T>::acc_t acc_t;
   const int lane = threadIdx.x & 63;
-  const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t pair = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (pair >= npairs) return;
   const int N = j.N, hd = d / nh;
   const int64_t cfg = pair / nh;
@@ -620,13 +621,20 @@ int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* 
   if (npairs == 0) return NQS_OK;
   const int N = j.N, hd = d / n_heads;
   if (N <= 16 && hd <= 32) {
-    const dim3 grid((unsigned)cdiv(npairs, 4)), block(256);
-    if (hd <= 16)
-      hipLaunchKernelGGL((k_attention_jet_mfma<T, 1>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
-    else
-      hipLaunchKernelGGL((k_attention_jet_mfma<T, 2>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
-    NQS_LAUNCH_CHECK();
-    return NQS_OK;
+    // launch shape (waves per workgroup) chosen by the autotuner per shape class
+    static const char* const names[4] = {"1wave", "2waves", "4waves", "8waves"};
+    const TuneKey key{TUNE_ATTENTION, (int64_t)sizeof(T) * 1000 + N * 64 + hd, (int64_t)j.nt * 2 + j.has_lap,
+                      tune_bucket(npairs)};
+    return tune_run(key, 4, 2, names, [&](int c) -> int {
+      const int wpg = 1 << c;
+      const dim3 grid((unsigned)cdiv(npairs, wpg)), block(64 * wpg);
+      if (hd <= 16)
+        hipLaunchKernelGGL((k_attention_jet_mfma<T, 1>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
+      else
+        hipLaunchKernelGGL((k_attention_jet_mfma<T, 2>), grid, block, 0, st, j, d, n_heads, qkv, out, npairs);
+      NQS_LAUNCH_CHECK();
+      return NQS_OK;
+    }, st);
   }
   const size_t per_wave = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
   NQS_REQUIRE(per_wave <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");

@@ -13,6 +13,7 @@
 #include "nqs_common.h"
 #include "kernels.h"
 #include "bwd.h"
+#include "tune.h"
 
 namespace nqs {
 
@@ -570,7 +571,7 @@ __device__ __forceinline__ cd bf_at(const T* __restrict__ bfc, int64_t rowlen, i
 }
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
+__global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
                                                    const int32_t* __restrict__ S, int64_t w0,
                                                    const T* __restrict__ bf, double* __restrict__ out_lp,
                                                    double* __restrict__ out_tan, double* __restrict__ out_lap) {
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, co
   __shared__ int spin[16];
   extern __shared__ cd res[];  // [K][nt + 2]: logdet, tangents, Laplacian
   const int N = j.N, K = p.K, nt = j.nt;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int64_t cfg = blockIdx.x;
   int64_t w;
   int flip = -1;
@@ -596,8 +597,8 @@ __global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, co
     spin[tid] = (tid == flip) ? 1 - s : s;
   }
   __syncthreads();
-  {  // envelope jet, entry (a, i) = (tid / 16, tid % 16)
-    const int a = tid >> 4, i = tid & 15;
+  for (int e = tid; e < 256; e += blockDim.x) {  // envelope jet, entry (a, i) = (e / 16, e % 16)
+    const int a = e >> 4, i = e & 15;
     cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0), lp = mk(0, 0);
     if (a < N && i < N) {
       for (int gg = 0; gg < p.n_g; ++gg) {
@@ -612,10 +613,10 @@ __global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, co
         lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
       }
     }
-    env[tid] = v;
-    dex[tid] = gx;
-    dey[tid] = gy;
-    lenv[tid] = lp;
+    env[e] = v;
+    dex[e] = gx;
+    dey[e] = gy;
+    lenv[e] = lp;
   }
   __syncthreads();
   const int C = j.C;
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(256, 2) void k_det_jet16(JetGeom j, DetParams p, co
   const T* bfc0 = bf + cfg * C * chs;
   const int l16 = lane & 15, g = lane >> 4;
   const int rs_ = nt + 2;
-  for (int k = wave; k < K; k += 4) {
+  for (int k = wave; k < K; k += nwaves) {
     Lu16 f;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -803,10 +804,16 @@ int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const 
   if (N <= 16 && j.nt <= 64) {
     const size_t lds = sizeof(cd) * (size_t)p.K * (j.nt + 2);
     NQS_REQUIRE(lds <= 96 * 1024, NQS_ERR_UNSUPPORTED, "determinant head exceeds LDS");
-    hipLaunchKernelGGL(k_det_jet16<T>, dim3((unsigned)j.n_cfg), dim3(256), lds, st, j, p, R, S, w0, bf, out_lp,
-                       out_tan, out_lap);
-    NQS_LAUNCH_CHECK();
-    return NQS_OK;
+    // workgroup size (determinants in flight per configuration) chosen by the autotuner
+    static const char* const names[3] = {"2waves", "4waves", "8waves"};
+    const TuneKey key{TUNE_DET16, (int64_t)sizeof(T) * 1000 + N * 64 + p.K, (int64_t)j.nt * 4 + j.has_lap * 2 +
+                      (j.mode == 2), tune_bucket(j.n_cfg)};
+    return tune_run(key, 3, 1, names, [&](int c) -> int {
+      hipLaunchKernelGGL(k_det_jet16<T>, dim3((unsigned)j.n_cfg), dim3(128 << c), lds, st, j, p, R, S, w0, bf,
+                         out_lp, out_tan, out_lap);
+      NQS_LAUNCH_CHECK();
+      return NQS_OK;
+    }, st);
   }
   NQS_REQUIRE(NN <= 4 * 256, NQS_ERR_UNSUPPORTED, "determinant head supports N <= 32");
   size_t lds = sizeof(cd) * (size_t)(7 * NN + (size_t)p.K * (j.nt + 2)) + sizeof(double) * 2 * N +

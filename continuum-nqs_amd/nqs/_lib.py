@@ -63,6 +63,9 @@ SIGNATURES = [
     ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),
     ("nqs_minsr_profile", c_i32, [c_i32, ctypes.POINTER(ctypes.c_float)]),
+    ("nqs_autotune_enable", c_i32, [c_i32]),
+    ("nqs_autotune_reset", c_i32, []),
+    ("nqs_autotune_report", c_i64, [ctypes.c_char_p, c_i64]),
     ("nqs_ewald", c_i32, [ctypes.POINTER(EwaldDesc), c_vp, c_i64, c_i32, c_vp, c_vp]),
     ("nqs_mcmc_sweeps", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_dbl,
                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
@@ -84,6 +87,15 @@ def load():
             f.argtypes = args
         _lib = lib
     return _lib
+
+
+def autotune_report():
+    """The autotuner's choices so far, one line per tuned shape class."""
+    lib = load()
+    n = lib.nqs_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    lib.nqs_autotune_report(buf, n + 1)
+    return buf.value.decode()
 
 
 def check(rc, what):

@@ -181,6 +181,17 @@ int nqs_minsr_direction(const nqs_minsr_desc* d, void* O_dev, const double* eps_
  * switches hipEvent recording on (1) or off (0) for later calls; ms_out (5 floats, host) receives
  * the last profiled call's centre, Gram, Cholesky, solve+refinement, push-through times. */
 int nqs_minsr_profile(int32_t enable, float* ms_out);
+/* Launch-shape autotuner (no reference counterpart; the reference's XLA compiler has no such
+ * knob): the first call of a tuned kernel (attention jet: waves per workgroup; determinant head:
+ * workgroup size) for a shape class times each candidate once on the caller's stream and keeps the
+ * fastest for the rest of the process.  Tuning synchronises the stream once per shape class and
+ * is skipped while the stream is captured into a graph.  enable = 0 uses the default shapes;
+ * reset forgets the choices; report writes one line per tuned shape class into buf (len bytes,
+ * NUL-terminated) and returns the full length. */
+int nqs_autotune_enable(int32_t enable);
+int nqs_autotune_reset(void);
+int64_t nqs_autotune_report(char* buf, int64_t len);
+
 /* Median/MAD clip of Re e (minsr.py:145-148), W <= 16384: e_clip_dev (W) complex (Im kept);
  * stats_dev[4] = mean Re e (unclipped), std(Re e)/sqrt(W), median, MAD. */
 int nqs_clip_energies(const double* e_dev, int64_t n_walkers, double clip_k, double* e_clip_dev,
