@@ -111,6 +111,10 @@ static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void*
   a.h_z = h->h_z;
   a.n_field = h->n_field;
   a.field = h->field_dev;
+  a.field_scale = h->field_scale;
+  a.n_pin = h->n_pin;
+  a.pin = h->pin_dev;
+  for (int q = 0; q < 4; ++q) a.A[q] = h->cell[q];
   a.R = R;
   a.S = S;
   a.lp = lp;
@@ -307,6 +311,15 @@ using namespace nqs;
 extern "C" {
 
 int nqs_version(void) { return 1; }
+int64_t nqs_desc_size(int32_t which) {
+  switch (which) {
+    case 0: return (int64_t)sizeof(nqs_model_desc);
+    case 1: return (int64_t)sizeof(nqs_hamiltonian_desc);
+    case 2: return (int64_t)sizeof(nqs_ewald_desc);
+    case 3: return (int64_t)sizeof(nqs_minsr_desc);
+    default: return -1;
+  }
+}
 const char* nqs_last_error(void) { return g_last_error.c_str(); }
 
 int64_t nqs_param_count(const nqs_model_desc* m) {
@@ -393,6 +406,8 @@ int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void*
   NQS_TRY(build_model(m, md));
   NQS_REQUIRE(h && packed && R && S && eloc && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
   NQS_REQUIRE(h->n_field == 0 || h->field_dev, NQS_ERR_ARG, "ext field terms without data");
+  NQS_REQUIRE(h->n_pin >= 0 && (h->n_pin == 0 || h->pin_dev), NQS_ERR_ARG, "pinning centres without data");
+  NQS_REQUIRE(h->field_scale >= 0.0 && h->field_scale < HUGE_VAL, NQS_ERR_ARG, "field_scale must be finite and >= 0");
   NQS_TRY(check_ws(m, NQS_OP_ELOC, W, ws_bytes));
   if (md.tsize == 8)
     return eloc_impl<double>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);

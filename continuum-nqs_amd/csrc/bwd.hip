@@ -19,46 +19,55 @@ namespace nqs {
 // Output column j belongs to segment s = j / seg_w with its own theta base; inside a segment the
 // leaf is (in, seg_w) row-major.
 template <typename T>
-__global__ __launch_bounds__(256) void k_wgrad(WgradArgs<T> a) {
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs<T> a, int ob) {
   extern __shared__ char smem_raw[];
   T* Xs = reinterpret_cast<T*>(smem_raw);  // N x IB
-  T* Gs = Xs + a.N * 16;                   // N x out
+  T* Gs = Xs + a.N * 16;                   // N x nb (output columns j0 .. j0 + nb)
   const int IB = 16;
   const int64_t cfg = blockIdx.y >> 1;
   const int c = blockIdx.y & 1;
   const int i0 = blockIdx.x * IB;
+  const int j0 = blockIdx.z * ob;
+  const int nb = min(ob, a.out - j0);
   const int tid = threadIdx.x;
   const int N = a.N;
   const T* X = a.X + cfg * (int64_t)N * a.ldx;
-  const T* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg;
+  const T* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg + j0;
   for (int e = tid; e < N * IB; e += blockDim.x) {
     const int t = e / IB, ii = e % IB;
     Xs[e] = (i0 + ii < a.in) ? X[(int64_t)t * a.ldx + i0 + ii] : (T)0;
   }
-  for (int e = tid; e < N * a.out; e += blockDim.x) {
-    const int t = e / a.out, j = e % a.out;
+  for (int e = tid; e < N * nb; e += blockDim.x) {
+    const int t = e / nb, j = e % nb;
     Gs[e] = G[(int64_t)t * a.ldg + j];
   }
   __syncthreads();
   const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
   T* O = a.O + row * a.ldo;
   const int nrow = min(IB, a.in - i0);
-  for (int e = tid; e < nrow * a.out; e += blockDim.x) {
-    const int ii = e / a.out, j = e % a.out;
+  for (int e = tid; e < nrow * nb; e += blockDim.x) {
+    const int ii = e / nb, j = e % nb;
     T s = 0;
-    for (int t = 0; t < N; ++t) s += Xs[t * IB + ii] * Gs[t * a.out + j];
-    const int sg = j / a.seg_w, jj = j % a.seg_w;
+    for (int t = 0; t < N; ++t) s += Xs[t * IB + ii] * Gs[t * nb + j];
+    const int sg = (j0 + j) / a.seg_w, jj = (j0 + j) % a.seg_w;
     O[a.seg_base[sg] + (int64_t)(i0 + ii) * a.seg_w + jj] = s;
   }
 }
 
+// Output columns go in chunks of ob: all of them when the N x out cotangent tile takes at most
+// 96 KB of LDS, else the largest multiple of 64 that keeps the tile within 64 KB.
 template <typename T>
 int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
   if (a.n_cfg == 0) return NQS_OK;
-  const size_t lds = sizeof(T) * (size_t)a.N * (16 + a.out);
-  NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "weight-gradient tile exceeds LDS");
+  int ob = a.out;
+  if (sizeof(T) * (size_t)a.N * (16 + ob) > 96 * 1024) {
+    ob = ((int)(64 * 1024 / (sizeof(T) * (size_t)a.N)) - 16) / 64 * 64;
+    NQS_REQUIRE(ob >= 64, NQS_ERR_UNSUPPORTED, "weight-gradient tile exceeds LDS");
+  }
+  const size_t lds = sizeof(T) * (size_t)a.N * (16 + ob);
   NQS_REQUIRE(a.out % a.seg_w == 0 && a.out / a.seg_w <= 3, NQS_ERR_ARG, "bad output segmentation");
-  hipLaunchKernelGGL(k_wgrad<T>, dim3((unsigned)cdiv(a.in, 16), (unsigned)(a.n_cfg * 2)), dim3(256), lds, st, a);
+  hipLaunchKernelGGL(k_wgrad<T>, dim3((unsigned)cdiv(a.in, 16), (unsigned)(a.n_cfg * 2), (unsigned)cdiv(a.out, ob)),
+                     dim3(256), lds, st, a, ob);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

@@ -10,6 +10,10 @@ struct EAssembleArgs {
   double m_star, lambda_r, h_z;
   int n_field;
   const double* field;  // n_field x 8
+  double field_scale;   // 0: raw Fourier sum; > 0: normalised texture times field_scale
+  int n_pin;
+  const double* pin;    // n_pin x 8: cx, cy, width, v0, hx, hy, hz, 0
+  double A[4];          // cell (row-major), for the pinning images
   const double* R;      // (W, N, 2)
   const int32_t* S;     // (W, N)
   const double* lp;     // (W) complex: log psi

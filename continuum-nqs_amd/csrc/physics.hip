@@ -94,13 +94,42 @@ int launch_ewald(const nqs_ewald_desc& e, const double* R, int64_t W, int N, dou
 }
 
 // ------------------------------------------------------------------ local energy assembly (local_energy.py:63-115)
-__device__ inline void ext_field_at(const double* fld, int nf, double x, double y, double f[3]) {
+// m(r) = sum_t c_t cos(q_t.r) + s_t sin(q_t.r); f = m, or scale m / (|m| + 1e-12) for a
+// normalised texture (pinning.py:29-37, 96-100)
+__device__ inline void ext_field_at(const double* fld, int nf, double scale, double x, double y, double f[3]) {
   f[0] = f[1] = f[2] = 0;
   for (int t = 0; t < nf; ++t) {
     const double* q = fld + 8 * t;
     double sn, cs;
     sincos(q[0] * x + q[1] * y, &sn, &cs);
     for (int a = 0; a < 3; ++a) f[a] += q[2 + a] * cs + q[5 + a] * sn;
+  }
+  if (scale > 0) {
+    const double k = scale / (sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + 1e-12);
+    for (int a = 0; a < 3; ++a) f[a] *= k;
+  }
+}
+
+// Gaussian pinning centres at (x, y): r - c reduced to its minimum image (fractional
+// s = A^-1 (r - c), s -= rint(s)), then the 3 x 3 images around it, so the profile g is periodic
+// for any positions; adds v0 g to *v and h g to f.
+__device__ inline void pins_at(const double* pin, int np, const double A[4], double x, double y, double* v,
+                               double f[3]) {
+  const double idet = 1.0 / (A[0] * A[3] - A[1] * A[2]);
+  for (int p = 0; p < np; ++p) {
+    const double* c = pin + 8 * p;
+    const double inv2w2 = 1.0 / (2.0 * c[2] * c[2]);
+    const double ux = x - c[0], uy = y - c[1];
+    const double sx = rint((A[3] * ux - A[1] * uy) * idet), sy = rint((A[0] * uy - A[2] * ux) * idet);
+    const double rx = ux - (A[0] * sx + A[1] * sy), ry = uy - (A[2] * sx + A[3] * sy);
+    double g = 0;
+    for (int nx = -1; nx <= 1; ++nx)
+      for (int ny = -1; ny <= 1; ++ny) {
+        const double dx = rx - (A[0] * nx + A[1] * ny), dy = ry - (A[2] * nx + A[3] * ny);
+        g += exp(-(dx * dx + dy * dy) * inv2w2);
+      }
+    *v += c[3] * g;
+    for (int a = 0; a < 3; ++a) f[a] += c[4 + a] * g;
   }
 }
 
@@ -117,39 +146,38 @@ __global__ void k_eloc_assemble(EAssembleArgs a) {
   }
   const cd lap = mk(a.lap[2 * w], a.lap[2 * w + 1]);
   const cd ekin = (-1.0 / (2.0 * a.m_star)) * (lap + g2);
-  // Zeeman + ext z
-  double ez = 0, sgs = 0;
+  // Zeeman
+  double sgs = 0;
   for (int i = 0; i < N; ++i) sgs += 1.0 - 2.0 * (double)a.S[w * N + i];
-  ez = -0.5 * a.h_z * sgs;
-  if (a.n_field > 0) {
-    for (int i = 0; i < N; ++i) {
-      double f[3];
-      ext_field_at(a.field, a.n_field, a.R[(w * N + i) * 2], a.R[(w * N + i) * 2 + 1], f);
-      ez -= f[2] * (1.0 - 2.0 * (double)a.S[w * N + i]);
-    }
-  }
-  // SOC from the flipped configurations (cfg = w N + i)
+  double ez = -0.5 * a.h_z * sgs;
+  // per electron: the field f(r_i) (ext field + pinning spin fields) and the pinning potential;
+  // f_z enters the diagonal, f_x, f_y the flipped configurations (cfg = w N + i) with the SOC
+  const bool fields = a.n_field > 0 || a.n_pin > 0;
+  double epin = 0;
   cd esoc = mk(0, 0);
   for (int i = 0; i < N; ++i) {
     const int64_t cfg = w * N + i;
+    const int s = a.S[w * N + i];
+    double f[3] = {0, 0, 0};
+    if (fields) {
+      const double x = a.R[(w * N + i) * 2], y = a.R[(w * N + i) * 2 + 1];
+      if (a.n_field > 0) ext_field_at(a.field, a.n_field, a.field_scale, x, y, f);
+      if (a.n_pin > 0) pins_at(a.pin, a.n_pin, a.A, x, y, &epin, f);
+      ez -= f[2] * (1.0 - 2.0 * (double)s);
+    }
     const cd Lf = mk(a.lpf[2 * cfg], a.lpf[2 * cfg + 1]);
     const cd ratio = cexp_(Lf - L0);
     cd ax = ratio * mk(a.tanf[(cfg * 2 + 0) * 2], a.tanf[(cfg * 2 + 0) * 2 + 1]);
     cd ay = ratio * mk(a.tanf[(cfg * 2 + 1) * 2], a.tanf[(cfg * 2 + 1) * 2 + 1]);
     if (!(isfinite(ax.re) && isfinite(ax.im))) ax = mk(0, 0);
     if (!(isfinite(ay.re) && isfinite(ay.im))) ay = mk(0, 0);
-    const int s = a.S[w * N + i];
     const cd sy = (s == 0) ? mk(0, -1) : mk(0, 1);   // (sigma_y)[s, 1-s]
     const cd mi = mk(0, -1);
     cd e = a.lambda_r * (mi * ay + sy * (mi * ax));
-    if (a.n_field > 0) {
-      double f[3];
-      ext_field_at(a.field, a.n_field, a.R[(w * N + i) * 2], a.R[(w * N + i) * 2 + 1], f);
-      e = e - (mk(f[0], 0) + f[1] * sy) * ratio;
-    }
+    if (a.n_field > 0 || f[0] != 0 || f[1] != 0) e = e - (mk(f[0], 0) + f[1] * sy) * ratio;
     esoc = esoc + e;
   }
-  const cd tot = ekin + mk(ez, 0) + esoc;
+  const cd tot = ekin + mk(ez + epin, 0) + esoc;
   a.out[2 * w] = tot.re;
   a.out[2 * w + 1] = tot.im;
 }

@@ -30,7 +30,8 @@ class ModelDesc(ctypes.Structure):
 
 class HamDesc(ctypes.Structure):
     _fields_ = [("m_star", c_dbl), ("lambda_r", c_dbl), ("h_z", c_dbl), ("n_field", c_i32),
-                ("reserved", c_i32), ("field_dev", c_vp)]
+                ("n_pin", c_i32), ("field_dev", c_vp), ("pin_dev", c_vp), ("cell", c_dbl * 4),
+                ("field_scale", c_dbl)]
 
 
 class EwaldDesc(ctypes.Structure):
@@ -46,6 +47,7 @@ class MinsrDesc(ctypes.Structure):
 # (name, restype, argtypes) of every exported symbol; the CPU tests check each one resolves.
 SIGNATURES = [
     ("nqs_version", c_i32, []),
+    ("nqs_desc_size", c_i64, [c_i32]),
     ("nqs_last_error", ctypes.c_char_p, []),
     ("nqs_param_count", c_i64, [ctypes.POINTER(ModelDesc)]),
     ("nqs_param_leaf", c_i32, [ctypes.POINTER(ModelDesc), c_i32, ctypes.c_char_p, c_i32,

@@ -177,12 +177,12 @@ class VMCStep:
 
     def __init__(self, model, theta, R, S, m_star=1.0, lambda_r=1.0, h_z=0.0, ext_field=None, coulomb=None,
                  lr=0.05, lam=1e-3, n_sweeps=10, step_size=0.4, clip_k=5.0, n_spin_flips=2, gram="auto",
-                 refine=1, graph=False):
+                 refine=1, graph=False, pinning=None):
         self.model = model
         self.theta = theta.to(_lib.device(), torch.float64).contiguous()
         self.lp = LogPsi(model, self.theta)
         self.chain = Chain(self.lp, R, S)
-        self.ham = dict(m_star=m_star, lambda_r=lambda_r, h_z=h_z, ext_field=ext_field)
+        self.ham = dict(m_star=m_star, lambda_r=lambda_r, h_z=h_z, ext_field=ext_field, pinning=pinning)
         self.coulomb = coulomb
         self.lr, self.lam, self.n_sweeps, self.step_size, self.clip_k = lr, lam, n_sweeps, step_size, clip_k
         self.n_spin_flips, self.gram, self.refine = n_spin_flips, gram, refine
@@ -243,7 +243,8 @@ class VMCStep:
 
 def train_minsr(model, params, L, n_elec, key, *, m_star=1.0, lambda_r=1.0, h_z=0.0, ext_field=None, coulomb=None,
                 use_folx=False, n_walkers=256, n_steps=100, lr=0.05, lam=1e-3, n_sweeps=10, step_size=0.4, burn=80,
-                clip_k=5.0, log_every=20, history=None, R0=None, S0=None, return_walkers=False, gram="auto"):
+                clip_k=5.0, log_every=20, history=None, R0=None, S0=None, return_walkers=False, gram="auto",
+                pinning=None):
     """Optimise the Psiformer with MinSR (minsr.py:92-159).  Returns (params, history) or
     (params, history, R, S)."""
     dev = _lib.device()
@@ -258,7 +259,8 @@ def train_minsr(model, params, L, n_elec, key, *, m_star=1.0, lambda_r=1.0, h_z=
         R = torch.as_tensor(np.asarray(R0) if not torch.is_tensor(R0) else R0).to(dev, torch.float64)
         S = torch.as_tensor(np.asarray(S0) if not torch.is_tensor(S0) else S0).to(dev, torch.int32)
     st = VMCStep(model, theta, R, S, m_star=m_star, lambda_r=lambda_r, h_z=h_z, ext_field=ext_field,
-                 coulomb=coulomb, lr=lr, lam=lam, n_sweeps=n_sweeps, step_size=step_size, clip_k=clip_k, gram=gram)
+                 coulomb=coulomb, lr=lr, lam=lam, n_sweeps=n_sweeps, step_size=step_size, clip_k=clip_k, gram=gram,
+                 pinning=pinning)
     seed = int(torch.randint(0, 2 ** 30, (1,), generator=g, device=dev).item())
     if R0 is None and burn > 0:
         st.chain.run(burn, step_size, 2, key=seed)

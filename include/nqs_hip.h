@@ -75,16 +75,26 @@ typedef struct nqs_model_desc {
 } nqs_model_desc;
 
 /* Single-particle Hamiltonian of make_local_energy (src/nqs/local_energy.py:40-44).
- * The external field is a Fourier series: f(r) = sum_t c_t cos(q_t.r) + s_t sin(q_t.r),
- * term t = 8 doubles (qx, qy, cx, cy, cz, sx, sy, sz) in device memory; a constant field is a
- * term with q = 0.  n_field = 0 means ext_field = None. */
+ * The external field ext_field(r), coupled as -sum_i f(r_i).sigma_i (local_energy.py:84-111), is a
+ * Fourier series m(r) = sum_t c_t cos(q_t.r) + s_t sin(q_t.r), term t = 8 doubles
+ * (qx, qy, cx, cy, cz, sx, sy, sz) in device memory (a constant field is a term with q = 0);
+ * f = m when field_scale = 0, and f = field_scale m / (|m| + 1e-12) when field_scale > 0 (the
+ * normalised textures of src/nqs/pinning.py:20-100).  n_field = 0 means ext_field = None. */
 typedef struct nqs_hamiltonian_desc {
   double m_star;
   double lambda_r;
   double h_z;
   int32_t n_field;
-  int32_t reserved;
+  int32_t n_pin;          /* Gaussian pinning centres (0 = none) */
   const double* field_dev;
+  /* Gaussian pinning centres (no reference counterpart): centre p has the periodic profile
+   * g_p(r) = sum_{n in {-1,0,1}^2} exp(-|u_p - A n|^2 / (2 width_p^2)), u_p = the minimum image
+   * of r - c_p (fractional coordinates rounded to the nearest integer).  It adds the scalar
+   * potential v0_p g_p(r_i) and the spin field h_p g_p(r_i) (coupled like ext_field) for every
+   * electron.  pin_dev (n_pin, 8) = (cx, cy, width, v0, hx, hy, hz, 0) in device memory. */
+  const double* pin_dev;
+  double cell[4];         /* A row-major, for the pinning images (ignored when n_pin = 0) */
+  double field_scale;     /* 0: f = m; > 0: f = field_scale m / (|m| + 1e-12) */
 } nqs_hamiltonian_desc;
 
 /* Precomputed Ewald tables of make_ewald_lattice (src/nqs/ewald.py:26-58). */
@@ -102,6 +112,10 @@ typedef struct nqs_ewald_desc {
 
 int nqs_version(void);
 const char* nqs_last_error(void);
+/* sizeof of the descriptor structs, for bindings to check their layout: which = 0 model,
+ * 1 hamiltonian, 2 ewald, 3 minsr; -1 for another value.  (No reference counterpart: the
+ * reference passes Python objects, not structs.) */
+int64_t nqs_desc_size(int32_t which);
 
 /* ---- parameters: flat float64 vector theta in real_ravel order (minsr.py:30-53) */
 int64_t nqs_param_count(const nqs_model_desc* m);

@@ -30,6 +30,13 @@ def test_library_exports_every_declared_symbol():
     assert lib.nqs_version() >= 1
 
 
+def test_descriptor_layouts_match_the_bindings():
+    lib = _lib.load()
+    for which, cls in enumerate([_lib.ModelDesc, _lib.HamDesc, _lib.EwaldDesc, _lib.MinsrDesc]):
+        assert lib.nqs_desc_size(which) == ctypes.sizeof(cls), cls.__name__
+    assert lib.nqs_desc_size(4) == -1
+
+
 ARCHS = [dict(embed_dim=16, depth=2, n_heads=4, n_freq=4),
          dict(embed_dim=16, depth=2, n_heads=2, n_freq=3, n_dets=3, use_jastrow=True, learnable_env=True, n_knots=6),
          dict(embed_dim=8, depth=12, n_heads=2, n_freq=2, n_dets=2, use_jastrow=True, jastrow_kind="cusp"),

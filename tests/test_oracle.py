@@ -265,3 +265,105 @@ def test_periodic_cusp_jastrow_is_smooth_periodic_and_cusped():
     jump_p = abs(gx(a, L / 2 + 1e-4) - gx(a, L / 2 - 1e-4))
     jump_m = abs(gx(am, L / 2 + 1e-4) - gx(am, L / 2 - 1e-4))
     assert jump_p < 1e-3 and jump_m > 1e-2
+
+
+def test_pinning_potential_closed_forms():
+    """Gaussian pinning centres (no reference counterpart): periodicity, continuity across the
+    cell edge, and the image sum against its Poisson-summed Fourier form
+    sum_n exp(-|u - A n|^2 / 2w^2) = (2 pi w^2 / area) sum_G exp(-w^2 |G|^2 / 2) cos(G.u)."""
+    L, w = 10.0, 1.5
+    A = np.array([[L, 2.0], [0.0, 0.9 * L]])
+    area = abs(np.linalg.det(A))
+    B = 2 * np.pi * np.linalg.inv(A).T
+    pin = [(2.0, 3.0, w, 0.05, 0.0, 0.0, 0.0), (6.0, 1.0, w, 0.0, 0.01, -0.02, 0.03)]
+    rng = np.random.default_rng(0)
+    r = rng.uniform(0, 1, (7, 2)) @ A.T
+    g = ole.pin_profiles(r, pin, A)
+    m = np.array([(i, j) for i in range(-9, 10) for j in range(-9, 10)], float)
+    G = m @ B.T
+    for p, (cx, cy, *_) in enumerate(pin):
+        u = r - np.array([cx, cy])
+        four = (2 * np.pi * w * w / area) * (np.exp(-0.5 * w * w * (G * G).sum(1))[None] * np.cos(u @ G.T)).sum(1)
+        assert np.max(np.abs(g[p] - four)) < 1e-13
+    # potential and field are the profiles weighted by v0 and h
+    assert abs(ole.pinning_potential(r, pin, A) - 0.05 * g[0].sum()) < 1e-15
+    assert np.max(np.abs(ole.pinning_field(r, pin, A) - g[1][:, None] * np.array([0.01, -0.02, 0.03]))) < 1e-17
+    # periodic under any lattice shift, continuous across the cell edge
+    shift = A @ np.array([1.0, -2.0])
+    assert np.max(np.abs(ole.pin_profiles(r + shift, pin, A) - g)) < 1e-12
+    for y in (0.5, 3.0, 8.7):
+        a_ = ole.pin_profiles(np.array([[1e-9 + 2.0 * y / (0.9 * L), y]]), pin, A)
+        b_ = ole.pin_profiles(np.array([[1e-9 + 2.0 * y / (0.9 * L) + L, y]]), pin, A)
+        assert np.max(np.abs(a_ - b_)) < 1e-12
+
+
+def _hex_cell(a=7.0):
+    A = np.array([[a, a / 2], [0.0, a * np.sqrt(3) / 2]])
+    B = 2 * np.pi * np.linalg.inv(A).T
+    return A, B[:, 0], B[:, 1]
+
+
+def test_oracle_skyrmion_textures():
+    """pinning.py:20-37 restated: the triple-Q texture on its own (hexagonal) magnetic cell has
+    skyrmion number 1 for |m0| small (Neel and Bloch, either chirality) and 0 when m0 > 3 keeps
+    m_z > 0; near r = 0 the in-plane part is radial (Neel) or azimuthal (Bloch), which needs the
+    three arms at 120 degrees; |f| = h_pin."""
+    from oracle import pinning as opn
+    A, q1, q2 = _hex_cell()
+    for neel in (True, False):
+        for ch in (1.0, -1.0):
+            f = opn.make_skx_pin(q1, q2, 0.1, m0=0.3, chirality=ch, neel=neel)
+            assert abs(opn.skyrmion_charge(f, A) - 1.0) < 1e-9
+            assert abs(opn.skyrmion_charge(opn.make_skx_pin(q1, q2, 0.1, m0=3.5, chirality=ch, neel=neel), A)) < 1e-9
+            r = torch.tensor([[1e-4, 3e-5], [-2e-5, 7e-5]])
+            v = f(r)
+            cross = r[:, 0] * v[:, 1] - r[:, 1] * v[:, 0]          # (r x f)_z
+            dot = v[:, 0] * r[:, 0] + v[:, 1] * r[:, 1]
+            inplane = torch.linalg.norm(v[:, :2], dim=1) * torch.linalg.norm(r, dim=1)
+            if neel:
+                assert float((cross.abs() / inplane).max()) < 1e-6 and float((ch * dot).min()) > 0
+            else:
+                assert float((dot.abs() / inplane).max()) < 1e-6 and float((ch * cross).min()) > 0
+            pts = torch.tensor(np.random.default_rng(1).uniform(0, 7, (50, 2)))
+            assert float((torch.linalg.norm(f(pts), dim=1) - 0.1).abs().max()) < 1e-10
+    # a planted sin <-> cos swap flips the skyrmion number
+    qs = torch.tensor(np.stack([q1, q2, -(q1 + q2)]))
+    qh = qs / torch.linalg.norm(qs, dim=1, keepdim=True)
+
+    def planted(r):
+        ph = r @ qs.T
+        m = torch.stack([(torch.cos(ph) * qh[:, 0]).sum(-1), (torch.cos(ph) * qh[:, 1]).sum(-1),
+                         0.3 + torch.sin(ph).sum(-1)], -1)
+        return m / torch.linalg.norm(m, dim=-1, keepdim=True)
+    assert abs(opn.skyrmion_charge(planted, A) + 1.0) < 1e-9
+
+
+def test_oracle_simple_pins_and_hf_texture():
+    """pinning.py:40-119 restated: closed forms of the ferromagnet, spiral, flat helix and
+    Overhauser targets; a converged-HF texture whose harmonics are the skyrmion's own
+    (S_0 = (0, 0, m0), S_qj = (-i chi e_j, 1)) reproduces make_skx_pin; make_pin dispatch."""
+    from oracle import pinning as opn
+    A, q1, q2 = _hex_cell()
+    Q = np.array([0.4, -0.3])
+    qh = Q / np.linalg.norm(Q)
+    r = torch.tensor(np.random.default_rng(2).uniform(0, 7, (20, 2)))
+    ph = (r @ torch.tensor(Q)).numpy()
+    fm = opn.make_fm_pin(0.2, axis=(1.0, 2.0, 2.0))(r).numpy()
+    assert np.max(np.abs(fm - 0.2 * np.array([1, 2, 2]) / 3.0)) < 1e-15
+    sp = opn.make_spiral_pin(Q, 0.2)(r).numpy()
+    # (q_hat carries the reference's +1e-12 in its norm)
+    assert np.max(np.abs(sp - 0.2 * np.stack([np.sin(ph) * qh[0], np.sin(ph) * qh[1], np.cos(ph)], 1))) < 1e-12
+    xy = opn.make_pin("spiral_xy", Q, None, 0.2)(r).numpy()
+    assert np.max(np.abs(xy - 0.2 * np.stack([np.cos(ph), np.sin(ph), 0 * ph], 1))) < 1e-15
+    oh = opn.make_pin("overhauser", Q, None, 0.2)(r).numpy()
+    assert np.max(np.abs(oh - 0.2 * np.stack([0 * ph, 0 * ph, np.cos(ph)], 1))) < 1e-15
+    assert opn.make_pin("fluid", Q, None, 0.2) is None
+    for neel, ch in ((True, 1.0), (False, -1.0)):
+        qs = np.stack([q1, q2, -(q1 + q2)])
+        qhat = qs / np.linalg.norm(qs, axis=1, keepdims=True)
+        e = qhat if neel else np.stack([-qhat[:, 1], qhat[:, 0]], 1)
+        keys = [(0, 0), (1, 0), (0, 1), (-1, -1)]
+        vals = [(0.0, 0.0, 0.3)] + [(-1j * ch * e[j, 0], -1j * ch * e[j, 1], 1.0) for j in range(3)]
+        hf = opn.make_hf_skx_pin(keys, vals, q1, q2, 0.1)(r).numpy()
+        sk = opn.make_pin("skx" if neel else "skx_bloch", q1, q2, 0.1, chirality=ch)(r).numpy()
+        assert np.max(np.abs(hf - sk)) < 1e-14
