@@ -4,6 +4,7 @@
 #include "model.h"
 #include "physics.h"
 #include "minsr.h"
+#include "obs.h"
 
 namespace nqs {
 
@@ -124,6 +125,47 @@ static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void*
   a.tanf = tanf;
   a.out = eloc;
   return launch_eloc_assemble(a, st);
+}
+
+// Spin estimators: log psi of every walker, then of every single-spin-flipped configuration
+// (value only, cfg = w N + i), then the per-electron estimators from the ratios.
+static int64_t spin_fwd_bytes(const Model& md, int64_t W) {
+  const int64_t c1 = chunk_walkers(md, W, 1, 1), c2 = chunk_walkers(md, W, md.N, 1);
+  return std::max(fwd_bytes(md, c1, 1, 0), fwd_bytes(md, c2 * md.N, 1, 0));
+}
+
+template <typename T>
+static int spin_impl(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t W, double* out,
+                     char* ws, hipStream_t st) {
+  const int N = md.N;
+  const int64_t c1 = chunk_walkers(md, W, 1, 1), c2 = chunk_walkers(md, W, N, 1);
+  char* p = ws;
+  char* fwd_base = p;
+  p += al(spin_fwd_bytes(md, W));
+  double* lp = (double*)p;
+  p += al(W * 16);
+  double* lpf = (double*)p;
+  {
+    char* q = fwd_base;
+    FwdBufs<T> b = carve_fwd<T>(md, q, c1, 1, 0);
+    for (int64_t w0 = 0; w0 < W; w0 += c1) {
+      JetGeom jg{std::min(c1, W - w0), N, 1, 0, 0, 0};
+      FwdBufs<T> bb = b;
+      bb.lp = lp + 2 * w0;
+      NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
+    }
+  }
+  {
+    char* q = fwd_base;
+    FwdBufs<T> b = carve_fwd<T>(md, q, c2 * N, 1, 0);
+    for (int64_t w0 = 0; w0 < W; w0 += c2) {
+      JetGeom jg{std::min(c2, W - w0) * N, N, 1, 0, 0, 2};
+      FwdBufs<T> bb = b;
+      bb.lp = lpf + 2 * w0 * N;
+      NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
+    }
+  }
+  return launch_spin_est(W, N, S, lp, lpf, out, st);
 }
 
 static int64_t grad_chunk(const Model& md, int64_t W) {
@@ -375,6 +417,8 @@ int64_t nqs_workspace_bytes(const nqs_model_desc* m, int32_t op, int64_t W) {
       McmcPlan p = mcmc_plan(md, W);
       return al(p.fwd) + al(W * md.N * 16) + al(W * md.N * 4) + al(W * 16);
     }
+    case NQS_OP_SPIN:
+      return al(spin_fwd_bytes(md, W)) + al(W * 16) + al(W * md.N * 16);
     default:
       set_error("unknown op");
       return -1;
@@ -412,6 +456,16 @@ int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void*
   if (md.tsize == 8)
     return eloc_impl<double>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
   return eloc_impl<float>(md, h, packed, R, S, W, eloc, lp, grad, lap, (char*)ws, (hipStream_t)stream);
+}
+
+int nqs_spin_estimators(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,
+                        double* spins, void* ws, int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && spins && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_TRY(check_ws(m, NQS_OP_SPIN, W, ws_bytes));
+  if (md.tsize == 8) return spin_impl<double>(md, packed, R, S, W, spins, (char*)ws, (hipStream_t)stream);
+  return spin_impl<float>(md, packed, R, S, W, spins, (char*)ws, (hipStream_t)stream);
 }
 
 int nqs_psiformer_grad(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,

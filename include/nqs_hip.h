@@ -48,7 +48,7 @@ enum { NQS_FP64 = 0, NQS_MIXED = 1 };
 enum { NQS_GEMM_BF16X3 = 0, NQS_GEMM_FP32 = 1, NQS_GEMM_BF16 = 2 };
 
 /* ops for nqs_workspace_bytes */
-enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3 };
+enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3, NQS_OP_SPIN = 4 };
 
 /* Architecture + cell of ContinuumSpinorPsiformer (src/nqs/psiformer.py:100-116). */
 typedef struct nqs_model_desc {
@@ -205,6 +205,32 @@ int nqs_minsr_profile(int32_t enable, float* ms_out);
 int nqs_autotune_enable(int32_t enable);
 int nqs_autotune_reset(void);
 int64_t nqs_autotune_report(char* buf, int64_t len);
+
+/* ---- spin observables (src/nqs/observables.py:31-128)
+ * Single-particle spin estimators of make_spin_estimators (observables.py:31-47) for every
+ * electron of every walker: ratio_i = exp(log psi(S with s_i flipped) - log psi(S)) (non-finite
+ * -> 0), sigma_x = ratio_i, sigma_y = -i (1 - 2 s_i) ratio_i, sigma_z = 1 - 2 s_i.  spins_dev:
+ * (W, N, 3) complex (x, y, z).  Workspace: nqs_workspace_bytes(m, NQS_OP_SPIN, W). */
+int nqs_spin_estimators(const nqs_model_desc* m, const void* packed_dev, const double* R_dev, const int32_t* S_dev,
+                        int64_t n_walkers, double* spins_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+/* Spin Fourier sums over walkers (structure_factor, spin_order_parameter: observables.py:94-124)
+ * from real spins (W, N, 3): per wave vector q (n_q, 2), out (n_q, 8) = m_x(q), m_y(q), m_z(q)
+ * (complex, <sum_i sigma_a^i exp(-i q.r_i)>/N), S_zz(q), S_perp(q) (<|.|^2>/N). */
+int nqs_spin_fourier(const double* R_dev, const double* spins_dev, int64_t n_walkers, int32_t n_elec,
+                     const double* q_dev, int32_t n_q, double* out_dev, void* stream);
+/* Texture sums of spin_texture (observables.py:56-76): bin (ix, iy) of each electron from its
+ * fractional coordinates mod 1 (cell_inv = A^-1 row-major, host memory), m_dev (ngrid, ngrid, 3)
+ * = sum of the real spins in the bin, rho_dev (ngrid, ngrid) = count (not yet divided by W). */
+int nqs_spin_texture(const double* R_dev, const double* spins_dev, int64_t n_walkers, int32_t n_elec,
+                     const double cell_inv[4], int32_t ngrid, double* m_dev, double* rho_dev, void* stream);
+
+/* ---- ratio estimators (mc_overlap, src/nqs/manybody_chern.py:23-34; rotation_eigenvalue_vmc,
+ * src/nqs/topology.py:114-122): r_w = phase_w exp(lb_w - la_w - shift) from two log psi arrays
+ * (W complex) and an optional phase (W complex, NULL = 1); shift = max_w Re(lb - la) when
+ * shift != 0, else 0.  out_dev[6] = mean Re r, mean Im r, mean |r|^2, effective-sample fraction
+ * (sum|r|)^2 / sum|r|^2 / W, shift, std r (sqrt(mean |r - mean r|^2)). */
+int nqs_ratio_stats(const double* la_dev, const double* lb_dev, const double* phase_dev, int64_t n_walkers,
+                    int32_t shift, double* out_dev, void* stream);
 
 /* Median/MAD clip of Re e (minsr.py:145-148), W <= 16384: e_clip_dev (W) complex (Im kept);
  * stats_dev[4] = mean Re e (unclipped), std(Re e)/sqrt(W), median, MAD. */

@@ -367,3 +367,66 @@ def test_oracle_simple_pins_and_hf_texture():
         hf = opn.make_hf_skx_pin(keys, vals, q1, q2, 0.1)(r).numpy()
         sk = opn.make_pin("skx" if neel else "skx_bloch", q1, q2, 0.1, chirality=ch)(r).numpy()
         assert np.max(np.abs(hf - sk)) < 1e-14
+
+
+def test_oracle_spin_estimators_spin_independent_trial():
+    """observables.py:31-47 restated; test_nqs_observables.py:17-39's analytic case: a
+    spin-independent trial is the +x product state, so sigma_x = 1 and Re sigma_y = 0 exactly for
+    every electron, and the texture sums give m_x = 1 per electron."""
+    from oracle import observables as oob
+
+    def logpsi(R, S):
+        return (-0.1 * (R ** 2).sum()).to(torch.complex128)
+    rng = np.random.default_rng(0)
+    N, L = 6, 5.0
+    R = rng.uniform(0, L, (20, N, 2))
+    S = rng.integers(0, 2, (20, N))
+    sp = []
+    for w in range(20):
+        sx, sy, sz = oob.spin_estimators(logpsi, R[w], S[w])
+        assert np.max(np.abs(sx - 1.0)) == 0.0 and np.max(np.abs(sy.real)) == 0.0
+        assert np.array_equal(sz.real, 1.0 - 2.0 * S[w])
+        sp.append(np.stack([sx.real, sy.real, sz.real], -1))
+    sp = np.array(sp)
+    m, rho = oob.texture_sums(R, sp, np.diag([L, L]), 7)
+    assert rho.sum() == 20 * N and abs(m[..., 0].sum() / (20 * N) - 1.0) < 1e-15
+    assert np.array_equal(m[..., 0], rho)
+
+
+def test_oracle_structure_factor_and_order_parameter():
+    """observables.py:94-121 restated: S_zz(0) = <(sum_i sz_i)^2>/N exactly; uncorrelated spins give
+    a flat S_zz (test_nqs_observables.py:88-101); +x spins give m_x(0) = 1 and S_perp(0) = N."""
+    from oracle import observables as oob
+    rng = np.random.default_rng(1)
+    W, N, L = 1200, 8, 5.0
+    R = rng.uniform(0, L, (W, N, 2))
+    sz = 1.0 - 2.0 * rng.integers(0, 2, (W, N))
+    sp = np.stack([np.ones((W, N)), np.zeros((W, N)), sz], -1)
+    B = 2 * np.pi / L
+    qs = np.array([(i * B, j * B) for i in range(-3, 4) for j in range(-3, 4)])
+    m, szz, sperp = oob.spin_fourier(R, sp, qs)
+    k0 = np.where(np.linalg.norm(qs, axis=1) < 1e-12)[0][0]
+    assert abs(szz[k0] - np.mean(sz.sum(1) ** 2) / N) < 1e-12
+    assert abs(m[0, k0] - 1.0) < 1e-15 and abs(sperp[k0] - N) < 1e-12
+    fin = np.linalg.norm(qs, axis=1) > 1e-12
+    assert szz[fin].max() < 2.5 and 0.5 < szz[fin].mean() < 1.6
+
+
+def test_oracle_berg_luscher_and_ratio_stats():
+    """observables.py:79-91 restated: charge 1 on the triple-Q skyrmion texture (equal to the
+    oracle pinning module's solid-angle sum) and 0 on a uniform one; ratio statistics of a constant
+    ratio are exact (ess 1, std 0) and the shift cancels in mean r / sqrt(mean |r|^2)."""
+    from oracle import observables as oob, pinning as opn
+    A, q1, q2 = _hex_cell()
+    f = opn.make_skx_pin(q1, q2, 1.0, m0=0.5)
+    ng = 24
+    fr = np.arange(ng) / ng
+    u, v = np.meshgrid(fr, fr, indexing="ij")
+    m = f(torch.tensor(np.stack([u, v], -1) @ A.T)).numpy()
+    assert abs(oob.berg_luscher(m) - 1.0) < 1e-9
+    assert abs(oob.berg_luscher(m) - opn.skyrmion_charge(f, A, ng)) < 1e-9
+    assert abs(oob.berg_luscher(np.tile([0.0, 0.0, 1.0], (ng, ng, 1)))) < 1e-12
+    la = np.random.default_rng(2).normal(size=50) + 1j * np.random.default_rng(3).normal(size=50)
+    st = oob.ratio_stats(la, la + (0.3 + 0.4j))
+    assert abs(st[3] - 1.0) < 1e-12 and st[5] < 1e-12
+    assert abs(complex(st[0], st[1]) / np.sqrt(st[2]) - np.exp(0.4j)) < 1e-12
