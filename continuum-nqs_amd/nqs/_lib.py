@@ -64,6 +64,10 @@ SIGNATURES = [
     ("nqs_minsr_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
     ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),
+    ("nqs_vmc_gradient_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
+    ("nqs_vmc_gradient", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_adam_update", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_i32,
+                                c_vp]),
     ("nqs_spin_estimators", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_spin_fourier", c_i32, [c_vp, c_vp, c_i64, c_i32, c_vp, c_i32, c_vp, c_vp]),
     ("nqs_spin_texture", c_i32, [c_vp, c_vp, c_i64, c_i32, ctypes.POINTER(c_dbl), c_i32, c_vp, c_vp, c_vp]),

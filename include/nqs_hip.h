@@ -206,6 +206,26 @@ int nqs_autotune_enable(int32_t enable);
 int nqs_autotune_reset(void);
 int64_t nqs_autotune_report(char* buf, int64_t len);
 
+/* ---- first-order training (src/nqs/train.py:28-65)
+ * Energy gradient of the surrogate loss 2 mean_w Re[conj(e_w - <e>) log psi(w)] (train.py:48-58):
+ * g_dev (P) = (2/W) sum_w [Re(e_w - <e>) Re O_w + Im(e_w - <e>) Im O_w], from the O rows of
+ * nqs_psiformer_grad (desc: n_walkers, n_params, ldo, o_dtype; the other fields are ignored) and the
+ * local energies e_dev (W complex).  O is read, not modified. */
+int64_t nqs_vmc_gradient_workspace_bytes(const nqs_minsr_desc* d);
+int nqs_vmc_gradient(const nqs_minsr_desc* d, const void* O_dev, const double* e_dev, double* g_dev, void* ws_dev,
+                     int64_t ws_bytes, void* stream);
+/* Adam step of optax.adam(lr, b1, b2, eps, eps_root) (optax scale_by_adam, then -lr) on the flat
+ * real ravel theta (P), in place with its moments mu, nu (P each, zero before the first step);
+ * count = 1-based step number.  partner_dev (P, int64, NULL = all real): for the real part of a
+ * complex leaf element, the index of its imaginary part; -2 at that imaginary part; -1 for real
+ * parameters.  A complex element is one Adam element (complex first moment, second moment |g|^2
+ * shared by both parts); complex_mode 0 feeds Adam g_re - i g_im (what jax.grad of a real loss
+ * gives for a complex parameter, as the reference's optax step receives it), 1 feeds
+ * g_re + i g_im (a descent step in both parts). */
+int nqs_adam_update(double* theta_dev, const double* g_dev, double* mu_dev, double* nu_dev, const int64_t* partner_dev,
+                    int64_t n_params, int64_t count, double lr, double b1, double b2, double eps, double eps_root,
+                    int32_t complex_mode, void* stream);
+
 /* ---- spin observables (src/nqs/observables.py:31-128)
  * Single-particle spin estimators of make_spin_estimators (observables.py:31-47) for every
  * electron of every walker: ratio_i = exp(log psi(S with s_i flipped) - log psi(S)) (non-finite

@@ -430,3 +430,80 @@ def test_oracle_berg_luscher_and_ratio_stats():
     st = oob.ratio_stats(la, la + (0.3 + 0.4j))
     assert abs(st[3] - 1.0) < 1e-12 and st[5] < 1e-12
     assert abs(complex(st[0], st[1]) / np.sqrt(st[2]) - np.exp(0.4j)) < 1e-12
+
+
+def _small_arch():
+    g = bands.rashba_gs(7.0, 4, 1.0, 1.0, 0.5, n_cut=8, twist=(0.1, 0.17))
+    arch = opf.Arch(4, g["Gs"], g["W"], 7.0, embed_dim=8, depth=1, n_heads=2, n_freq=2)
+    return arch, opf.init_theta(arch, 1, backflow_scale=0.05, scale_all=0.02)
+
+
+def test_oracle_surrogate_gradient_pins():
+    """train.py:48-58 restated: reverse mode through the surrogate equals the per-sample form
+    (2/W) sum_w Re[conj(e_w - <e>) O_w] with the oracle's O, and central differences."""
+    from oracle import train as otr
+    arch, theta = _small_arch()
+    rng = np.random.default_rng(3)
+    R, S = rng.uniform(0, 7.0, (5, 4, 2)), rng.integers(0, 2, (5, 4))
+    e = rng.normal(size=5) + 1j * rng.normal(size=5)
+    g = otr.surrogate_gradient(arch, theta, R, S, e).numpy()
+    O = oms.per_sample_O(arch, theta, R, S).numpy()
+    ec = e - e.mean()
+    g2 = (2.0 / 5) * (ec.real[:, None] * O.real + ec.imag[:, None] * O.imag).sum(0)
+    assert np.max(np.abs(g - g2)) < 1e-12 * max(1.0, np.max(np.abs(g)))
+    for k in range(2):
+        v = torch.tensor(rng.normal(size=theta.shape[0]))
+        h = 1e-6
+        fd = (otr.surrogate(arch, theta + h * v, R, S, e) - otr.surrogate(arch, theta - h * v, R, S, e)) / (2 * h)
+        assert abs(float(fd) - float(g @ v.numpy())) < 1e-6 * max(1.0, abs(float(fd)))
+
+
+def test_oracle_adam_pins():
+    """optax.adam restated: real leaves follow torch.optim.Adam; a constant gradient gives the
+    step -lr g/(|g| + eps) at every count (complex elements with g_re -/+ i g_im); a complex
+    element with g_im = 0 moves like a real one."""
+    from oracle import train as otr
+    arch, theta = _small_arch()
+    layout, P = opf.ravel_layout(arch)
+    real_idx = np.concatenate([np.arange(off, off + (int(np.prod(s)) if len(s) else 1))
+                               for _, s, c, off in layout if not c])
+    rng = np.random.default_rng(4)
+    lr = 0.01
+    th_o, st = theta.numpy(), None
+    th_t = theta.clone().requires_grad_(True)
+    opt = torch.optim.Adam([th_t], lr=lr, betas=(0.9, 0.999), eps=1e-8)
+    for it in range(5):
+        g = rng.normal(size=P)
+        th_o, st = otr.adam_step(arch, th_o, g, st, lr=lr)
+        th_t.grad = torch.tensor(g)
+        opt.step()
+        ref = th_t.detach().numpy()
+        assert np.max(np.abs(th_o[real_idx] - ref[real_idx])) < 1e-14
+    for mode, sgn in (("jax", -1.0), ("descent", 1.0)):
+        g = rng.normal(size=P)
+        th, st = theta.numpy(), None
+        for it in range(4):
+            prev = th.copy()
+            th, st = otr.adam_step(arch, th, g, st, lr=lr, complex_grads=mode)
+            for _, s, c, off in layout:
+                n = int(np.prod(s)) if len(s) else 1
+                if c:
+                    gc = g[off:off + n] + sgn * 1j * g[off + n:off + 2 * n]
+                    dz = (th[off:off + n] - prev[off:off + n]) + 1j * (th[off + n:off + 2 * n] - prev[off + n:off + 2 * n])
+                    assert np.max(np.abs(dz + lr * gc / (np.abs(gc) + 1e-8))) < 1e-14
+                else:
+                    d = th[off:off + n] - prev[off:off + n]
+                    gg = g[off:off + n]
+                    assert np.max(np.abs(d + lr * gg / (np.abs(gg) + 1e-8))) < 1e-14
+    g = rng.normal(size=P)
+    for _, s, c, off in layout:
+        if c:
+            n = int(np.prod(s)) if len(s) else 1
+            g[off + n:off + 2 * n] = 0.0
+    th, _ = otr.adam_step(arch, theta.numpy(), g, None, lr=lr)
+    for _, s, c, off in layout:
+        if c:
+            n = int(np.prod(s)) if len(s) else 1
+            gg = g[off:off + n]
+            assert np.max(np.abs(th[off:off + n] - (theta.numpy()[off:off + n] - lr * gg / (np.abs(gg) + 1e-8)))) < 1e-15
+            assert np.array_equal(th[off + n:off + 2 * n], theta.numpy()[off + n:off + 2 * n])
