@@ -8,6 +8,7 @@
 // from the stored Q, K rather than keeping it per walker) and the per-walker weight / bias
 // gradients, which are written straight into the rows of the O matrix:
 //   row w            = Re O_w,      row W_total + w = Im O_w,      column = real_ravel index.
+#include <algorithm>
 #include "nqs_common.h"
 #include "kernels.h"
 #include "bwd.h"
@@ -54,18 +55,68 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs<T> a, int ob) {
   }
 }
 
+// Column form, used when the walker has at most 32 tokens: one workgroup per (cfg, channel, 256
+// output columns [, chunk of rows i]); each thread keeps the N cotangents of its column j in
+// registers and sweeps the rows i, reading X^T (staged in LDS, tokens padded to a multiple of 4)
+// as broadcasts.  O is written once, a wave writing 64 consecutive columns of one row i; X and G
+// are read once per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wgrad_cols(WgradArgs<T> a, int ib) {
+  extern __shared__ char smem_raw[];
+  T* Xt = reinterpret_cast<T*>(smem_raw);  // [ib][NP]
+  const int N = a.N, NP = (N + 3) & ~3;
+  const int64_t cfg = blockIdx.y >> 1;
+  const int c = blockIdx.y & 1;
+  const int i0 = blockIdx.z * ib;
+  const int nrow = min(ib, a.in - i0);
+  const T* X = a.X + cfg * (int64_t)N * a.ldx;
+  for (int e = threadIdx.x; e < ib * NP; e += blockDim.x) {
+    const int ii = e / NP, t = e % NP;
+    Xt[e] = (t < N && ii < nrow) ? X[(int64_t)t * a.ldx + i0 + ii] : (T)0;
+  }
+  __syncthreads();
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.out) return;
+  const T* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg + j;
+  T g[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) g[t] = t < N ? G[(int64_t)t * a.ldg] : (T)0;
+  const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
+  const int sg = j / a.seg_w, jj = j % a.seg_w;
+  T* O = a.O + row * a.ldo + a.seg_base[sg] + jj;
+  for (int ii = 0; ii < nrow; ++ii) {
+    const T* xr = Xt + ii * NP;
+    T s = 0;
+#pragma unroll
+    for (int t = 0; t < 32; t += 4) {
+      if (t < NP) s += xr[t] * g[t] + xr[t + 1] * g[t + 1] + xr[t + 2] * g[t + 2] + xr[t + 3] * g[t + 3];
+    }
+    O[(int64_t)(i0 + ii) * a.seg_w] = s;
+  }
+}
+
 // Output columns go in chunks of ob: all of them when the N x out cotangent tile takes at most
 // 96 KB of LDS, else the largest multiple of 64 that keeps the tile within 64 KB.
 template <typename T>
 int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
   if (a.n_cfg == 0) return NQS_OK;
+  NQS_REQUIRE(a.out % a.seg_w == 0 && a.out / a.seg_w <= 3, NQS_ERR_ARG, "bad output segmentation");
+  if (a.N <= 32) {
+    const int NP = (a.N + 3) & ~3;
+    int ib = (int)std::min<int64_t>(a.in, (48 * 1024) / ((int64_t)NP * sizeof(T)));
+    const size_t lds = sizeof(T) * (size_t)ib * NP;
+    hipLaunchKernelGGL(k_wgrad_cols<T>, dim3((unsigned)cdiv(a.out, 256), (unsigned)(a.n_cfg * 2),
+                                             (unsigned)cdiv(a.in, ib)),
+                       dim3(256), lds, st, a, ib);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  }
   int ob = a.out;
   if (sizeof(T) * (size_t)a.N * (16 + ob) > 96 * 1024) {
     ob = ((int)(64 * 1024 / (sizeof(T) * (size_t)a.N)) - 16) / 64 * 64;
     NQS_REQUIRE(ob >= 64, NQS_ERR_UNSUPPORTED, "weight-gradient tile exceeds LDS");
   }
   const size_t lds = sizeof(T) * (size_t)a.N * (16 + ob);
-  NQS_REQUIRE(a.out % a.seg_w == 0 && a.out / a.seg_w <= 3, NQS_ERR_ARG, "bad output segmentation");
   hipLaunchKernelGGL(k_wgrad<T>, dim3((unsigned)cdiv(a.in, 16), (unsigned)(a.n_cfg * 2), (unsigned)cdiv(a.out, ob)),
                      dim3(256), lds, st, a, ob);
   NQS_LAUNCH_CHECK();

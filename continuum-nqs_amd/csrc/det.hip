@@ -433,6 +433,8 @@ __global__ __launch_bounds__(256) void k_det_jet(JetGeom j, DetParams p, const d
 }
 
 // ------------------------------------------------------------------ wave-level head for N <= 16
+// plane waves whose phases the N <= 16 head tabulates in LDS (more are evaluated per entry)
+constexpr int kPhaseMax = 64;
 // One workgroup of 4 waves per configuration; the envelope jet is built once in LDS, then each
 // wave takes determinants k = wave, wave + 4, ... entirely in registers.  Matrices are padded to
 // 16 x 16 with an identity block (same determinant, inverse and traces).
@@ -576,6 +578,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
                                                    const T* __restrict__ bf, double* __restrict__ out_lp,
                                                    double* __restrict__ out_tan, double* __restrict__ out_lap) {
   __shared__ cd env[256], dex[256], dey[256], lenv[256];
+  __shared__ cd ph[kPhaseMax * 16];  // exp(i G_g . r_i), [g][i], when n_g <= kPhaseMax
   __shared__ double pos[32];
   __shared__ int spin[16];
   extern __shared__ cd res[];  // [K][nt + 2]: logdet, tangents, Laplacian
@@ -597,6 +600,17 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
     spin[tid] = (tid == flip) ? 1 - s : s;
   }
   __syncthreads();
+  // the phases depend on (g, i) only: evaluated once here instead of once per orbital
+  const bool tab = p.n_g <= kPhaseMax;
+  if (tab) {
+    for (int e = tid; e < p.n_g * 16; e += blockDim.x) {
+      const int gg = e >> 4, i = e & 15;
+      double sn = 0, cs = 0;
+      if (i < N) sincos(p.gs[2 * gg] * pos[2 * i] + p.gs[2 * gg + 1] * pos[2 * i + 1], &sn, &cs);
+      ph[e] = mk(cs, sn);
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < 256; e += blockDim.x) {  // envelope jet, entry (a, i) = (e / 16, e % 16)
     const int a = e >> 4, i = e & 15;
     cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0), lp = mk(0, 0);
@@ -604,7 +618,12 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
       for (int gg = 0; gg < p.n_g; ++gg) {
         const double Gx = p.gs[2 * gg], Gy = p.gs[2 * gg + 1];
         double sn, cs;
-        sincos(Gx * pos[2 * i] + Gy * pos[2 * i + 1], &sn, &cs);
+        if (tab) {
+          cs = ph[gg * 16 + i].re;
+          sn = ph[gg * 16 + i].im;
+        } else {
+          sincos(Gx * pos[2 * i] + Gy * pos[2 * i + 1], &sn, &cs);
+        }
         const double* wp = p.wenv + ((size_t)(a * p.n_g + gg) * 2 + spin[i]) * 2;
         const cd t = mk(wp[0], wp[1]) * mk(cs, sn);
         v = v + t;

@@ -780,48 +780,109 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
 }
 
 // ------------------------------------------------------------------ triangular solves with the factor
-// b_blk <- Linv_blk b_blk (trans = 0) or Linv_blk^T b_blk (trans = 1); one workgroup
-__global__ void k_trsv_diag(const double* __restrict__ Li, double* __restrict__ b, int nb, int trans) {
-  __shared__ double x[CH_NB];
+// L y = b then L^T x = y, in super-blocks of TS_SB = 4 x 64 rows.  Per super-block one workgroup
+// solves the diagonal part (the four 64 x 64 blocks through their inverses Linv, with the
+// in-super-block updates between them), then a grid of workgroups applies the super-block to the
+// rest of the vector: forward, b[r] -= L[r, J0:J0+256] y[J0:J0+256] for the rows below (one wave
+// per row, lanes along the row); backward, x[c] -= sum_r L[J0 + r][c] x[J0 + r] for the columns
+// to the left (one thread per column, rows streamed).
+#define TS_SB 256
+
+// sb = rows of this super-block (a multiple of 64, at most TS_SB; the last one may be short)
+__global__ __launch_bounds__(256) void k_trsv_super(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
+                                                    const double* __restrict__ Linv, double* __restrict__ b,
+                                                    int trans) {
+  __shared__ double x[TS_SB];
+  __shared__ double y[CH_NB];
   const int t = threadIdx.x;
-  if (t < nb) x[t] = b[t];
+  if (t < sb) x[t] = b[J0 + t];
   __syncthreads();
-  if (t < nb) {
+  const int nblk = sb / CH_NB;
+  for (int q = 0; q < nblk; ++q) {
+    const int blk = trans ? nblk - 1 - q : q;
+    const int o = blk * CH_NB;
+    const double* Li = Linv + ((J0 + o) / CH_NB) * CH_NB * CH_NB;
+    if (t < CH_NB) {
+      double s = 0;
+      for (int k = 0; k < CH_NB; ++k) s += (trans ? Li[k * CH_NB + t] : Li[t * CH_NB + k]) * x[o + k];
+      y[t] = s;
+    }
+    __syncthreads();
+    if (t < CH_NB) x[o + t] = y[t];
+    if (!trans) {
+      // rows below the block inside the super-block
+      if (t >= o + CH_NB && t < sb) {
+        double s = 0;
+        const double* row = A + (J0 + t) * lda + J0 + o;
+        for (int k = 0; k < CH_NB; ++k) s += row[k] * y[k];
+        x[t] -= s;
+      }
+    } else {
+      // columns left of the block inside the super-block
+      if (t < o) {
+        double s = 0;
+        for (int k = 0; k < CH_NB; ++k) s += A[(J0 + o + k) * lda + J0 + t] * y[k];
+        x[t] -= s;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < sb) b[J0 + t] = x[t];
+}
+
+// forward update: rows r in [r0, n): b[r] -= L[r, J0:J0+sb] . b[J0:J0+sb]; 8 rows per wave
+__global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
+                                                       int64_t r0, int64_t n, double* __restrict__ b) {
+  __shared__ double z[TS_SB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  z[t] = t < sb ? b[J0 + t] : 0.0;
+  __syncthreads();
+  const int64_t rbase = r0 + ((int64_t)blockIdx.x * 4 + wave) * 8;
+  for (int q = 0; q < 8; ++q) {
+    const int64_t r = rbase + q;
+    if (r >= n) break;
+    const double* row = A + r * lda + J0;
     double s = 0;
-    for (int k = 0; k < nb; ++k) s += (trans ? Li[k * nb + t] : Li[t * nb + k]) * x[k];
-    b[t] = s;
+    for (int c = lane; c < sb; c += 64) s += row[c] * z[c];
+    s = wave_sum(s);
+    if (lane == 0) b[r] -= s;
   }
 }
-// forward: rows r >= r0: b[r] -= sum_c L[r][j0 + c] z[c]
-__global__ void k_trsv_fwd_update(const double* __restrict__ A, int64_t lda, int64_t j0, int nb, int64_t n,
-                                  double* __restrict__ b) {
-  const int64_t r = j0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double s = 0;
-  for (int c = 0; c < nb; ++c) s += A[r * lda + j0 + c] * b[j0 + c];
-  b[r] -= s;
-}
-// backward: columns c < j0: z[c] -= sum_r L[j0 + r][c] y[j0 + r]
-__global__ void k_trsv_bwd_update(const double* __restrict__ A, int64_t lda, int64_t j0, int nb,
-                                  double* __restrict__ z) {
+
+// backward update: columns c < J0: x[c] -= sum_{r < sb} L[J0 + r][c] x[J0 + r]
+__global__ __launch_bounds__(256) void k_trsv_bwd_cols(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
+                                                       double* __restrict__ x) {
+  __shared__ double z[TS_SB];
+  z[threadIdx.x] = (int)threadIdx.x < sb ? x[J0 + threadIdx.x] : 0.0;
+  __syncthreads();
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= j0) return;
-  double s = 0;
-  for (int r = 0; r < nb; ++r) s += A[(j0 + r) * lda + c] * z[j0 + r];
-  z[c] -= s;
+  if (c >= J0) return;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int r = 0; r < sb; r += 4) {
+    s0 += A[(J0 + r) * lda + c] * z[r];
+    s1 += A[(J0 + r + 1) * lda + c] * z[r + 1];
+    s2 += A[(J0 + r + 2) * lda + c] * z[r + 2];
+    s3 += A[(J0 + r + 3) * lda + c] * z[r + 3];
+  }
+  x[c] -= (s0 + s1) + (s2 + s3);
 }
 
 int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st) {
-  for (int64_t j0 = 0; j0 < n; j0 += CH_NB) {
-    hipLaunchKernelGGL(k_trsv_diag, dim3(1), dim3(64), 0, st, Linv + (j0 / CH_NB) * CH_NB * CH_NB, x + j0, CH_NB, 0);
-    const int64_t m = n - j0 - CH_NB;
-    if (m > 0)
-      hipLaunchKernelGGL(k_trsv_fwd_update, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, st, L, lda, j0, CH_NB, n, x);
+  NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "triangular solve size must be a multiple of 64");
+  for (int64_t J0 = 0; J0 < n; J0 += TS_SB) {
+    const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 0);
+    const int64_t r0 = J0 + sb;
+    if (r0 < n)
+      hipLaunchKernelGGL(k_trsv_fwd_rows, dim3((unsigned)cdiv(n - r0, 32)), dim3(256), 0, st, L, lda, J0, sb, r0, n,
+                         x);
   }
-  for (int64_t j0 = n - CH_NB; j0 >= 0; j0 -= CH_NB) {
-    hipLaunchKernelGGL(k_trsv_diag, dim3(1), dim3(64), 0, st, Linv + (j0 / CH_NB) * CH_NB * CH_NB, x + j0, CH_NB, 1);
-    if (j0 > 0)
-      hipLaunchKernelGGL(k_trsv_bwd_update, dim3((unsigned)cdiv(j0, 256)), dim3(256), 0, st, L, lda, j0, CH_NB, x);
+  const int64_t last = ((n - 1) / TS_SB) * TS_SB;
+  for (int64_t J0 = last; J0 >= 0; J0 -= TS_SB) {
+    const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 1);
+    if (J0 > 0)
+      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 256)), dim3(256), 0, st, L, lda, J0, sb, x);
   }
   NQS_LAUNCH_CHECK();
   return NQS_OK;
