@@ -1,6 +1,7 @@
 // extern "C" entry points of libnqs_hip.so (declared in include/nqs_hip.h).
 #include <algorithm>
 #include <cstring>
+#include <vector>
 #include "model.h"
 #include "physics.h"
 #include "minsr.h"
@@ -285,7 +286,7 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   const int64_t W = d->n_walkers, n = 2 * W, P = d->n_params, ldo = d->ldo;
   MinsrWs w = minsr_ws(d, ws);
   NQS_TRY(minsr_mark(0, st));
-  NQS_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), st));
+  NQS_TRY(fill_i32(info, 1, 0, st));
   // the bf16 Gram reads only the centred bf16 copy, so O stays as given and the GEMVs below
   // apply the centring on the fly; the float64 / float32 Gram reads O, which is centred in place
   const bool keep = d->gram == NQS_GRAM_BF16;
@@ -343,6 +344,146 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   NQS_TRY(minsr_mark(4, st));
   NQS_TRY(gemv_cols_c<T>(O, ldo, W, P, w.y, mc, w.part, w.red, dth, P, st));
   NQS_TRY(minsr_mark(5, st));
+  return NQS_OK;
+}
+
+// ------------------------------------------------------------------ MinSR of many same-size systems
+// Systems share n = 2W.  Per-system buffers (column means, bf16 operand, GEMV scratch) are carved
+// one after another; the n x n matrices, inverse blocks, panels and right-hand sides of all
+// systems sit at uniform strides so that the Cholesky and the triangular solves take every
+// system in each launch, and the bf16 Grams of all systems are one launch.
+struct MinsrSysWs {
+  double *mean, *et, *u, *v, *ts, *sc, *part, *red;
+  __bf16* Ob;
+};
+struct MinsrBatchWs {
+  double *A64, *Linv, *panel, *Y, *Rr;
+  float* T32;
+  std::vector<MinsrSysWs> sys;
+  int64_t bytes;
+};
+static MinsrBatchWs minsr_batch_ws(const nqs_minsr_desc* d, int nsys, char* base) {
+  MinsrBatchWs w;
+  const int64_t n = 2 * d[0].n_walkers;
+  char* p = base;
+  auto grab = [&](int64_t b) {
+    char* r = p;
+    p += al(b);
+    return r;
+  };
+  w.A64 = (double*)grab(nsys * n * n * 8);
+  w.Linv = (double*)grab(nsys * n * 64 * 8);
+  w.panel = (double*)grab(nsys * n * 64 * 8);
+  w.Y = (double*)grab(nsys * n * 8);
+  w.Rr = (double*)grab(nsys * n * 8);
+  w.T32 = d[0].gram != NQS_GRAM_FP64 ? (float*)grab(nsys * n * n * 4) : nullptr;
+  for (int s = 0; s < nsys; ++s) {
+    MinsrSysWs q;
+    q.mean = (double*)grab(2 * d[s].n_params * 8);
+    q.Ob = d[s].gram == NQS_GRAM_BF16 ? (__bf16*)grab(n * d[s].ldo * 2) : nullptr;
+    q.et = (double*)grab(n * 8);
+    q.u = (double*)grab(d[s].ldo * 8);
+    q.v = (double*)grab(n * 8);
+    q.ts = (double*)grab(2 * n * 8);
+    q.sc = (double*)grab(8 * 8);
+    q.part = (double*)grab(colsum_part_elems(d[s].ldo) * 8);
+    q.red = (double*)grab(4 * 8);
+    w.sys.push_back(q);
+  }
+  w.bytes = p - base;
+  return w;
+}
+
+template <typename T>
+static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov, const double* const* eps,
+                              double* const* dth, int32_t* info, char* ws, hipStream_t st) {
+  const int64_t W = d[0].n_walkers, n = 2 * W;
+  MinsrBatchWs w = minsr_batch_ws(d, nsys, ws);
+  NQS_TRY(fill_i32(info, nsys, 0, st));
+  const bool keep = d[0].gram == NQS_GRAM_BF16;
+  for (int s = 0; s < nsys; ++s) {
+    const MinsrSysWs& q = w.sys[s];
+    NQS_TRY(minsr_center<T>((T*)Ov[s], d[s].ldo, W, d[s].n_params, q.mean, q.Ob, keep ? 0 : 1, q.part, st));
+    NQS_TRY(make_etil(eps[s], W, q.et, st));
+  }
+  if (keep) {
+    for (int s0 = 0; s0 < nsys; s0 += kGramBatch) {
+      const int cnt = std::min(kGramBatch, nsys - s0);
+      GramBatch gb;
+      for (int k = 0; k < cnt; ++k) {
+        gb.A[k] = w.sys[s0 + k].Ob;
+        gb.K[k] = d[s0 + k].ldo;
+        gb.C[k] = w.T32 + (s0 + k) * n * n;
+      }
+      NQS_TRY(gram_bf16_batched(gb, cnt, n, n, st));
+    }
+  }
+  for (int s = 0; s < nsys; ++s) {
+    double* A = w.A64 + s * n * n;
+    if (keep) {
+      NQS_TRY(to_f64_shift<float>(w.T32 + s * n * n, n, n, 0.0, A, n, st));
+    } else {
+      GemmArgs<T> g;
+      g.A = g.B = (const T*)Ov[s];
+      g.M = g.N = n;
+      g.K = d[s].n_params;
+      g.lda = g.ldb = d[s].ldo;
+      g.strideA = g.strideB = g.strideC = 0;
+      g.alpha = (T)1;
+      g.beta = (T)0;
+      g.bias = nullptr;
+      g.bias_period = 1;
+      g.bias_len = 0;
+      g.tri = 1;
+      g.mirror = 0;
+      g.split = 0;
+      g.ldc = n;
+      if (sizeof(T) == 8) {
+        g.C = (T*)A;
+        NQS_TRY(gemm_nt<T>(g, 1, st));
+      } else {
+        g.C = (T*)(w.T32 + s * n * n);
+        NQS_TRY(gemm_nt<T>(g, 1, st));
+        NQS_TRY(to_f64_shift<float>(w.T32 + s * n * n, n, n, 0.0, A, n, st));
+      }
+    }
+    NQS_TRY(deflate_centring(A, n, W, d[s].lam, w.sys[s].ts, w.sys[s].sc, st));
+  }
+  NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st, nsys, n * n, n * 64, n * 64));
+  for (int s = 0; s < nsys; ++s)
+    NQS_CHECK_HIP(hipMemcpyAsync(w.Y + s * n, w.sys[s].et, n * 8, hipMemcpyDeviceToDevice, st));
+  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Y, st, nsys, n * n, n * 64, n));
+  for (int it = 0; it < d[0].refine; ++it) {
+    for (int s = 0; s < nsys; ++s) {
+      const MinsrSysWs& q = w.sys[s];
+      const double* mc = keep ? q.mean : nullptr;
+      const T* O = (const T*)Ov[s];
+      double* y = w.Y + s * n;
+      double* r = w.Rr + s * n;
+      NQS_TRY(gemv_cols_c<T>(O, d[s].ldo, W, d[s].n_params, y, mc, q.part, q.red, q.u, d[s].ldo, st));
+      NQS_TRY(gemv_rows_c<T>(O, d[s].ldo, W, d[s].n_params, q.u, mc, q.red + 2, q.v, st));
+      NQS_TRY(axpby(n, 1.0, q.et, -1.0, q.v, r, st));
+      NQS_TRY(axpby(n, 1.0, r, -d[s].lam, y, r, st));
+    }
+    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Rr, st, nsys, n * n, n * 64, n));
+    NQS_TRY(axpby(nsys * n, 1.0, w.Y, 1.0, w.Rr, w.Y, st));
+  }
+  for (int s = 0; s < nsys; ++s) {
+    const MinsrSysWs& q = w.sys[s];
+    NQS_TRY(gemv_cols_c<T>((const T*)Ov[s], d[s].ldo, W, d[s].n_params, w.Y + s * n, keep ? q.mean : nullptr, q.part,
+                           q.red, dth[s], d[s].n_params, st));
+  }
+  return NQS_OK;
+}
+
+static int check_minsr_desc(const nqs_minsr_desc* d) {
+  NQS_REQUIRE(d->n_walkers > 0 && d->n_params > 0 && d->ldo >= d->n_params, NQS_ERR_ARG, "bad MinSR descriptor");
+  NQS_REQUIRE(d->ldo % 64 == 0, NQS_ERR_ARG, "ldo must be a multiple of 64");
+  NQS_REQUIRE((2 * d->n_walkers) % 64 == 0, NQS_ERR_UNSUPPORTED, "2W must be a multiple of 64");
+  NQS_REQUIRE(d->o_dtype == NQS_FP64 || d->o_dtype == NQS_MIXED, NQS_ERR_ARG, "bad O dtype");
+  NQS_REQUIRE(!(d->gram == NQS_GRAM_FP64 && d->o_dtype != NQS_FP64), NQS_ERR_ARG, "a float64 Gram needs a float64 O");
+  NQS_REQUIRE(!(d->gram == NQS_GRAM_FP32 && d->o_dtype != NQS_MIXED), NQS_ERR_ARG, "a float32 Gram needs a float32 O");
+  NQS_REQUIRE(d->gram >= 0 && d->gram <= 2 && d->refine >= 0, NQS_ERR_ARG, "bad Gram mode");
   return NQS_OK;
 }
 
@@ -510,6 +651,38 @@ int nqs_minsr_direction(const nqs_minsr_desc* d, void* O, const double* eps, dou
   hipStream_t st = (hipStream_t)stream;
   if (d->o_dtype == NQS_FP64) return minsr_impl<double>(d, (double*)O, eps, dth, info, (char*)ws, st);
   return minsr_impl<float>(d, (float*)O, eps, dth, info, (char*)ws, st);
+}
+
+int64_t nqs_minsr_batched_workspace_bytes(const nqs_minsr_desc* d, int32_t n_sys) {
+  if (!d || n_sys < 1) {
+    set_error("need at least one MinSR descriptor");
+    return -1;
+  }
+  for (int s = 0; s < n_sys; ++s) {
+    if (check_minsr_desc(d + s) != NQS_OK) return -1;
+    if (d[s].n_walkers != d[0].n_walkers || d[s].gram != d[0].gram || d[s].o_dtype != d[0].o_dtype ||
+        d[s].refine != d[0].refine) {
+      set_error("batched MinSR systems must share walkers, Gram mode, O dtype and refinement count");
+      return -1;
+    }
+  }
+  if (d[0].gram == NQS_GRAM_BF16 && (2 * d[0].n_walkers) % 256 != 0) {
+    set_error("the batched bf16 Gram needs 2W to be a multiple of 256");
+    return -1;
+  }
+  return minsr_batch_ws(d, n_sys, nullptr).bytes;
+}
+
+int nqs_minsr_direction_batched(const nqs_minsr_desc* d, int32_t n_sys, void* const* O, const double* const* eps,
+                                double* const* dth, int32_t* info, void* ws, int64_t ws_bytes, void* stream) {
+  NQS_REQUIRE(d && O && eps && dth && info && ws && n_sys >= 1, NQS_ERR_ARG, "null pointer or no systems");
+  for (int s = 0; s < n_sys; ++s) NQS_REQUIRE(O[s] && eps[s] && dth[s], NQS_ERR_ARG, "null system pointer");
+  const int64_t need = nqs_minsr_batched_workspace_bytes(d, n_sys);
+  NQS_REQUIRE(need >= 0, NQS_ERR_ARG, g_last_error);
+  NQS_REQUIRE(ws_bytes >= need, NQS_ERR_WORKSPACE, "batched MinSR workspace too small: need " + std::to_string(need));
+  hipStream_t st = (hipStream_t)stream;
+  if (d[0].o_dtype == NQS_FP64) return minsr_batched_impl<double>(d, n_sys, O, eps, dth, info, (char*)ws, st);
+  return minsr_batched_impl<float>(d, n_sys, O, eps, dth, info, (char*)ws, st);
 }
 
 int nqs_minsr_profile(int32_t enable, float* ms_out) {

@@ -21,11 +21,24 @@ int make_etil(const double* e, int64_t W, double* et, hipStream_t st);
 // true when minsr_center writes the bf16 copy in the blocked layout the 256-tile Gram reads
 bool gram_bf16_blocked(int64_t n, int64_t K);
 int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st);
+// Grams of up to kGramBatch same-size systems in one launch (operands in the blocked layout)
+constexpr int kGramBatch = 32;
+struct GramBatch {
+  const __bf16* A[kGramBatch];
+  int64_t K[kGramBatch];
+  float* C[kGramBatch];
+};
+int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st);
 template <typename T>
 int to_f64_shift(const T* Tin, int64_t ldt, int64_t n, double lam, double* A, int64_t lda, hipStream_t st);
 int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, double* sc, hipStream_t st);
-int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st);
-int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st);
+// nb same-size systems share each launch: system z uses A + z sA, Linv + z sL, panel + z sP,
+// info + z (x + z sx in the solve)
+int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st, int nb = 1,
+             int64_t sA = 0, int64_t sL = 0, int64_t sP = 0);
+int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb = 1,
+               int64_t sA = 0, int64_t sL = 0, int64_t sx = 0);
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 int axpby(int64_t n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
 int clip_stats(const double* e, int64_t W, double clip_k, double* e_clip, double* stats, hipStream_t st);
 

@@ -383,8 +383,11 @@ static int64_t supertile_count(int64_t nt) {
   return (int64_t)SUPER * SUPER * ns * (ns - 1) / 2 + (int64_t)SUPER * (SUPER + 1) / 2 * ns;
 }
 
-__global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict__ A, int64_t lda, int64_t K,
-                                                       float* __restrict__ C, int64_t ldc, int64_t ntiles, int nt) {
+// one launch computes the Grams of up to kGramBatch systems of the same size (blockIdx.y = system)
+__global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc, int64_t ntiles, int nt) {
+  const __bf16* __restrict__ A = gb.A[blockIdx.y];
+  const int64_t K = gb.K[blockIdx.y];
+  float* __restrict__ C = gb.C[blockIdx.y];
   extern __shared__ __bf16 g2s[];
   __bf16(*As)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s);
   __bf16(*Bs)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s + 2 * G2_BM * G2_LD);
@@ -470,23 +473,37 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(const __bf16* __restrict_
       }
 }
 
+int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st) {
+  NQS_REQUIRE(nsys >= 1 && nsys <= kGramBatch, NQS_ERR_ARG, "batched Gram takes 1 to 32 systems");
+  NQS_REQUIRE(gram_bf16_blocked(n, 64), NQS_ERR_UNSUPPORTED, "batched bf16 Gram needs 2W to be a multiple of 256");
+  for (int s = 0; s < nsys; ++s) {
+    NQS_REQUIRE(gb.K[s] % 64 == 0 && ((uintptr_t)gb.A[s] & 15) == 0, NQS_ERR_ARG,
+                "bf16 Gram operand must be 16-byte aligned with K a multiple of 64");
+  }
+  const int64_t nt = n / G2_BM;
+  const int64_t tiles = supertile_count(nt);
+  const size_t lds = sizeof(__bf16) * 4 * G2_BM * G2_LD;
+  static bool attr = false;
+  if (!attr) {
+    NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_syrk_bf16_256, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((tiles + 7) / 8)), (unsigned)nsys), dim3(512), lds, st, gb,
+                     ldc, tiles, (int)nt);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
 int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st) {
   NQS_REQUIRE(n % GB_BM == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs 2W to be a multiple of 128");
   NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
   if (gram_bf16_blocked(n, K)) {  // A in the blocked layout written by k_center
-    const int64_t nt = n / G2_BM;
-    const int64_t tiles = supertile_count(nt);
-    const size_t lds = sizeof(__bf16) * 4 * G2_BM * G2_LD;
-    static bool attr = false;
-    if (!attr) {
-      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_syrk_bf16_256, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-      attr = true;
-    }
-    hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(512), lds, st, A, lda, K, C, ldc,
-                       tiles, (int)nt);
-    NQS_LAUNCH_CHECK();
-    return NQS_OK;
+    GramBatch gb;
+    gb.A[0] = A;
+    gb.K[0] = K;
+    gb.C[0] = C;
+    return gram_bf16_batched(gb, 1, n, ldc, st);
   }
   NQS_REQUIRE(K % GB_BK == 0 && lda % 8 == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs K % 64 == 0 and lda % 8 == 0");
   NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
@@ -659,7 +676,11 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // forward substitution, column c in lane c, reading L from LDS (the same address on every lane).
 // info[0] = first non-positive pivot + 1 (the pivot is then taken as 1 and the factor goes on).
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0,
-                                                   double* __restrict__ Linv, int* __restrict__ info) {
+                                                   double* __restrict__ Linv, int* __restrict__ info, int64_t sA,
+                                                   int64_t sL) {
+  A += blockIdx.z * sA;
+  Linv += blockIdx.z * sL;
+  info += blockIdx.z;
   __shared__ double Ls[CH_NB][CH_NB + 1];
   const int i = threadIdx.x;
   double a[CH_NB];
@@ -712,7 +733,9 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
 }
 
 __global__ void k_copy_panel(const double* __restrict__ P, int64_t m, int nb, double* __restrict__ A, int64_t lda,
-                             int64_t r0, int64_t c0) {
+                             int64_t r0, int64_t c0, int64_t sP, int64_t sA) {
+  P += blockIdx.z * sP;
+  A += blockIdx.z * sA;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m * nb) return;
   const int64_t r = q / nb, c = q % nb;
@@ -745,35 +768,50 @@ static GemmArgs<double> g64(const double* A, const double* B, double* C, int64_t
 
 // A (n x n, lower) -> L with A = L L^T, in place.  Linv: (n/nb) diagonal-block inverses,
 // panel: n x nb scratch.
-int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st) {
+static GemmArgs<double> strided(GemmArgs<double> g, int64_t sA, int64_t sB, int64_t sC) {
+  g.strideA = sA;
+  g.strideB = sB;
+  g.strideC = sC;
+  return g;
+}
+
+// nb independent factorizations of the same size (system z at A + z sA, Linv + z sL,
+// panel + z sP, info + z) share every launch.
+int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st, int nb,
+             int64_t sA, int64_t sL, int64_t sP) {
   NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "Cholesky size must be a multiple of 64");
+  NQS_REQUIRE(nb >= 1 && nb <= 65535, NQS_ERR_ARG, "bad batch count");
   for (int64_t J0 = 0; J0 < n; J0 += CH_NBO) {
     const int64_t nbo = n - J0 < CH_NBO ? n - J0 : CH_NBO;
     for (int64_t j0 = J0; j0 < J0 + nbo; j0 += CH_NB) {
       double* Li = Linv + (j0 / CH_NB) * CH_NB * CH_NB;
-      hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, A, lda, j0, Li, info);
+      hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, (unsigned)nb), dim3(64), 0, st, A, lda, j0, Li, info, sA, sL);
       NQS_LAUNCH_CHECK();
       const int64_t m = n - j0 - CH_NB;
       if (m == 0) break;
       // panel = A21 Linv^T  (NT GEMM with B = Linv, K = 64)
       NQS_TRY(gemm_nt<double>(
-          g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0), 1, st));
+          strided(g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0), sA,
+                  sL, sP),
+          nb, st));
       // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= panel panel[0:w]^T (lower part)
       const int64_t w = J0 + nbo - (j0 + CH_NB);
       if (w > 0)
-        NQS_TRY(gemm_nt<double>(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NB, CH_NB, lda,
-                                    -1.0, 1.0, 1),
-                                1, st));
-      hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256)), dim3(256), 0, st, panel, m, CH_NB, A,
-                         lda, j0 + CH_NB, j0);
+        NQS_TRY(gemm_nt<double>(strided(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NB,
+                                            CH_NB, lda, -1.0, 1.0, 1),
+                                        sP, sP, sA),
+                                nb, st));
+      hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256), 1, (unsigned)nb), dim3(256), 0, st, panel,
+                         m, CH_NB, A, lda, j0 + CH_NB, j0, sP, sA);
       NQS_LAUNCH_CHECK();
     }
     // trailing update with the whole outer panel: A22 -= Lp Lp^T (lower tiles), K = nbo
     const int64_t M = n - J0 - nbo;
     if (M > 0) {
       const double* Lp = A + (J0 + nbo) * lda + J0;
-      NQS_TRY(gemm_nt<double>(g64(Lp, Lp, A + (J0 + nbo) * lda + J0 + nbo, M, M, nbo, lda, lda, lda, -1.0, 1.0, 1), 1,
-                              st));
+      NQS_TRY(gemm_nt<double>(
+          strided(g64(Lp, Lp, A + (J0 + nbo) * lda + J0 + nbo, M, M, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA), nb,
+          st));
     }
   }
   return NQS_OK;
@@ -791,7 +829,10 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
 // sb = rows of this super-block (a multiple of 64, at most TS_SB; the last one may be short)
 __global__ __launch_bounds__(256) void k_trsv_super(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
                                                     const double* __restrict__ Linv, double* __restrict__ b,
-                                                    int trans) {
+                                                    int trans, int64_t sA, int64_t sL, int64_t sx) {
+  A += blockIdx.z * sA;
+  Linv += blockIdx.z * sL;
+  b += blockIdx.z * sx;
   __shared__ double x[TS_SB];
   __shared__ double y[CH_NB];
   const int t = threadIdx.x;
@@ -832,7 +873,10 @@ __global__ __launch_bounds__(256) void k_trsv_super(const double* __restrict__ A
 
 // forward update: rows r in [r0, n): b[r] -= L[r, J0:J0+sb] . b[J0:J0+sb]; 8 rows per wave
 __global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                       int64_t r0, int64_t n, double* __restrict__ b) {
+                                                       int64_t r0, int64_t n, double* __restrict__ b, int64_t sA,
+                                                       int64_t sx) {
+  A += blockIdx.z * sA;
+  b += blockIdx.z * sx;
   __shared__ double z[TS_SB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   z[t] = t < sb ? b[J0 + t] : 0.0;
@@ -851,7 +895,9 @@ __global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict_
 
 // backward update: columns c < J0: x[c] -= sum_{r < sb} L[J0 + r][c] x[J0 + r]
 __global__ __launch_bounds__(256) void k_trsv_bwd_cols(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                       double* __restrict__ x) {
+                                                       double* __restrict__ x, int64_t sA, int64_t sx) {
+  A += blockIdx.z * sA;
+  x += blockIdx.z * sx;
   __shared__ double z[TS_SB];
   z[threadIdx.x] = (int)threadIdx.x < sb ? x[J0 + threadIdx.x] : 0.0;
   __syncthreads();
@@ -867,23 +913,38 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_cols(const double* __restrict_
   x[c] -= (s0 + s1) + (s2 + s3);
 }
 
-int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st) {
+int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb,
+               int64_t sA, int64_t sL, int64_t sx) {
   NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "triangular solve size must be a multiple of 64");
+  NQS_REQUIRE(nb >= 1 && nb <= 65535, NQS_ERR_ARG, "bad batch count");
+  const unsigned z = (unsigned)nb;
   for (int64_t J0 = 0; J0 < n; J0 += TS_SB) {
     const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 0);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 0, sA, sL, sx);
     const int64_t r0 = J0 + sb;
     if (r0 < n)
-      hipLaunchKernelGGL(k_trsv_fwd_rows, dim3((unsigned)cdiv(n - r0, 32)), dim3(256), 0, st, L, lda, J0, sb, r0, n,
-                         x);
+      hipLaunchKernelGGL(k_trsv_fwd_rows, dim3((unsigned)cdiv(n - r0, 32), 1, z), dim3(256), 0, st, L, lda, J0, sb, r0,
+                         n, x, sA, sx);
   }
   const int64_t last = ((n - 1) / TS_SB) * TS_SB;
   for (int64_t J0 = last; J0 >= 0; J0 -= TS_SB) {
     const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 1);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 1, sA, sL, sx);
     if (J0 > 0)
-      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 256)), dim3(256), 0, st, L, lda, J0, sb, x);
+      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 256), 1, z), dim3(256), 0, st, L, lda, J0, sb, x, sA,
+                         sx);
   }
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+// a fill kernel rather than a memset, so that a caller may capture the calls into a graph
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, p, n, v);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

@@ -64,6 +64,9 @@ SIGNATURES = [
     ("nqs_minsr_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
     ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),
+    ("nqs_minsr_batched_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc), c_i32]),
+    ("nqs_minsr_direction_batched", c_i32, [ctypes.POINTER(MinsrDesc), c_i32, ctypes.POINTER(c_vp),
+                                            ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_vp, c_vp, c_i64, c_vp]),
     ("nqs_vmc_gradient_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
     ("nqs_vmc_gradient", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_adam_update", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_i32,

@@ -195,6 +195,17 @@ int nqs_minsr_direction(const nqs_minsr_desc* d, void* O_dev, const double* eps_
  * switches hipEvent recording on (1) or off (0) for later calls; ms_out (5 floats, host) receives
  * the last profiled call's centre, Gram, Cholesky, solve+refinement, push-through times. */
 int nqs_minsr_profile(int32_t enable, float* ms_out);
+/* MinSR directions of n_sys independent systems with the same walker count (the batched uneven
+ * ensemble; no reference counterpart: the reference solves one system per call).  descs (n_sys)
+ * must share n_walkers, gram, o_dtype and refine; n_params, ldo and lam may differ.  O_dev,
+ * eps_dev, dtheta_dev are host arrays of n_sys device pointers, info_dev (n_sys) device int32.
+ * The bf16 Grams of all systems are one launch (2W a multiple of 256); the Cholesky
+ * factorizations and the triangular solves of all systems share each launch.  Per system the
+ * result equals nqs_minsr_direction's. */
+int64_t nqs_minsr_batched_workspace_bytes(const nqs_minsr_desc* descs, int32_t n_sys);
+int nqs_minsr_direction_batched(const nqs_minsr_desc* descs, int32_t n_sys, void* const* O_dev,
+                                const double* const* eps_dev, double* const* dtheta_dev, int32_t* info_dev,
+                                void* ws_dev, int64_t ws_bytes, void* stream);
 /* Launch-shape autotuner (no reference counterpart; the reference's XLA compiler has no such
  * knob): the first call of a tuned kernel (attention jet: waves per workgroup; determinant head:
  * workgroup size) for a shape class times each candidate once on the caller's stream and keeps the
