@@ -572,7 +572,9 @@ __device__ __forceinline__ cd bf_at(const T* __restrict__ bfc, int64_t rowlen, i
   return ok ? mk(re, im) : mk(0, 0);
 }
 
-template <typename T>
+// KIND (compile time, so that each variant keeps only the registers it needs): 0 log psi only,
+// 1 with tangents, 2 with tangents and the Laplacian.
+template <typename T, int KIND>
 __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
                                                    const int32_t* __restrict__ S, int64_t w0,
                                                    const T* __restrict__ bf, double* __restrict__ out_lp,
@@ -627,9 +629,11 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
         const double* wp = p.wenv + ((size_t)(a * p.n_g + gg) * 2 + spin[i]) * 2;
         const cd t = mk(wp[0], wp[1]) * mk(cs, sn);
         v = v + t;
-        gx = gx + t * mk(0, Gx);
-        gy = gy + t * mk(0, Gy);
-        lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
+        if constexpr (KIND > 0) {
+          gx = gx + t * mk(0, Gx);
+          gy = gy + t * mk(0, Gy);
+        }
+        if constexpr (KIND == 2) lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
       }
     }
     env[e] = v;
@@ -655,7 +659,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
     lu16_factor(f, lane);
     const cd ld = lu16_logdet(f, lane, N);
     if (lane == 0) res[k * rs_] = ld;
-    if (nt == 0 && !j.has_lap) continue;
+    if constexpr (KIND == 0) continue;
     cd mi[4];
     lu16_inverse(f, lane, mi);
     // (1 + bf) in the column-chunk layout
@@ -685,7 +689,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
       }
       tr = wave_sum_cd(tr);
       if (lane == 0) res[k * rs_ + c] = tr;
-      if (j.has_lap) {
+      if constexpr (KIND == 2) {
         d4v are = (d4v)0.0, aim = (d4v)0.0, tre = (d4v)0.0, tim = (d4v)0.0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -704,7 +708,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
         hsum = hsum + wave_sum_cd(h);
       }
     }
-    if (j.has_lap) {
+    if constexpr (KIND == 2) {
       const int cl = nt + 1;
       const T* bfl = bfc0 + cl * chs;
       cd tr = mk(0, 0);
@@ -746,17 +750,17 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
   const int c = lane + 1;
   const bool has_c = c <= nt;
   cd gsum = mk(0, 0), lsum = mk(0, 0);
-  if (has_c || j.has_lap)
+  if (has_c || KIND == 2)
     for (int k = 0; k < K; ++k) {
       const cd pk = pk_of(k);
       if (has_c) {
         const cd gk = res[k * rs_ + c];
         gsum = gsum + pk * gk;
-        if (j.has_lap) lsum = lsum + pk * (gk * gk);
+        if constexpr (KIND == 2) lsum = lsum + pk * (gk * gk);
       }
     }
   cd lap = mk(0, 0);
-  if (j.has_lap) {
+  if constexpr (KIND == 2) {
     cd lk = mk(0, 0);
     for (int k = 0; k < K; ++k) lk = lk + pk_of(k) * res[k * rs_ + nt + 1];
     const cd g2 = wave_sum_cd(has_c ? gsum * gsum : mk(0, 0));
@@ -803,7 +807,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
   if (lane == 0) {
     out_lp[2 * cfg] = L.re;
     out_lp[2 * cfg + 1] = L.im;
-    if (j.has_lap && out_lap) {
+    if (KIND == 2 && out_lap) {
       out_lap[2 * cfg] = lap.re;
       out_lap[2 * cfg + 1] = lap.im;
     }
@@ -827,9 +831,15 @@ int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const 
     static const char* const names[3] = {"2waves", "4waves", "8waves"};
     const TuneKey key{TUNE_DET16, (int64_t)sizeof(T) * 1000 + N * 64 + p.K, (int64_t)j.nt * 4 + j.has_lap * 2 +
                       (j.mode == 2), tune_bucket(j.n_cfg)};
+    const int kind = j.has_lap ? 2 : (j.nt > 0 ? 1 : 0);
     return tune_run(key, 3, 1, names, [&](int c) -> int {
-      hipLaunchKernelGGL(k_det_jet16<T>, dim3((unsigned)j.n_cfg), dim3(128 << c), lds, st, j, p, R, S, w0, bf,
-                         out_lp, out_tan, out_lap);
+      const dim3 grid((unsigned)j.n_cfg), block(128 << c);
+      if (kind == 0)
+        hipLaunchKernelGGL((k_det_jet16<T, 0>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
+      else if (kind == 1)
+        hipLaunchKernelGGL((k_det_jet16<T, 1>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
+      else
+        hipLaunchKernelGGL((k_det_jet16<T, 2>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
       NQS_LAUNCH_CHECK();
       return NQS_OK;
     }, st);
