@@ -97,3 +97,14 @@ def test_small_shapes_each_gemm_mode(gemm):
     err = float((v - ref).real.abs().max())
     print(f"gemm={gemm}: small-shape max|dlogpsi.re|={err:.3e}", flush=True)
     assert err < tol
+
+
+def test_batch_size_does_not_change_the_bits():
+    """Every walker's log psi is a function of that walker alone: a large batch (one launch per
+    layer over 131k rows) equals, bit for bit, the same walkers evaluated in batches of 1024."""
+    sysd = _system(8192, seed=4)
+    m = ContinuumSpinorPsiformer(n_elec=N_E, Gs=sysd["Gs"], W=sysd["Wenv"], L=sysd["L"], precision="mixed", **ARCH)
+    lp = LogPsi(m, sysd["theta"])
+    big = lp(sysd["R"], sysd["S"])
+    small = torch.cat([lp(sysd["R"][i:i + 1024], sysd["S"][i:i + 1024]) for i in range(0, 8192, 1024)])
+    assert torch.equal(big, small)
