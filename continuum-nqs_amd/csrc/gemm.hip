@@ -93,7 +93,7 @@ struct Vec16<float> {
 };
 
 template <typename T, int BK>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(GemmArgs<T> g) {
+__global__ __launch_bounds__(256, 2) void k_gemm_nt_v2(GemmArgs<T> g) {
   constexpr int BM = 128, VN = Vec16<T>::n, PAD = VN, LD = BK + PAD;
   constexpr int CH = BK / VN;         // 16-byte chunks per tile row
   constexpr int PER = 128 * CH / 256; // chunks per thread per operand
@@ -164,19 +164,24 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(Gemm
   }
 #undef NQS_GEMM_LOAD
 #undef NQS_GEMM_STORE
-  // Epilogue in two phases: every C / bias operand is read first (clamped addresses, uniform
-  // branches only), then the tile is combined and stored under the per-element predicate.
+  // Epilogue in two phases per 16-row block i (so that only 16 operands are live at a time):
+  // every C / bias operand of the block is read first (clamped addresses, uniform branches only),
+  // then the block is combined and stored under the per-element predicate.
   const bool has_beta = g.beta != (T)0, has_bias = g.bias != nullptr;
-  T addv[4][4][4];
+  T bv[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int j = 0; j < 4; ++j) {
+    const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+    bv[j] = has_bias ? g.bias[col < g.N ? col : g.N - 1] : (T)0;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    T addv[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) addv[i][j][r] = 0;
-  if (has_beta) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 4; ++r) addv[j][r] = 0;
+    if (has_beta) {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -184,25 +189,18 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(Gemm
           const int64_t row = m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r);
           const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
           const int64_t rr = row < g.M ? row : g.M - 1, cc = col < g.N ? col : g.N - 1;
-          addv[i][j][r] = g.beta * C[rr * g.ldc + cc];
+          addv[j][r] = g.beta * C[rr * g.ldc + cc];
         }
-  }
-  if (has_bias) {
+    }
+    if (has_bias) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
-      const T bv = g.bias[col < g.N ? col : g.N - 1];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t row = (uint32_t)(m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r));
-          addv[i][j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv : (T)0;
+          addv[j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv[j] : (T)0;
         }
     }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -210,12 +208,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_gemm_nt_v2(Gemm
         const int64_t row = m0 + wm * 64 + i * 16 + Mma16<T>::row(lane, r);
         const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
         const bool in = row < g.M && col < g.N && !(g.tri == 1 && col > row);
-        const T v = g.alpha * acc[i][j][r] + addv[i][j][r];
+        const T v = g.alpha * acc[i][j][r] + addv[j][r];
         if (in) {
           C[row * g.ldc + col] = v;
           if (g.mirror && row != col) C[col * g.ldc + row] = v;
         }
       }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
