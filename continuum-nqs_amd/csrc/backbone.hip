@@ -442,6 +442,30 @@ __device__ __forceinline__ void attn_load(const T* __restrict__ base, int lane, 
                                           int64_t rs, AttnRegs<T, NQ>& x) {
   const int li = lane & 15, g = lane >> 4;
   const int tq = li < N ? li : N - 1;
+  if constexpr (sizeof(T) == 4) {
+    if (hd == 16 * NQ) {
+      // full-width head: a lane's 4 consecutive q (k) features are one 16-byte load
+      typedef T v4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+      for (int t = 0; t < NQ; ++t) {
+        const v4 vq = *reinterpret_cast<const v4*>(base + tq * rs + h * hd + 16 * t + 4 * g);
+        const v4 vk = *reinterpret_cast<const v4*>(base + tq * rs + d + h * hd + 16 * t + 4 * g);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          x.q[4 * t + e] = li < N ? vq[e] : (T)0;
+          x.k[4 * t + e] = li < N ? vk[e] : (T)0;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4 * NQ; ++s) {
+        const int b = Mma16<T>::row(lane, s & 3), qv = 16 * (s >> 2) + li;
+        const int bc = b < N ? b : N - 1;
+        const T vv = base[bc * rs + 2 * d + h * hd + qv];
+        x.v[s] = b < N ? vv : (T)0;
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int s = 0; s < 4 * NQ; ++s) {
     const int q = 16 * (s >> 2) + 4 * g + (s & 3);
