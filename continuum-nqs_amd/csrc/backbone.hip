@@ -639,6 +639,214 @@ __global__ __launch_bounds__(512) void k_attention_jet_mfma(JetGeom j, int d, in
   attn_store<T, NQ>(out0 + cl * chs_out, lane, N, hd, d, h, o, (T)2, accPV);
 }
 
+// 16 < N <= 32: the same register/MFMA scheme with two key blocks, one wave per (cfg, head,
+// query block of 16).  Keys 16 kb + row(lane, r) sit in slot (kb, r); the softmax of a query
+// runs over both key blocks; P V sums the two blocks.  Head width hd == 16 NQ.
+template <typename T, int NQ>
+__global__ __launch_bounds__(512) void k_attention_jet_mfma32(JetGeom j, int d, int nh, const T* __restrict__ qkv,
+                                                              T* __restrict__ out, int64_t nwork) {
+  typedef typename Mma16<T>::acc_t acc_t;
+  constexpr int NKB = 2;
+  const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+  const int64_t work = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (work >= nwork) return;
+  const int N = j.N, hd = d / nh;
+  const int qb = (int)(work % NKB);
+  const int64_t pair = work / NKB;
+  const int64_t cfg = pair / nh;
+  const int h = (int)(pair % nh);
+  const int64_t rs = 3 * (int64_t)d;
+  const int64_t chs_in = (int64_t)N * rs, chs_out = (int64_t)N * d;
+  const T* in0 = qkv + cfg * j.C * chs_in;
+  T* out0 = out + cfg * j.C * chs_out;
+  const T sc = (T)1 / sqrt((T)hd);
+  const acc_t zacc = (acc_t)(T)0;
+  // q of this wave's 16 queries, k and v of all 32 keys (zeros past N)
+  auto load = [&](const T* base, T (&q)[4 * NQ], T (&k)[NKB][4 * NQ], T (&v)[NKB][4 * NQ]) {
+    const int tq = 16 * qb + li;
+    const int tqc = tq < N ? tq : N - 1;
+#pragma unroll
+    for (int s = 0; s < 4 * NQ; ++s) {
+      const int f = 16 * (s >> 2) + 4 * g + (s & 3);
+      const T vq = base[tqc * rs + h * hd + f];
+      q[s] = tq < N ? vq : (T)0;
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        const int tk = 16 * kb + li, tkc = tk < N ? tk : N - 1;
+        const T vk = base[tkc * rs + d + h * hd + f];
+        k[kb][s] = tk < N ? vk : (T)0;
+        const int b = 16 * kb + Mma16<T>::row(lane, s & 3), bc = b < N ? b : N - 1;
+        const T vv = base[bc * rs + 2 * d + h * hd + 16 * (s >> 2) + li];
+        v[kb][s] = b < N ? vv : (T)0;
+      }
+    }
+  };
+  auto store = [&](T* o, const acc_t (&acc)[NQ], T scale2, const acc_t (&add)[NQ]) {
+#pragma unroll
+    for (int t = 0; t < NQ; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = 16 * qb + Mma16<T>::row(lane, r), f = 16 * t + li;
+        if (a < N) o[a * d + h * hd + f] = acc[t][r] + scale2 * add[t][r];
+      }
+  };
+  acc_t zq[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) zq[t] = zacc;
+  // ---- value channel
+  T q0[4 * NQ], k0[NKB][4 * NQ], v0[NKB][4 * NQ];
+  load(in0, q0, k0, v0);
+  T P[NKB][4];
+  {
+    T sv[NKB][4], mx = -INFINITY;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      acc_t s0 = zacc;
+#pragma unroll
+      for (int s = 0; s < 4 * NQ; ++s) s0 = Mma16<T>::mma(k0[kb][s], q0[s], s0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sv[kb][r] = 16 * kb + Mma16<T>::row(lane, r) < N ? s0[r] * sc : -INFINITY;
+        mx = sv[kb][r] > mx ? sv[kb][r] : mx;
+      }
+    }
+    T o = __shfl_xor(mx, 16);
+    mx = o > mx ? o : mx;
+    o = __shfl_xor(mx, 32);
+    mx = o > mx ? o : mx;
+    T z = 0;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        P[kb][r] = exp(sv[kb][r] - mx);
+        z += P[kb][r];
+      }
+    const T iz = (T)1 / xsum16(z);
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) P[kb][r] *= iz;
+  }
+  {
+    acc_t o[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+      o[t] = zacc;
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) o[t] = Mma16<T>::mma(P[kb][s4], v0[kb][4 * t + s4], o[t]);
+    }
+    store(out0, o, (T)0, zq);
+  }
+  if (j.nt == 0 && !j.has_lap) return;
+  // ---- tangent channels
+  acc_t accS[NKB], accPV[NQ];
+  T accP[NKB][4];
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb) {
+    accS[kb] = zacc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) accP[kb][r] = 0;
+  }
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) accPV[t] = zacc;
+  for (int c = 1; c <= j.nt; ++c) {
+    T qc[4 * NQ], kc[NKB][4 * NQ], vc[NKB][4 * NQ];
+    load(in0 + c * chs_in, qc, kc, vc);
+    T sa[NKB][4];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      acc_t a = zacc;
+#pragma unroll
+      for (int s = 0; s < 4 * NQ; ++s) {
+        a = Mma16<T>::mma(k0[kb][s], qc[s], a);
+        a = Mma16<T>::mma(kc[kb][s], q0[s], a);
+        accS[kb] = Mma16<T>::mma(kc[kb][s], qc[s], accS[kb]);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sa[kb][r] = a[r] * sc;
+    }
+    T m = 0;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) m += P[kb][r] * sa[kb][r];
+    m = xsum16(m);
+    T u[NKB][4], m2 = 0;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        u[kb][r] = sa[kb][r] - m;
+        m2 += P[kb][r] * u[kb][r] * u[kb][r];
+      }
+    m2 = xsum16(m2);
+    T Pc[NKB][4];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accP[kb][r] += u[kb][r] * u[kb][r] - m2;
+        Pc[kb][r] = P[kb][r] * u[kb][r];
+      }
+    acc_t o[NQ];
+#pragma unroll
+    for (int t = 0; t < NQ; ++t) {
+      o[t] = zacc;
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          o[t] = Mma16<T>::mma(Pc[kb][s4], v0[kb][4 * t + s4], o[t]);
+          o[t] = Mma16<T>::mma(P[kb][s4], vc[kb][4 * t + s4], o[t]);
+          accPV[t] = Mma16<T>::mma(Pc[kb][s4], vc[kb][4 * t + s4], accPV[t]);
+        }
+    }
+    store(out0 + c * chs_out, o, (T)0, zq);
+  }
+  if (!j.has_lap) return;
+  // ---- Laplacian channel
+  const int cl = j.nt + 1;
+  T ql[4 * NQ], kl[NKB][4 * NQ], vl[NKB][4 * NQ];
+  load(in0 + cl * chs_in, ql, kl, vl);
+  T w[NKB][4], ml = 0;
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb) {
+    acc_t a = zacc;
+#pragma unroll
+    for (int s = 0; s < 4 * NQ; ++s) {
+      a = Mma16<T>::mma(k0[kb][s], ql[s], a);
+      a = Mma16<T>::mma(kl[kb][s], q0[s], a);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      w[kb][r] = (a[r] + (T)2 * accS[kb][r]) * sc;
+      ml += P[kb][r] * w[kb][r];
+    }
+  }
+  ml = xsum16(ml);
+  T Pl[NKB][4];
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Pl[kb][r] = P[kb][r] * (w[kb][r] - ml + accP[kb][r]);
+  acc_t o[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) {
+    o[t] = zacc;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        o[t] = Mma16<T>::mma(Pl[kb][s4], v0[kb][4 * t + s4], o[t]);
+        o[t] = Mma16<T>::mma(P[kb][s4], vl[kb][4 * t + s4], o[t]);
+      }
+  }
+  store(out0 + cl * chs_out, o, (T)2, accPV);
+}
+
 template <typename T>
 int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* out, hipStream_t st) {
   const int64_t npairs = j.n_cfg * n_heads;
@@ -659,6 +867,18 @@ int launch_attention_jet(const JetGeom& j, int d, int n_heads, const T* qkv, T* 
       NQS_LAUNCH_CHECK();
       return NQS_OK;
     }, st);
+  }
+  if (N <= 32 && (hd == 16 || (hd == 32 && sizeof(T) == 4))) {  // float64 with hd 32 would spill
+    const int64_t nwork = npairs * 2;
+    const int wpg = 4;
+    if (hd == 16)
+      hipLaunchKernelGGL((k_attention_jet_mfma32<T, 1>), dim3((unsigned)cdiv(nwork, wpg)), dim3(64 * wpg), 0, st, j, d,
+                         n_heads, qkv, out, nwork);
+    else
+      hipLaunchKernelGGL((k_attention_jet_mfma32<T, 2>), dim3((unsigned)cdiv(nwork, wpg)), dim3(64 * wpg), 0, st, j, d,
+                         n_heads, qkv, out, nwork);
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
   }
   const size_t per_wave = sizeof(T) * (size_t)(7 * N * hd + 5 * N * N + 2 * N);
   NQS_REQUIRE(per_wave <= 160 * 1024, NQS_ERR_UNSUPPORTED, "attention tile exceeds LDS");

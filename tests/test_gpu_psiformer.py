@@ -183,3 +183,29 @@ def test_psiformer_general_cell_matches_oracle():
     Eo = ole.make_local_energy(opf.make_logpsi(arch, theta), 1.0, 1.0, 0.3)
     er = np.array([Eo(torch.tensor(R[w]), torch.tensor(S[w])) for w in range(3)])
     assert np.max(np.abs(e - er)) < 1e-8 * max(1.0, np.max(np.abs(er)))
+
+
+@pytest.mark.parametrize("N,d,heads", [(20, 32, 2), (24, 64, 2), (27, 32, 2)])
+def test_attention_beyond_16_tokens(N, d, heads):
+    """16 < N <= 32 tokens (the register / MFMA attention with two key blocks for head width 16,
+    and for width 32 in float32): float64 log psi and E_loc against the oracle, mixed against
+    float64."""
+    L = 2.0 * np.sqrt(np.pi * N)
+    g = bands.rashba_gs(L, N, 1.0, 0.7, 0.2, n_cut=10, twist=(0.1, 0.17))
+    kw = dict(embed_dim=d, depth=1, n_heads=heads, n_freq=2, n_dets=2)
+    arch = opf.Arch(N, g["Gs"], g["W"], L, **kw)
+    theta = opf.init_theta(arch, 5, backflow_scale=0.05, scale_all=0.02)
+    R, S = _configs(3, N, L, 21)
+    lp64 = make_logpsi(ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, **kw), theta.cuda())
+    v = lp64(R, S).cpu().numpy()
+    ref = np.array([complex(opf.logpsi(arch, theta, torch.tensor(R[w]), torch.tensor(S[w]))) for w in range(3)])
+    assert np.max(np.abs(v.real - ref.real)) < 1e-10
+    e64 = make_local_energy(lp64, 1.0, 0.7, 0.2)(R[:1], S[:1]).cpu().numpy()
+    Eo = ole.make_local_energy(opf.make_logpsi(arch, theta), 1.0, 0.7, 0.2)
+    er = Eo(torch.tensor(R[0]), torch.tensor(S[0]))
+    assert abs(e64[0] - er) < 1e-8 * max(1.0, abs(er))
+    lpm = make_logpsi(ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, precision="mixed", **kw),
+                      theta.cuda())
+    assert np.max(np.abs(lpm(R, S).cpu().numpy().real - v.real)) < 2e-4
+    em = make_local_energy(lpm, 1.0, 0.7, 0.2)(R[:1], S[:1]).cpu().numpy()
+    assert abs(em[0] - e64[0]) < 5e-3 * max(1.0, abs(e64[0]))
