@@ -83,13 +83,13 @@ def cpu_baseline(sysd, n_walkers=8, seed=0):
                       f"stage seconds {json.dumps({k: round(v, 3) for k, v in tm.items()})}"}
 
 
-def gram_traffic(precision):
-    """HBM bytes per launch of the Gram kernel, measured with rocprofv3 PMC counters on this workload
-    (FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel); None when no measurement is present."""
+def gram_pmc(precision, field):
+    """A rocprofv3 PMC measurement of the Gram kernel on this workload (traffic_bytes: FETCH_SIZE /
+    WRITE_SIZE scaled by a plain copy kernel; mfma_util_pct: MfmaUtil); None when not measured."""
     f = os.path.join(ROOT, "profiles", "gram_hbm_traffic_n16_w8192.json")
     try:
         with open(f) as fh:
-            return float(json.load(fh)[precision]["traffic_bytes"])
+            return float(json.load(fh)[precision][field])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -210,7 +210,9 @@ def main():
            "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
                         "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / peak,
-                        "traffic": gram_traffic(args.precision) if W == 8192 else None, "gram_ms": float(mins[1])},
+                        "traffic": gram_pmc(args.precision, "traffic_bytes") if W == 8192 else None,
+                        "mfma_util_pct": gram_pmc(args.precision, "mfma_util_pct") if W == 8192 else None,
+                        "gram_ms": float(mins[1])},
            "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
                         "minsr": float(stages[3]), "update_repack": float(stages[4]),
                         "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
