@@ -448,7 +448,7 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
-    if (sizeof(T) == 8)
+    if constexpr (sizeof(T) == 8)
       hipLaunchKernelGGL((k_gemm_nt_v2<T, 16>), grid, dim3(256), 0, st, g);
     else
       hipLaunchKernelGGL((k_gemm_nt_v2<T, 32>), grid, dim3(256), 0, st, g);
@@ -456,7 +456,7 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
     return NQS_OK;
   }
   dim3 grid((unsigned)cdiv(g.N, 64), (unsigned)cdiv(g.M, 64), (unsigned)batch);
-  if (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 8)
     hipLaunchKernelGGL((k_gemm_nt<T, 16>), grid, dim3(256), 0, st, g);
   else
     hipLaunchKernelGGL((k_gemm_nt<T, 32>), grid, dim3(256), 0, st, g);
