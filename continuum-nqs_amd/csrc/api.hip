@@ -295,7 +295,8 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   NQS_TRY(make_etil(eps, W, w.et, st));
   NQS_TRY(minsr_mark(1, st));
   if (d->gram == NQS_GRAM_BF16) {
-    NQS_TRY(gram_bf16(w.Ob, ldo, n, ldo, w.T32, n, st));
+    // A64 is written only after the Gram, so it lends its space to the Gram's partials
+    NQS_TRY(gram_bf16(w.Ob, ldo, n, ldo, w.T32, n, (float*)w.A64, 2 * n * n, st));
     NQS_TRY(to_f64_shift<float>(w.T32, n, n, 0.0, w.A64, n, st));
   } else {
     GemmArgs<T> g;

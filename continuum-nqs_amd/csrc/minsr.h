@@ -20,7 +20,10 @@ int gemv_rows_c(const T* O, int64_t ldo, int64_t W, int64_t P, const double* u, 
 int make_etil(const double* e, int64_t W, double* et, hipStream_t st);
 // true when minsr_center writes the bf16 copy in the blocked layout the 256-tile Gram reads
 bool gram_bf16_blocked(int64_t n, int64_t K);
-int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st);
+// scratch (scratch_elems floats, may be null) holds the K-split partials of a last partial round
+// of tiles; it is free again when the call returns (on the stream)
+int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, float* scratch,
+              int64_t scratch_elems, hipStream_t st);
 // Grams of up to kGramBatch same-size systems in one launch (operands in the blocked layout)
 constexpr int kGramBatch = 32;
 struct GramBatch {
@@ -28,7 +31,8 @@ struct GramBatch {
   int64_t K[kGramBatch];
   float* C[kGramBatch];
 };
-int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st);
+int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st, float* scratch = nullptr,
+                      int64_t scratch_elems = 0);
 template <typename T>
 int to_f64_shift(const T* Tin, int64_t ldt, int64_t n, double lam, double* A, int64_t lda, hipStream_t st);
 int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, double* sc, hipStream_t st);

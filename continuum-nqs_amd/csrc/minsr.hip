@@ -383,21 +383,45 @@ static int64_t supertile_count(int64_t nt) {
   return (int64_t)SUPER * SUPER * ns * (ns - 1) / 2 + (int64_t)SUPER * (SUPER + 1) / 2 * ns;
 }
 
+// Split of the last, partly filled round of tiles: tiles [t0, t0 + ntail) (super-tile order) are
+// cut into nsplit chunks of kchunk k-steps, one workgroup per (tile, chunk), each writing a
+// 256 x 256 partial into part[(tile - t0) * nsplit + chunk]; k_gram_tail_sum adds the chunks in
+// a fixed order.  part == nullptr: whole tiles.
+struct GramTail {
+  float* part;
+  int64_t t0, ntail, kchunk;
+  int nsplit;
+};
+
 // one launch computes the Grams of up to kGramBatch systems of the same size (blockIdx.y = system)
-__global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc, int64_t ntiles, int nt) {
+__global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc, int64_t ntiles, int nt,
+                                                       GramTail tail) {
   const __bf16* __restrict__ A = gb.A[blockIdx.y];
   const int64_t K = gb.K[blockIdx.y];
   float* __restrict__ C = gb.C[blockIdx.y];
   extern __shared__ __bf16 g2s[];
   __bf16(*As)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s);
   __bf16(*Bs)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s + 2 * G2_BM * G2_LD);
-  const int64_t per_xcd = (ntiles + 7) / 8;
-  const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (t >= ntiles) return;
+  int64_t t, kb0 = 0, kb1 = K / G2_BK;
+  if (tail.part == nullptr) {
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (t >= ntiles) return;
+  } else {
+    const int64_t item = blockIdx.x;
+    if (item >= tail.ntail * tail.nsplit) return;
+    t = tail.t0 + item / tail.nsplit;
+    kb0 = (item % tail.nsplit) * tail.kchunk;
+    kb1 = kb0 + tail.kchunk < kb1 ? kb0 + tail.kchunk : kb1;
+    C = tail.part + item * (int64_t)(G2_BM * G2_BM);
+    ldc = G2_BM;
+  }
   int bm, bn;
   supertile_of(t, bm, bn);
   if (bm >= nt || bn > bm) return;
   const int64_t m0 = (int64_t)bm * G2_BM, n0 = (int64_t)bn * G2_BM;
+  // output origin: the tile in C, or the partial's own 256 x 256 square
+  const int64_t om = tail.part == nullptr ? m0 : 0, on = tail.part == nullptr ? n0 : 0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   f16v acc[2][4];
@@ -413,10 +437,10 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc
   u4v ra[4], rb[4];  // ext vectors: HIP's uint4 struct would be demoted to scratch here
   const int srow = tid >> 3, scol = (tid & 7) * 8;
   const int64_t nkb = K / G2_BK;
-  const __bf16* pa = A + (int64_t)bm * nkb * (G2_BM * G2_BK) + srow * G2_BK + scol;
-  const __bf16* pb = A + (int64_t)bn * nkb * (G2_BM * G2_BK) + srow * G2_BK + scol;
+  const __bf16* pa = A + ((int64_t)bm * nkb + kb0) * (G2_BM * G2_BK) + srow * G2_BK + scol;
+  const __bf16* pb = A + ((int64_t)bn * nkb + kb0) * (G2_BM * G2_BK) + srow * G2_BK + scol;
   const int64_t rstep = 64 * G2_BK;
-  const int64_t nk = K / G2_BK;
+  const int64_t nk = kb1 - kb0;  // >= 1: chunks are never empty
   // Software pipeline over k-steps: the registers carry stage kb + 1 across the loop back edge,
   // so its loads are issued a whole stage of MFMA work before they are stored to LDS (loads
   // consumed within one iteration get sunk next to their stores by the compiler).
@@ -467,13 +491,40 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int64_t col = n0 + wn * 128 + j * 32 + (lane & 31);
-        if (row >= col) C[row * ldc + col] = acc[i][j][r];
+        const int64_t row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = wn * 128 + j * 32 + (lane & 31);
+        if (m0 + row >= n0 + col) C[(om + row) * ldc + on + col] = acc[i][j][r];
       }
 }
 
-int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st) {
+// C (lower part of the tail tiles) = sum over the chunks of the partials, chunk 0 first.
+// grid (ntail, 256 rows), block 256 columns
+__global__ __launch_bounds__(256) void k_gram_tail_sum(GramTail tail, int nt, float* __restrict__ C, int64_t ldc) {
+  const int64_t i = blockIdx.x;
+  int bm, bn;
+  supertile_of(tail.t0 + i, bm, bn);
+  if (bm >= nt || bn > bm) return;
+  const int row = blockIdx.y, col = threadIdx.x;
+  const int64_t gr = (int64_t)bm * G2_BM + row, gc = (int64_t)bn * G2_BM + col;
+  if (gr < gc) return;
+  const float* p = tail.part + i * tail.nsplit * (int64_t)(G2_BM * G2_BM) + row * G2_BM + col;
+  float s = 0.f;
+  for (int q = 0; q < tail.nsplit; ++q) s += p[(int64_t)q * G2_BM * G2_BM];
+  C[gr * ldc + gc] = s;
+}
+
+// CUs of the current device (0 if unknown: no tail split then)
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cus[dev] == 0 && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus[dev] = 0;
+  return cus[dev];
+}
+
+int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hipStream_t st, float* scratch,
+                      int64_t scratch_elems) {
   NQS_REQUIRE(nsys >= 1 && nsys <= kGramBatch, NQS_ERR_ARG, "batched Gram takes 1 to 32 systems");
   NQS_REQUIRE(gram_bf16_blocked(n, 64), NQS_ERR_UNSUPPORTED, "batched bf16 Gram needs 2W to be a multiple of 256");
   for (int s = 0; s < nsys; ++s) {
@@ -489,13 +540,41 @@ int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hip
                                       (int)lds));
     attr = true;
   }
-  hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((tiles + 7) / 8)), (unsigned)nsys), dim3(512), lds, st, gb,
-                     ldc, tiles, (int)nt);
+  // One workgroup fills a CU (its LDS), so the tiles run in rounds of #CU.  With one system, a
+  // last round that would leave most CUs idle is split along K instead (partials in `scratch`).
+  GramTail tail{nullptr, 0, 0, 0, 1};
+  int64_t main_tiles = tiles;
+  const int cus = (nsys == 1 && scratch != nullptr) ? device_cus() : 0;
+  if (cus > 0 && tiles > cus) {
+    const int64_t r = tiles % cus, nk = gb.K[0] / G2_BK;
+    int64_t s = r > 0 ? cus / r : 0;
+    if (s > 16) s = 16;
+    if (s > nk) s = nk;
+    if (s >= 2) {
+      const int64_t kchunk = (nk + s - 1) / s;
+      s = (nk + kchunk - 1) / kchunk;  // every chunk non-empty
+      if (r * s * G2_BM * G2_BM <= scratch_elems) {
+        main_tiles = tiles - r;
+        tail = GramTail{scratch, main_tiles, r, kchunk, (int)s};
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(8 * ((main_tiles + 7) / 8)), (unsigned)nsys), dim3(512), lds, st,
+                     gb, ldc, main_tiles, (int)nt, GramTail{nullptr, 0, 0, 0, 1});
   NQS_LAUNCH_CHECK();
+  if (tail.part != nullptr) {
+    hipLaunchKernelGGL(k_syrk_bf16_256, dim3((unsigned)(tail.ntail * tail.nsplit)), dim3(512), lds, st, gb, ldc,
+                       tiles, (int)nt, tail);
+    NQS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gram_tail_sum, dim3((unsigned)tail.ntail, G2_BM), dim3(G2_BM), 0, st, tail, (int)nt, gb.C[0],
+                       ldc);
+    NQS_LAUNCH_CHECK();
+  }
   return NQS_OK;
 }
 
-int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, hipStream_t st) {
+int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int64_t ldc, float* scratch,
+              int64_t scratch_elems, hipStream_t st) {
   NQS_REQUIRE(n % GB_BM == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs 2W to be a multiple of 128");
   NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");
   if (gram_bf16_blocked(n, K)) {  // A in the blocked layout written by k_center
@@ -503,7 +582,7 @@ int gram_bf16(const __bf16* A, int64_t lda, int64_t n, int64_t K, float* C, int6
     gb.A[0] = A;
     gb.K[0] = K;
     gb.C[0] = C;
-    return gram_bf16_batched(gb, 1, n, ldc, st);
+    return gram_bf16_batched(gb, 1, n, ldc, st, scratch, scratch_elems);
   }
   NQS_REQUIRE(K % GB_BK == 0 && lda % 8 == 0, NQS_ERR_UNSUPPORTED, "bf16 Gram needs K % 64 == 0 and lda % 8 == 0");
   NQS_REQUIRE(((uintptr_t)A & 15) == 0, NQS_ERR_ARG, "bf16 Gram operand must be 16-byte aligned");

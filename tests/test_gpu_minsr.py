@@ -116,6 +116,36 @@ def test_low_precision_gram_close_to_fp64(gram, tol, W):
     assert rel < tol, rel
 
 
+def test_bf16_gram_tail_split_matches_unsplit():
+    """W = 4096: the 256-tile bf16 Gram has 528 tiles, so on a 256-CU device its last round holds
+    16 tiles, which the single-system call splits 16 ways along K (partials summed in a fixed
+    order); the batched call of one system keeps whole tiles.  Well-conditioned synthetic O
+    (i.i.d. Gaussian, P = 32768 >> 2W), no refinement: the two directions agree to rounding
+    (1e-4) and both lie within bf16 accuracy (2e-2) of the float64 Gram's."""
+    from nqs.ensemble import minsr_direction_batched
+    W, P = 4096, 32768
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    ldo = nms._ldo(P)
+    data = torch.zeros(2 * W, ldo, dtype=torch.float32, device="cuda")
+    data[:, :P] = torch.randn(2 * W, P, generator=g, device="cuda")
+    O32 = nms.OMatrix(data, W, P)
+    O64 = nms.OMatrix(data.double(), W, P)
+    eps = torch.complex(torch.randn(W, generator=g, dtype=torch.float64, device="cuda"),
+                        torch.randn(W, generator=g, dtype=torch.float64, device="cuda"))
+    lam = 1e-3
+    d_split = nms.minsr_direction(O32, eps, lam, gram="bf16", refine=0)
+    (d_whole,), info = minsr_direction_batched([O32], [eps], [lam], "bf16", 0)
+    d64 = nms.minsr_direction(O64, eps, lam, gram="fp64", refine=0)
+    assert int(info[0]) == 0
+    den = float(torch.max(torch.abs(d64)))
+    rel_split = float(torch.max(torch.abs(d_split - d_whole))) / den
+    rel64 = float(torch.max(torch.abs(d_split - d64))) / den
+    print(f"split vs whole {rel_split:.2e}, bf16 vs fp64 {rel64:.2e}", flush=True)
+    assert rel_split < 1e-4
+    assert rel64 < 2e-2
+
+
 def test_clip_matches_oracle():
     rng = np.random.default_rng(0)
     for W in (7, 256, 1000):
