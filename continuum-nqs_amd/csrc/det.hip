@@ -572,6 +572,106 @@ __device__ __forceinline__ cd bf_at(const T* __restrict__ bfc, int64_t rowlen, i
   return ok ? mk(re, im) : mk(0, 0);
 }
 
+// Log-sum-exp over the determinants, tangents and Laplacian of log psi, Jastrow: one wave,
+// from res[k * rs_ + c] (c = 0 log det, 1..nt tangents, nt + 1 Laplacian term).
+template <int KIND>
+__device__ __forceinline__ void det_head_tail(const JetGeom& j, const DetParams& p, const cd* res, int rs_,
+                                              const double* pos, const int* spin, int64_t cfg, int flip,
+                                              double* __restrict__ out_lp, double* __restrict__ out_tan,
+                                              double* __restrict__ out_lap) {
+  const int N = j.N, K = p.K, nt = j.nt;
+  const int lane = threadIdx.x & 63;
+  // ---- log-sum-exp over determinants, tangents and Laplacian of log psi, Jastrow
+  cd L;
+  double amax = -INFINITY;
+  for (int k = 0; k < K; ++k) {
+    cd z = res[k * rs_];
+    if (K > 1) z = z + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]);
+    amax = z.re > amax ? z.re : amax;
+  }
+  if (K == 1) {
+    L = res[0];
+  } else {
+    if (!isfinite(amax)) amax = 0.0;
+    cd s = mk(0, 0);
+    for (int k = 0; k < K; ++k) s = s + cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - mk(amax, 0));
+    L = clog_(s) + mk(amax, 0);
+  }
+  auto pk_of = [&](int k) -> cd {
+    return K == 1 ? mk(1, 0) : cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - L);
+  };
+  // tangent c = lane + 1 (nt <= 2N <= 32)
+  const int c = lane + 1;
+  const bool has_c = c <= nt;
+  cd gsum = mk(0, 0), lsum = mk(0, 0);
+  if (has_c || KIND == 2)
+    for (int k = 0; k < K; ++k) {
+      const cd pk = pk_of(k);
+      if (has_c) {
+        const cd gk = res[k * rs_ + c];
+        gsum = gsum + pk * gk;
+        if constexpr (KIND == 2) lsum = lsum + pk * (gk * gk);
+      }
+    }
+  cd lap = mk(0, 0);
+  if constexpr (KIND == 2) {
+    cd lk = mk(0, 0);
+    for (int k = 0; k < K; ++k) lk = lk + pk_of(k) * res[k * rs_ + nt + 1];
+    const cd g2 = wave_sum_cd(has_c ? gsum * gsum : mk(0, 0));
+    lap = wave_sum_cd(lsum) + lk - g2;
+  }
+  if (p.use_jastrow) {
+    // lane a (< N) sums its pairs: gradient of J wrt r_a over all partners, J and its
+    // Laplacian over partners b > a
+    const int a = lane;
+    double J = 0, lJ = 0, gxa = 0, gya = 0;
+    if (a < N) {
+      for (int b = 0; b < N; ++b) {
+        if (b == a) continue;
+        const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+        double r, grx, gry, lapr;
+        pair_dist(p, dx, dy, r, grx, gry, lapr);
+        double u, u1, u2;
+        jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
+        gxa += grx * u1;
+        gya += gry * u1;
+        if (b > a) {
+          J += u;
+          lJ += 2.0 * (u2 * (grx * grx + gry * gry) + u1 * lapr);
+        }
+      }
+    }
+    J = wave_sum(J);
+    lJ = wave_sum(lJ);
+    int ec = 0, dc = 0;
+    if (has_c) {
+      if (j.mode == 2) {
+        ec = flip;
+        dc = c - 1;
+      } else {
+        ec = (c - 1) >> 1;
+        dc = (c - 1) & 1;
+      }
+    }
+    const double gxe = __shfl(gxa, ec), gye = __shfl(gya, ec);
+    gsum = gsum + mk(dc == 0 ? gxe : gye, 0);
+    L = L + mk(J, 0);
+    lap = lap + mk(lJ, 0);
+  }
+  if (lane == 0) {
+    out_lp[2 * cfg] = L.re;
+    out_lp[2 * cfg + 1] = L.im;
+    if (KIND == 2 && out_lap) {
+      out_lap[2 * cfg] = lap.re;
+      out_lap[2 * cfg + 1] = lap.im;
+    }
+  }
+  if (has_c) {
+    out_tan[(cfg * nt + (c - 1)) * 2] = gsum.re;
+    out_tan[(cfg * nt + (c - 1)) * 2 + 1] = gsum.im;
+  }
+}
+
 // KIND (compile time, so that each variant keeps only the registers it needs): 0 log psi only,
 // 1 with tangents, 2 with tangents and the Laplacian.
 template <typename T, int KIND>
@@ -727,70 +827,390 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
   }
   __syncthreads();
   if (wave != 0) return;
-  // ---- log-sum-exp over determinants, tangents and Laplacian of log psi, Jastrow
-  cd L;
-  double amax = -INFINITY;
-  for (int k = 0; k < K; ++k) {
-    cd z = res[k * rs_];
-    if (K > 1) z = z + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]);
-    amax = z.re > amax ? z.re : amax;
-  }
-  if (K == 1) {
-    L = res[0];
-  } else {
-    if (!isfinite(amax)) amax = 0.0;
-    cd s = mk(0, 0);
-    for (int k = 0; k < K; ++k) s = s + cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - mk(amax, 0));
-    L = clog_(s) + mk(amax, 0);
-  }
-  auto pk_of = [&](int k) -> cd {
-    return K == 1 ? mk(1, 0) : cexp_(res[k * rs_] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - L);
-  };
-  // tangent c = lane + 1 (nt <= 2N <= 32)
-  const int c = lane + 1;
-  const bool has_c = c <= nt;
-  cd gsum = mk(0, 0), lsum = mk(0, 0);
-  if (has_c || KIND == 2)
-    for (int k = 0; k < K; ++k) {
-      const cd pk = pk_of(k);
-      if (has_c) {
-        const cd gk = res[k * rs_ + c];
-        gsum = gsum + pk * gk;
-        if constexpr (KIND == 2) lsum = lsum + pk * (gk * gk);
+  det_head_tail<KIND>(j, p, res, rs_, pos, spin, cfg, flip, out_lp, out_tan, out_lap);
+}
+
+// ------------------------------------------------------------------ wave-level head for 16 < N <= 32
+// The N <= 16 scheme on 32 x 32 matrices (padded with an identity block): a matrix is 2 x 2
+// blocks of 16 x 16, each in one of the two layouts above, lane = 16 g + l:
+//   row chunk    slot [2 bi + bj][s] holds X[16 bi + l][16 bj + 4 g + s]   (M, LU factors, Mi)
+//   column chunk slot [2 bi + bj][s] holds X[16 bj + 4 g + s][16 bi + l]   (dM)
+// so tr(Mi dM) is a lane-local dot product over the 16 slots, and every 16 x 16 block of
+// Mi dM is eight v_mfma_f64_16x16x4f64 steps (two inner blocks of four).  64 registers per
+// matrix; rows move with cross-lane shuffles as in the N <= 16 head.
+struct Lu32 {
+  cd m[4][4];   // row chunk; after factor(): L below the diagonal (unit), U on and above
+  int orig[2];  // original index of the row now at position 16 bi + l
+  int parity;
+  bool zero_pivot;
+};
+
+// Pivot steps q = 16 BQ .. 16 BQ + 15 (column q in block column BQ, slot q & 3 of lanes
+// 16 ((q >> 2) & 3) + l).  With BQ = 1 rows 0..15 are final and the pivot row lies in block row 1,
+// so every block index is static.
+template <int BQ>
+__device__ __forceinline__ void lu32_half(Lu32& f, int lane) {
+  const int l = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int q4 = 16 * BQ; q4 < 16 * BQ + 16; q4 += 4) {
+    const int gq = (q4 >> 2) & 3;
+#pragma unroll
+    for (int sq = 0; sq < 4; ++sq) {
+      const int q = q4 + sq, lq = q & 15;
+      // column q of this lane's rows l (block row 0) and 16 + l (block row 1)
+      cd c0 = BQ == 0 ? shfl_cd(f.m[0][sq], l + 16 * gq) : mk(0, 0);
+      cd c1 = shfl_cd(f.m[2 + BQ][sq], l + 16 * gq);
+      const double v0 = (BQ == 0 && l >= q) ? fabs(c0.re) + fabs(c0.im) : -1.0;
+      const double v1 = 16 + l >= q ? fabs(c1.re) + fabs(c1.im) : -1.0;
+      double v = v1 > v0 ? v1 : v0;
+      int br = v1 > v0 ? 16 + l : l;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int obr = __shfl_xor(br, off);
+        const bool take = ov > v || (ov == v && obr < br);
+        v = take ? ov : v;
+        br = take ? obr : br;
       }
-    }
-  cd lap = mk(0, 0);
-  if constexpr (KIND == 2) {
-    cd lk = mk(0, 0);
-    for (int k = 0; k < K; ++k) lk = lk + pk_of(k) * res[k * rs_ + nt + 1];
-    const cd g2 = wave_sum_cd(has_c ? gsum * gsum : mk(0, 0));
-    lap = wave_sum_cd(lsum) + lk - g2;
-  }
-  if (p.use_jastrow) {
-    // lane a (< N) sums its pairs: gradient of J wrt r_a over all partners, J and its
-    // Laplacian over partners b > a
-    const int a = lane;
-    double J = 0, lJ = 0, gxa = 0, gya = 0;
-    if (a < N) {
-      for (int b = 0; b < N; ++b) {
-        if (b == a) continue;
-        const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-        double r, grx, gry, lapr;
-        pair_dist(p, dx, dy, r, grx, gry, lapr);
-        double u, u1, u2;
-        jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
-        gxa += grx * u1;
-        gya += gry * u1;
-        if (b > a) {
-          J += u;
-          lJ += 2.0 * (u2 * (grx * grx + gry * gry) + u1 * lapr);
+      const int p = __builtin_amdgcn_readfirstlane(br);
+      if (__builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0)) f.zero_pivot = true;
+      const int bp = BQ == 1 ? 1 : p >> 4, lp = p & 15;
+      // swap rows p and q (branch-free: with p == q every write stores the value it replaces, and
+      // the matrix keeps one set of registers)
+      f.parity ^= p != q ? 1 : 0;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          cd& mq = f.m[2 * BQ + bj][s];
+          const cd xp = shfl_cd(bp ? f.m[2 + bj][s] : f.m[bj][s], lp + 16 * g);
+          const cd xq = shfl_cd(mq, lq + 16 * g);
+          if (l == lq) mq = xp;
+          if (l == lp) {
+            if (bp) f.m[2 + bj][s] = xq;
+            else f.m[bj][s] = xq;
+          }
+          __builtin_amdgcn_sched_barrier(0);  // one slot's shuffles live at a time
         }
+      const int op = __shfl(bp ? f.orig[1] : f.orig[0], lp + 16 * g);
+      const int oq = __shfl(f.orig[BQ], lq + 16 * g);
+      if (l == lq) f.orig[BQ] = op;
+      if (l == lp) {
+        if (bp) f.orig[1] = oq;
+        else f.orig[0] = oq;
+      }
+      const cd xpq = shfl_cd(bp ? c1 : c0, lp), xqq = shfl_cd(BQ ? c1 : c0, lq);
+      if (l == lq) {
+        if (BQ) c1 = xpq;
+        else c0 = xpq;
+      }
+      if (l == lp) {
+        if (bp) c1 = xqq;
+        else c0 = xqq;
+      }
+      const cd pv = xpq;
+      const bool zero = pv.re == 0.0 && pv.im == 0.0;
+      const cd l0 = (BQ == 0 && l > q && !zero) ? cdiv_(c0, pv) : mk(0, 0);
+      const cd l1 = (16 + l > q && !zero) ? cdiv_(c1, pv) : mk(0, 0);
+      // eliminate below row q (columns of block column 0 are final once q >= 16)
+#pragma unroll
+      for (int bj = BQ; bj < 2; ++bj)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const cd rq = shfl_cd(f.m[2 * BQ + bj][s], lq + 16 * g);
+          const int col = 16 * bj + 4 * g + s;
+          if (BQ == 0 && l > q) {
+            if (col == q) f.m[bj][s] = l0;
+            else if (col > q) f.m[bj][s] = f.m[bj][s] - l0 * rq;
+          }
+          if (16 + l > q) {
+            if (col == q) f.m[2 + bj][s] = l1;
+            else if (col > q) f.m[2 + bj][s] = f.m[2 + bj][s] - l1 * rq;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+  }
+}
+
+__device__ __forceinline__ void lu32_factor(Lu32& f, int lane) {
+  const int l = lane & 15;
+  f.orig[0] = l;
+  f.orig[1] = 16 + l;
+  f.parity = 0;
+  f.zero_pivot = false;
+  lu32_half<0>(f, lane);
+  lu32_half<1>(f, lane);
+}
+
+// log det from the factors (principal branch), as lu_logdet for the first N pivots
+__device__ __forceinline__ cd lu32_logdet(const Lu32& f, int lane, int N) {
+  const int l = lane & 15, g = lane >> 4;
+  const bool on = (l >> 2) == g;  // this lane holds U[l][l] (block 0) and U[16 + l][16 + l] (block 3)
+  const cd u0 = pick4(f.m[0], l & 3), u1 = pick4(f.m[3], l & 3);
+  const bool m0 = on && l < N, m1 = on && 16 + l < N;
+  const double a0 = m0 ? cabs_(u0) : 1.0, a1 = m1 ? cabs_(u1) : 1.0;
+  double la = wave_sum(log(a0) + log(a1));
+  cd ph0 = m0 && a0 > 0 ? mk(u0.re / a0, u0.im / a0) : mk(1, 0);
+  cd ph1 = m1 && a1 > 0 ? mk(u1.re / a1, u1.im / a1) : mk(1, 0);
+  cd ph = ph0 * ph1;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) ph = ph * shfl_cd(ph, lane ^ off);
+  if (f.parity) ph = mk(-ph.re, -ph.im);
+  return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
+}
+
+// Mi = U^-1 L^-1 P in the row-chunk layout.  Forward: rows below q minus L[r][q] row q;
+// backward: row q divided by U[q][q], rows above q minus U[r][q] row q.
+template <int BQ>
+__device__ __forceinline__ void lu32_fwd_half(const Lu32& f, int lane, cd (&y)[4][4]) {
+  const int l = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int q4 = 16 * BQ; q4 < 16 * BQ + 16; q4 += 4) {
+    const int gq = (q4 >> 2) & 3;
+    // a loop (not unrolled): the compiler would otherwise sink the updates of rows that no later
+    // step reads to the end of the group, holding every step's broadcast row in registers
+#pragma unroll 1
+    for (int sq = 0; sq < 4; ++sq) {
+      const int q = q4 + sq, lq = q & 15;
+      const cd c0 = BQ == 0 ? shfl_cd(pick4(f.m[0], sq), l + 16 * gq) : mk(0, 0);
+      const cd c1 = shfl_cd(pick4(f.m[2 + BQ], sq), l + 16 * gq);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const cd yq = shfl_cd(y[2 * BQ + bj][s], lq + 16 * g);
+          if (BQ == 0 && l > q) y[bj][s] = y[bj][s] - c0 * yq;
+          if (16 + l > q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
+          __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+  }
+}
+
+template <int BQ>
+__device__ __forceinline__ void lu32_bwd_half(const Lu32& f, int lane, cd (&y)[4][4]) {
+  const int l = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int q4 = 16 * BQ + 12; q4 >= 16 * BQ; q4 -= 4) {
+    const int gq = (q4 >> 2) & 3;
+#pragma unroll 1
+    for (int sq = 3; sq >= 0; --sq) {
+      const int q = q4 + sq, lq = q & 15;
+      const cd ru = cdiv_(mk(1, 0), shfl_cd(pick4(f.m[3 * BQ], sq), lq + 16 * gq));  // 1 / U[q][q]
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          cd& yy = y[2 * BQ + bj][s];
+          const cd dv = yy * ru;
+          if (l == lq) yy = dv;
+        }
+      const cd c0 = shfl_cd(pick4(f.m[BQ], sq), l + 16 * gq);
+      const cd c1 = BQ == 1 ? shfl_cd(pick4(f.m[3], sq), l + 16 * gq) : mk(0, 0);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const cd yq = shfl_cd(y[2 * BQ + bj][s], lq + 16 * g);
+          if (l < q) y[bj][s] = y[bj][s] - c0 * yq;
+          if (BQ == 1 && 16 + l < q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
+          __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+  }
+}
+
+__device__ __forceinline__ void lu32_inverse(const Lu32& f, int lane, cd (&y)[4][4]) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) y[2 * bi + bj][s] = mk(16 * bj + 4 * g + s == f.orig[bi] ? 1.0 : 0.0, 0);
+  lu32_fwd_half<0>(f, lane, y);
+  lu32_fwd_half<1>(f, lane, y);
+  lu32_bwd_half<1>(f, lane, y);
+  lu32_bwd_half<0>(f, lane, y);
+}
+
+// sum over the wave of tr(X Y) for X = A_IJ, Y = the accumulators of (A_JI)^T (f64 MFMA C
+// layouts of the same 16 x 16 block): one complex product per accumulator slot
+__device__ __forceinline__ cd acc_dot(const d4v& are, const d4v& aim, const d4v& tre, const d4v& tim) {
+  cd h = mk(0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) h = h + mk(are[r], aim[r]) * mk(tre[r], tim[r]);
+  return h;
+}
+
+// block (I, J) of A = Mi dM and of A^T's block (I, J) = (A_JI)^T: Mi row chunk (A operand of
+// the MFMA, k index 16 B + 4 g + s), dM column chunk (B operand); A^T from the same registers
+// with the operands swapped
+__device__ __forceinline__ void mfma_block_pair(const cd (&mi)[4][4], const cd (&dm)[4][4], int I, int J, d4v& are,
+                                                d4v& aim, d4v& tre, d4v& tim) {
+  are = (d4v)0.0;
+  aim = (d4v)0.0;
+  tre = (d4v)0.0;
+  tim = (d4v)0.0;
+#pragma unroll
+  for (int B = 0; B < 2; ++B)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cd x = mi[2 * I + B][s], y = dm[2 * J + B][s];  // A_IJ += Mi_IB dM_BJ
+      are = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, y.re, are, 0, 0, 0);
+      are = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.im, y.im, are, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, y.im, aim, 0, 0, 0);
+      aim = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, y.re, aim, 0, 0, 0);
+      const cd u = dm[2 * I + B][s], v = mi[2 * J + B][s];  // (A_JI)^T += (dM_BI)^T (Mi_JB)^T
+      tre = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, v.re, tre, 0, 0, 0);
+      tre = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.im, v.im, tre, 0, 0, 0);
+      tim = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, v.im, tim, 0, 0, 0);
+      tim = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, v.re, tim, 0, 0, 0);
+    }
+}
+
+// One workgroup per configuration (1, 2 or 4 waves); envelope jet built once in LDS, then each
+// wave takes determinants k = wave, wave + nwaves, ... in registers.  KIND as k_det_jet16.
+// x, opaque to the optimiser: per-slot addresses built from it inside a loop stay inside the
+// loop instead of being hoisted (and held in registers) across the determinant's whole life
+__device__ __forceinline__ int opaque_i(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 2 : 1)))
+void k_det_jet32(JetGeom j, DetParams p, const double* __restrict__ R,
+                                                   const int32_t* __restrict__ S, int64_t w0,
+                                                   const T* __restrict__ bf, double* __restrict__ out_lp,
+                                                   double* __restrict__ out_tan, double* __restrict__ out_lap) {
+  // LDS: env [32 x 32] (entry (a, i) at 32 a + i), then dex, dey (KIND >= 1), lenv (KIND 2); then
+  // a region holding the phase table exp(i G_g . r_i) [g][i] while the jet is built and res
+  // [K][nt + 2] (log det, tangents, Laplacian term per determinant) afterwards
+  extern __shared__ cd sh32[];
+  constexpr int NJ = KIND == 0 ? 1 : (KIND == 1 ? 3 : 4);
+  cd* env = sh32;
+  cd* dex = sh32 + 1024;
+  cd* dey = sh32 + 2048;
+  cd* lenv = sh32 + 3072;
+  cd* tail = sh32 + NJ * 1024;
+  __shared__ double pos[64];
+  __shared__ int spin[32];
+  const int N = j.N, K = p.K, nt = j.nt;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  const int64_t cfg = blockIdx.x;
+  int64_t w;
+  int flip = -1;
+  if (j.mode == 2) {
+    w = w0 + cfg / N;
+    flip = (int)(cfg % N);
+  } else {
+    w = w0 + cfg;
+  }
+  if (tid < N) {
+    pos[2 * tid] = R[(w * N + tid) * 2];
+    pos[2 * tid + 1] = R[(w * N + tid) * 2 + 1];
+    const int s = S[w * N + tid];
+    spin[tid] = (tid == flip) ? 1 - s : s;
+  }
+  __syncthreads();
+  const bool tab = p.n_g <= kPhaseMax;
+  if (tab) {
+    for (int e = tid; e < p.n_g * 32; e += blockDim.x) {
+      const int gg = e >> 5, i = e & 31;
+      double sn = 0, cs = 0;
+      if (i < N) sincos(p.gs[2 * gg] * pos[2 * i] + p.gs[2 * gg + 1] * pos[2 * i + 1], &sn, &cs);
+      tail[e] = mk(cs, sn);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < 1024; e += blockDim.x) {  // envelope jet, entry (a, i) = (e / 32, e % 32)
+    const int a = e >> 5, i = e & 31;
+    cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0), lp = mk(0, 0);
+    if (a < N && i < N) {
+      for (int gg = 0; gg < p.n_g; ++gg) {
+        const double Gx = p.gs[2 * gg], Gy = p.gs[2 * gg + 1];
+        double sn, cs;
+        if (tab) {
+          cs = tail[gg * 32 + i].re;
+          sn = tail[gg * 32 + i].im;
+        } else {
+          sincos(Gx * pos[2 * i] + Gy * pos[2 * i + 1], &sn, &cs);
+        }
+        const double* wp = p.wenv + ((size_t)(a * p.n_g + gg) * 2 + spin[i]) * 2;
+        const cd t = mk(wp[0], wp[1]) * mk(cs, sn);
+        v = v + t;
+        if constexpr (KIND > 0) {
+          gx = gx + t * mk(0, Gx);
+          gy = gy + t * mk(0, Gy);
+        }
+        if constexpr (KIND == 2) lp = lp + (-(Gx * Gx + Gy * Gy)) * t;
       }
     }
-    J = wave_sum(J);
-    lJ = wave_sum(lJ);
-    int ec = 0, dc = 0;
-    if (has_c) {
+    env[e] = v;
+    if constexpr (KIND > 0) {
+      dex[e] = gx;
+      dey[e] = gy;
+    }
+    if constexpr (KIND == 2) lenv[e] = lp;
+  }
+  __syncthreads();  // the phase table is dead from here; tail holds res
+  cd* res = tail;
+  const int C = j.C;
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
+  const T* bfc0 = bf + cfg * C * chs;
+  const int l = lane & 15, g = lane >> 4;
+  const int rs_ = nt + 2;
+  for (int k = wave; k < K; k += nwaves) {
+    Lu32 f;
+    {
+      const int l = opaque_i(lane & 15), g = opaque_i(lane >> 4);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int a = 16 * bi + l, i = 16 * bj + 4 * g + s;
+            const cd b = bf_at(bfc0, rowlen, KN, N, k, a, i);
+            f.m[2 * bi + bj][s] = (a < N && i < N) ? env[a * 32 + i] * (mk(1, 0) + b) : mk(a == i ? 1.0 : 0.0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+    }
+    lu32_factor(f, lane);
+    const cd ld = lu32_logdet(f, lane, N);
+    if (lane == 0) res[k * rs_] = ld;
+    if constexpr (KIND == 0) continue;
+    cd mi[4][4];
+    lu32_inverse(f, lane, mi);
+    cd lap_tr = mk(0, 0);
+    if constexpr (KIND == 2) {  // tr(Mi Lap M) first, while only Mi is held in registers
+      const int cl = nt + 1;
+      const T* bfl = bfc0 + cl * chs;
+      const int l = opaque_i(lane & 15), g = opaque_i(lane >> 4);
+      cd tr = mk(0, 0);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int a = 16 * bj + 4 * g + s, i = 16 * bi + l, e_ = a * 32 + i;
+            const cd ob = mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, a, i);
+            const cd v = lenv[e_] * ob + env[e_] * bf_at(bfl, rowlen, KN, N, k, a, i);
+            tr = tr + mi[2 * bi + bj][s] * v;
+            __builtin_amdgcn_sched_barrier(0);
+          }
+      tr = wave_sum_cd(tr);
+      lap_tr = tr;  // plus the cross terms and minus sum_c tr((Mi dM_c)^2), below
+    }
+    // Lap M = Lap env (1 + bf) + env Lap bf + 2 sum_c denv_c dbf_c: the cross term's column i is
+    // tangent c = 1 + 2 i + d's own column, so it is summed in the tangent loop
+    cd lapx = mk(0, 0);
+    cd hsum = mk(0, 0);
+    for (int c = 1; c <= nt; ++c) {
+      int ec, dc;
       if (j.mode == 2) {
         ec = flip;
         dc = c - 1;
@@ -798,24 +1218,51 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
         ec = (c - 1) >> 1;
         dc = (c - 1) & 1;
       }
+      const T* bfc = bfc0 + c * chs;
+      const cd* den = dc == 0 ? dex : dey;
+      const int l = opaque_i(lane & 15), g = opaque_i(lane >> 4);
+      cd dm[4][4];
+      cd tr = mk(0, 0);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int a = 16 * bj + 4 * g + s, i = 16 * bi + l, e_ = a * 32 + i;
+            const cd bv = bf_at(bfc, rowlen, KN, N, k, a, i);
+            cd v = env[e_] * bv;
+            if (i == ec && a < N) {
+              const cd de = den[e_];
+              v = v + de * (mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, a, i));
+              if constexpr (KIND == 2) lapx = lapx + mi[2 * bi + bj][s] * (2.0 * (de * bv));
+            }
+            dm[2 * bi + bj][s] = v;
+            tr = tr + mi[2 * bi + bj][s] * v;
+            __builtin_amdgcn_sched_barrier(0);
+          }
+      tr = wave_sum_cd(tr);
+      if (lane == 0) res[k * rs_ + c] = tr;
+      if constexpr (KIND == 2) {
+        // tr(A^2) = tr(A_00^2) + tr(A_11^2) + 2 tr(A_01 A_10)
+        d4v are, aim, tre, tim;
+        mfma_block_pair(mi, dm, 0, 0, are, aim, tre, tim);
+        cd h = acc_dot(are, aim, tre, tim);
+        mfma_block_pair(mi, dm, 1, 1, are, aim, tre, tim);
+        h = h + acc_dot(are, aim, tre, tim);
+        mfma_block_pair(mi, dm, 0, 1, are, aim, tre, tim);
+        h = h + 2.0 * acc_dot(are, aim, tre, tim);
+        hsum = hsum + wave_sum_cd(h);
+      }
     }
-    const double gxe = __shfl(gxa, ec), gye = __shfl(gya, ec);
-    gsum = gsum + mk(dc == 0 ? gxe : gye, 0);
-    L = L + mk(J, 0);
-    lap = lap + mk(lJ, 0);
-  }
-  if (lane == 0) {
-    out_lp[2 * cfg] = L.re;
-    out_lp[2 * cfg + 1] = L.im;
-    if (KIND == 2 && out_lap) {
-      out_lap[2 * cfg] = lap.re;
-      out_lap[2 * cfg + 1] = lap.im;
+    if constexpr (KIND == 2) {
+      lap_tr = lap_tr + wave_sum_cd(lapx);
+      if (lane == 0) res[k * rs_ + nt + 1] = lap_tr - hsum;
     }
   }
-  if (has_c) {
-    out_tan[(cfg * nt + (c - 1)) * 2] = gsum.re;
-    out_tan[(cfg * nt + (c - 1)) * 2 + 1] = gsum.im;
-  }
+  __syncthreads();
+  if (wave != 0) return;
+  det_head_tail<KIND>(j, p, res, rs_, pos, spin, cfg, flip, out_lp, out_tan, out_lap);
 }
 
 // Note on rint vs jnp.round: both round half to even; minimum image is exact either way.
@@ -845,6 +1292,37 @@ int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const 
     }, st);
   }
   NQS_REQUIRE(NN <= 4 * 256, NQS_ERR_UNSUPPORTED, "determinant head supports N <= 32");
+  if (j.nt <= 64) {
+    const int kind = j.has_lap ? 2 : (j.nt > 0 ? 1 : 0);
+    const int nj = kind == 0 ? 1 : (kind == 1 ? 3 : 4);
+    const int64_t tab = p.n_g <= kPhaseMax ? (int64_t)p.n_g * 32 : 0;
+    const int64_t rs = (int64_t)p.K * (j.nt + 2);
+    const size_t lds = sizeof(cd) * (size_t)(nj * 1024 + (tab > rs ? tab : rs));
+    if (lds + 1024 <= 160 * 1024) {  // + the static positions and spins
+      static size_t lds_set[3] = {0, 0, 0};
+      if (lds > 64 * 1024 && lds > lds_set[kind]) {
+        const void* fn = kind == 0 ? (const void*)k_det_jet32<T, 0>
+                                   : (kind == 1 ? (const void*)k_det_jet32<T, 1> : (const void*)k_det_jet32<T, 2>);
+        NQS_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set[kind] = lds;
+      }
+      // determinants in flight per configuration (waves per workgroup) chosen by the autotuner
+      static const char* const names[3] = {"1wave", "2waves", "4waves"};
+      const TuneKey key{TUNE_DET32, (int64_t)sizeof(T) * 1000 + N * 64 + p.K,
+                        (int64_t)j.nt * 4 + j.has_lap * 2 + (j.mode == 2), tune_bucket(j.n_cfg)};
+      return tune_run(key, 3, 2, names, [&](int c) -> int {
+        const dim3 grid((unsigned)j.n_cfg), block(64 << c);
+        if (kind == 0)
+          hipLaunchKernelGGL((k_det_jet32<T, 0>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
+        else if (kind == 1)
+          hipLaunchKernelGGL((k_det_jet32<T, 1>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
+        else
+          hipLaunchKernelGGL((k_det_jet32<T, 2>), grid, block, lds, st, j, p, R, S, w0, bf, out_lp, out_tan, out_lap);
+        NQS_LAUNCH_CHECK();
+        return NQS_OK;
+      }, st);
+    }
+  }
   size_t lds = sizeof(cd) * (size_t)(7 * NN + (size_t)p.K * (j.nt + 2)) + sizeof(double) * 2 * N +
                sizeof(int) * (2 * N + 4) + sizeof(double) * (size_t)(8 + 2 + 2 * (j.nt + 1) + (j.nt + 1) + 8);
   lds = round_up((int64_t)lds, 16);

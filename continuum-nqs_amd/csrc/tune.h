@@ -23,7 +23,7 @@ struct TuneKey {
   }
 };
 
-enum { TUNE_ATTENTION = 1, TUNE_DET16 = 2 };
+enum { TUNE_ATTENTION = 1, TUNE_DET16 = 2, TUNE_DET32 = 3 };
 
 // Launch candidate `choice` (0..n-1) and return NQS_OK or an error code.
 int tune_run(const TuneKey& key, int n_candidates, int default_choice, const char* const* names,
