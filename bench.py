@@ -130,6 +130,115 @@ def accuracy_vs_fp64(vs, sysd, coul, gemm):
     return out
 
 
+C4 = dict(N=32, rs=5.0, lam=0.5, hz=0.0, arch=dict(embed_dim=128, depth=4, n_heads=8, n_freq=8, n_dets=16))
+
+
+def bench_c4(args):
+    """The large configuration: N = 32, rs = 5, lambda = 0.5, h_z = 0, d = 128, 4 layers, 8 heads,
+    K = 16 (P = 666,784), 16384 walkers, Ewald 5 x 5 real-space images and |m| <= 8, two Gaussian
+    z-field pins (0.05 Ha, width rs), float32 MFMA Gram; mixed-precision backbone."""
+    from nqs.rashba_reference import noninteracting_gs, occupied_orbital_matrix, planewave_envelope
+    from nqs.psiformer import ContinuumSpinorPsiformer, ravel_params
+    from nqs.ewald import make_ewald
+    from nqs.pinning import make_gaussian_pins
+    from nqs.minsr import VMCStep
+    from nqs import _lib
+    N, rs = C4["N"], C4["rs"]
+    W = args.walkers if args.walkers != 8192 else 16384
+    L = rs * math.sqrt(math.pi * N)
+    g = noninteracting_gs(N, L, 1.0, C4["lam"], C4["hz"], n_cut=10, twist=(0.1, 0.17))
+    Gs, Wenv = planewave_envelope(*occupied_orbital_matrix(g))
+    model = ContinuumSpinorPsiformer(n_elec=N, Gs=Gs, W=Wenv, L=L, precision="mixed", gemm=args.gemm, **C4["arch"])
+    params = model.init(0)
+    rng = np.random.default_rng(1)
+    kr = params["params"]["backflow"]["kernel"]
+    z = torch.tensor(rng.standard_normal(tuple(kr.shape)) * 0.05)
+    params["params"]["backflow"]["kernel"] = (z + 1j * z).to(torch.complex128)
+    theta = ravel_params(params)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(2)
+    R = torch.rand(W, N, 2, generator=gen, dtype=torch.float64, device="cuda") * L
+    S = torch.randint(0, 2, (W, N), generator=gen, device="cuda").to(torch.int32)
+    pins = make_gaussian_pins([(0.25 * L, 0.3 * L), (0.7 * L, 0.65 * L)], 0.05, rs, kind="spin")
+    _, _, coul = make_ewald(L, 1.0, n_real=2, n_recip=8)
+    vs = VMCStep(model, theta, R, S, m_star=1.0, lambda_r=C4["lam"], h_z=C4["hz"], coulomb=coul, lr=0.05, lam=1e-4,
+                 n_sweeps=args.sweeps, step_size=0.3 * rs, clip_k=5.0, gram="fp32", refine=1, pinning=pins)
+    if args.burn:
+        vs.chain.run(args.burn, 0.3 * rs, 2, key=7)
+    lib = _lib.load()
+    for k in range(args.warmup):
+        vs.step(key=100 + k)
+    torch.cuda.synchronize()
+    lib.nqs_minsr_profile(1, None)
+    vs.timers = []
+    gram_ms = []
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for k in range(args.steps):
+        vs.step(key=1000 + k)
+        ms5 = (ctypes.c_float * 5)()
+        _lib.check(lib.nqs_minsr_profile(-1, ms5), "nqs_minsr_profile")
+        gram_ms.append(list(ms5))
+    t1.record()
+    t1.synchronize()
+    lib.nqs_minsr_profile(0, None)
+    ms = t0.elapsed_time(t1) / args.steps
+    stages = np.array([[a.elapsed_time(b) for a, b in zip(t[:-1], t[1:])] for t in vs.timers]).mean(0)
+    mins = np.array(gram_ms).mean(0)
+    P = model.n_params()
+    n = 2 * W
+    achieved = float(n) * (n + 1) * P / (mins[1] / 1e3)
+    stats = vs.last["stats"].cpu().numpy()
+    return {"metric": "walker-steps/s incl. MinSR at N=32 (large configuration)", "value": W / (ms / 1000.0),
+            "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16+f64",
+            "data": "synthetic",
+            "config": {"workload": f"N=32 rs=5 lambda=0.5 h_z=0 d=128 layers=4 heads=8 K=16 F=8 walkers={W} "
+                                   "Ewald 5x5 images |m|<=8, two Gaussian z pins 0.05 Ha width rs",
+                       "walkers": W, "params": P, "sweeps": args.sweeps, "gram": "fp32", "gemm": args.gemm},
+            "roofline": {"bound": "mfma", "kernel": "Gram (k_gemm_nt<float>, lower tiles)",
+                         "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK["fp32"], "traffic": None, "gram_ms": float(mins[1])},
+            "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
+                         "minsr": float(stages[3]), "update_repack": float(stages[4]),
+                         "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
+                         "minsr_cholesky": float(mins[2]), "minsr_solve_refine": float(mins[3]),
+                         "minsr_pushthrough": float(mins[4])},
+            "E_mean": float(stats[0]), "E_err": float(stats[1]), "cpu_baseline": None}
+
+
+def bench_c5(args):
+    """The batched uneven ensemble: 24 systems, N from {4, 6, 8, 12, 16, 24, 32}, rs in [1, 6],
+    lambda in [0, 1], h_z in [0, 0.2], d = 64, 3 layers, 4 heads, K = 4, 1024 walkers each."""
+    from nqs.ensemble import Ensemble, make_c5_systems
+    W = args.walkers if args.walkers != 8192 else 1024
+    systems = make_c5_systems(24, seed=0)
+    ens = Ensemble(systems, n_walkers=W, n_sweeps=args.sweeps, graph=True)
+    if args.burn:
+        ens.burn(args.burn)
+    for k in range(args.warmup):
+        ens.step(key=100 + k)
+    torch.cuda.synchronize()
+    timers = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ens.step(key=1000 + k, timers=timers)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    st = np.array(timers).mean(0)
+    return {"metric": "walker-steps/s incl. MinSR, 24-system uneven ensemble", "value": 24 * W / (ms / 1000.0),
+            "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16+f64",
+            "data": "synthetic",
+            "config": {"workload": f"24 systems, N in {{4..32}}, d=64 layers=3 heads=4 K=4, {W} walkers each",
+                       "systems": [[s["N"], round(s["rs"], 3)] for s in systems], "walkers": W,
+                       "sweeps": args.sweeps},
+            "stage_ms": {"sweeps": float(st[0]), "eloc_clip": float(st[1]), "grad": float(st[2]),
+                         "minsr_batched": float(st[3]), "update": float(st[4])},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +254,9 @@ def main():
                     help="dense-layer MFMA mode of the mixed backbone")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps from a captured HIP graph")
+    ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5"],
+                    help="c3: the headline N=16 workload (default); c4: the N=32 large configuration; c5: the "
+                         "24-system uneven ensemble (each prints its own line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the calibration kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -159,6 +271,12 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 << 28, "copies": 5}))
+        return
+    if args.config == "c4":
+        print(json.dumps(bench_c4(args)))
+        return
+    if args.config == "c5":
+        print(json.dumps(bench_c5(args)))
         return
     from nqs import _lib
     from nqs.ewald import make_ewald

@@ -1080,7 +1080,7 @@ __device__ __forceinline__ int opaque_i(int x) {
 }
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 2 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 2 ? 1 : 2)))
 void k_det_jet32(JetGeom j, DetParams p, const double* __restrict__ R,
                                                    const int32_t* __restrict__ S, int64_t w0,
                                                    const T* __restrict__ bf, double* __restrict__ out_lp,
