@@ -6,7 +6,9 @@ envelope, parameters and walkers; all share one architecture.  A step advances e
 
   sweeps   one HIP graph replay with every system's Metropolis sweeps (nqs.graphs.EnsembleSweeps,
            side-stream branches so that small systems overlap);
-  E_loc    per system (forward-Laplacian kinetic + SOC + Zeeman, Ewald), median/MAD clip, O rows;
+  E_loc    per system (forward-Laplacian kinetic + SOC + Zeeman, Ewald), median/MAD clip, O rows,
+           spread over the same number of side streams (each system with its own workspaces) so that
+           small systems run beside large ones;
   MinSR    one nqs_minsr_direction_batched call: the bf16 Grams of all systems in one launch, the
            Cholesky factorizations and triangular solves of all systems sharing each launch;
   update   theta -= lr dtheta per system, repacked in place so the captured graph stays valid.
@@ -97,6 +99,8 @@ class System:
         _, _, self.coulomb = make_ewald(self.L, 1.0, n_real=n_real, n_recip=n_recip)
         self.E = make_local_energy(self.lp, 1.0, self.lambda_r, self.h_z)
         self.O = None
+        self.ws_tag = f"ensemble{seed}"  # own workspaces: systems' E_loc and O run concurrently
+        self.E.ws_tag = self.ws_tag + "_eloc"
 
 
 class Ensemble:
@@ -104,7 +108,7 @@ class Ensemble:
 
     def __init__(self, systems, arch=None, n_walkers=1024, precision="mixed", gemm="bf16x3", lr=0.05, lam=1e-4,
                  n_sweeps=10, clip_k=5.0, n_spin_flips=2, gram="auto", refine=1, graph=True, seed=0, n_real=4,
-                 n_recip=8, branches=4):
+                 n_recip=8, branches=4, concurrent=True):
         arch = dict(C5_ARCH if arch is None else arch)
         self.systems = [System(s["N"], s["rs"], s["lambda_r"], s["h_z"], arch, n_walkers, precision, gemm,
                                seed + 17 * i, n_real, n_recip) for i, s in enumerate(systems)]
@@ -113,6 +117,8 @@ class Ensemble:
         self.gram, self.refine = gram, refine
         self.graph = graph
         self.branches = branches
+        self.concurrent = concurrent
+        self._side = None
         self._sweeps = None
         self.failed = np.zeros(len(self.systems), int)
 
@@ -134,7 +140,7 @@ class Ensemble:
 
     def step(self, key=0, timers=None):
         """One VMC step of every system; returns (n_sys, 4) statistics [mean, err, median, MAD].
-        timers: a list that receives the stage times in ms (sweeps, E_loc + clip, O, MinSR, update)."""
+        timers: a list that receives a dict of stage times in ms (sweeps, eloc_clip_grad, minsr, update)."""
         ev = []
 
         def mark():
@@ -145,18 +151,36 @@ class Ensemble:
         mark()
         self.sweep(key)
         mark()
-        eps, stats = [], []
-        for s in self.systems:
-            e = s.E(s.chain.R, s.chain.S) + s.coulomb(s.chain.R)
-            ec, st = clip_energies(e, self.clip_k)
-            eps.append(ec)
-            stats.append(st)
+        n = len(self.systems)
+        eps, stats = [None] * n, [None] * n
+        main = torch.cuda.current_stream()
+        if self.concurrent:
+            nb = max(1, min(int(self.branches), n))
+            if self._side is None:
+                self._side = [torch.cuda.Stream() for _ in range(nb)]  # default priority
+                # largest first onto the least loaded stream, cost ~ N^3 (determinants and jets)
+                load = [0.0] * nb
+                self._branch = [0] * n
+                for i in sorted(range(n), key=lambda i: -self.systems[i].N):
+                    b = min(range(nb), key=lambda b: load[b])
+                    self._branch[i] = b
+                    load[b] += float(self.systems[i].N) ** 3
+            for st in self._side:
+                st.wait_stream(main)
+        for i, s in enumerate(self.systems):
+            st = self._side[self._branch[i]] if self.concurrent else main
+            with torch.cuda.stream(st):
+                e = s.E(s.chain.R, s.chain.S) + s.coulomb(s.chain.R)
+                eps[i], stats[i] = clip_energies(e, self.clip_k)
+                s.O, _ = grad_params(s.lp, s.chain.R, s.chain.S, s.O, ws_tag=s.ws_tag + "_grad")
+            if st is not main:
+                eps[i].record_stream(main)
+                stats[i].record_stream(main)
+        if self.concurrent:
+            for st in self._side:
+                main.wait_stream(st)
         mark()
-        for s in self.systems:
-            s.O, _ = grad_params(s.lp, s.chain.R, s.chain.S, s.O)
-        mark()
-        dth, info = minsr_direction_batched([s.O for s in self.systems], eps, [self.lam] * len(self.systems),
-                                            self.gram, self.refine)
+        dth, info = minsr_direction_batched([s.O for s in self.systems], eps, [self.lam] * n, self.gram, self.refine)
         mark()
         ok = (info == 0).cpu().numpy()
         for i, (s, d) in enumerate(zip(self.systems, dth)):
@@ -170,5 +194,6 @@ class Ensemble:
         mark()
         if timers is not None:
             torch.cuda.synchronize()
-            timers.append([a.elapsed_time(b) for a, b in zip(ev[:-1], ev[1:])])
+            ms = [a.elapsed_time(b) for a, b in zip(ev[:-1], ev[1:])]
+            timers.append(dict(zip(("sweeps", "eloc_clip_grad", "minsr", "update"), ms)))
         return torch.stack(stats).cpu().numpy()

@@ -74,6 +74,8 @@ class LocalEnergy:
         # Gaussian pinning centres (nqs.pinning.GaussianPin): no counterpart in the reference
         self.pinning = None if pinning is None else pin_table(pinning)
         self._pin = None
+        # device workspace name: calls that may run concurrently (on different streams) need their own
+        self.ws_tag = "default"
 
     def _ham(self):
         h = _lib.HamDesc()
@@ -112,7 +114,7 @@ class LocalEnergy:
         g = torch.empty(W, 2 * N, 2, dtype=torch.float64, device=R.device)
         lap = torch.empty(W, 2, dtype=torch.float64, device=R.device)
         nb = lib.nqs_workspace_bytes(ctypes.byref(d), _lib.OP_ELOC, W)
-        ws = _lib.workspace(nb)
+        ws = _lib.workspace(nb, self.ws_tag)
         _lib.check(lib.nqs_eloc(ctypes.byref(d), ctypes.byref(h), _lib.ptr(self.logpsi.packed), _lib.ptr(R),
                                 _lib.ptr(S), W, _lib.ptr(e), _lib.ptr(lp), _lib.ptr(g), _lib.ptr(lap), _lib.ptr(ws),
                                 ws.numel(), _lib.stream()), "nqs_eloc")

@@ -71,8 +71,9 @@ class OMatrix:
         return torch.complex(d[:self.W], d[self.W:])
 
 
-def grad_params(logpsi, R, S, out=None):
-    """O rows of the walkers (R, S) by the backward kernel; returns (OMatrix, log psi)."""
+def grad_params(logpsi, R, S, out=None, ws_tag="default"):
+    """O rows of the walkers (R, S) by the backward kernel; returns (OMatrix, log psi).  ws_tag names
+    the device workspace (calls running concurrently on different streams need their own)."""
     lib = _lib.load()
     R, S, _ = as_walkers(R, S)
     W = R.shape[0]
@@ -84,7 +85,7 @@ def grad_params(logpsi, R, S, out=None):
         out = OMatrix(torch.zeros(2 * W, _ldo(P), dtype=dt, device=R.device), W, P)
     lp = torch.empty(W, 2, dtype=torch.float64, device=R.device)
     nb = lib.nqs_workspace_bytes(ctypes.byref(d), _lib.OP_GRAD, W)
-    ws = _lib.workspace(nb)
+    ws = _lib.workspace(nb, ws_tag)
     _lib.check(lib.nqs_psiformer_grad(ctypes.byref(d), _lib.ptr(logpsi.packed), _lib.ptr(R), _lib.ptr(S), W,
                                       _lib.ptr(out.data), out.ldo, _lib.ptr(lp), _lib.ptr(ws), ws.numel(),
                                       _lib.stream()), "nqs_psiformer_grad")

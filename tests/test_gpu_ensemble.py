@@ -75,5 +75,5 @@ def test_c5_ensemble_step():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     print(f"c5 step: {1e3 * dt:.1f} ms, {24 * 1024 / dt:.0f} walker-steps/s over 24 systems; stage ms "
-          f"(sweeps, E_loc, O, MinSR, update) {np.round(tm[0], 1).tolist()}", flush=True)
+          f"{ {k: round(v, 1) for k, v in tm[0].items()} }", flush=True)
     assert np.all(np.isfinite(st))
