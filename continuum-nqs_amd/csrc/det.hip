@@ -452,6 +452,13 @@ constexpr int kPhaseMax = 64;
 // LU with partial pivoting (largest |re| + |im|, first index on ties, as the block kernel
 // above) moves rows with cross-lane shuffles; `orig` follows the original index of the row at
 // each position, so P I is known without replaying the swaps.
+// x, opaque to the optimiser: per-slot addresses built from it inside a loop stay inside the
+// loop instead of being hoisted (and held in registers) across the determinant's whole life
+__device__ __forceinline__ int opaque_i(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 __device__ __forceinline__ cd shfl_cd(cd v, int src) { return mk(__shfl(v.re, src), __shfl(v.im, src)); }
 __device__ __forceinline__ cd wave_sum_cd(cd v) { return mk(wave_sum(v.re), wave_sum(v.im)); }
 
@@ -496,14 +503,14 @@ __device__ __forceinline__ void lu16_factor(Lu16& f, int lane) {
     }
     const int p = __builtin_amdgcn_readfirstlane(br);
     if (__builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0)) f.zero_pivot = true;
-    if (p != q) {
-      f.parity ^= 1;
-      const int src = (r == q ? p : (r == p ? q : r)) + 16 * g;
+    // swap rows p and q, branch-free (p == q shuffles every lane onto itself), so the matrix keeps
+    // one set of registers
+    f.parity ^= p != q ? 1 : 0;
+    const int src = (r == q ? p : (r == p ? q : r)) + 16 * g;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) f.m[s] = shfl_cd(f.m[s], src);
-      f.orig = __shfl(f.orig, src);
-      cq = shfl_cd(cq, src);
-    }
+    for (int s = 0; s < 4; ++s) f.m[s] = shfl_cd(f.m[s], src);
+    f.orig = __shfl(f.orig, src);
+    cq = shfl_cd(cq, src);
     const cd pv = shfl_cd(cq, q);
     const bool zero = pv.re == 0.0 && pv.im == 0.0;
     const cd l = (r > q && !zero) ? cdiv_(cq, pv) : mk(0, 0);
@@ -675,7 +682,8 @@ __device__ __forceinline__ void det_head_tail(const JetGeom& j, const DetParams&
 // KIND (compile time, so that each variant keeps only the registers it needs): 0 log psi only,
 // 1 with tangents, 2 with tangents and the Laplacian.
 template <typename T, int KIND>
-__global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 4 : (KIND == 1 ? 3 : 2))))
+void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
                                                    const int32_t* __restrict__ S, int64_t w0,
                                                    const T* __restrict__ bf, double* __restrict__ out_lp,
                                                    double* __restrict__ out_tan, double* __restrict__ out_lap) {
@@ -749,6 +757,7 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
   const int rs_ = nt + 2;
   for (int k = wave; k < K; k += nwaves) {
     Lu16 f;
+    const int l16 = opaque_i(lane & 15), g = opaque_i(lane >> 4);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int a = l16, i = 4 * g + s;
@@ -767,6 +776,9 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
 #pragma unroll
     for (int s = 0; s < 4; ++s) ob[s] = mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, 4 * g + s, l16);
     cd hsum = mk(0, 0);
+    // Lap M's cross term 2 sum_c denv_c dbf_c lives in column i of tangent c = 1 + 2 i + d, which the
+    // tangent loop below already reads: summed there
+    cd lapx = mk(0, 0);
     for (int c = 1; c <= nt; ++c) {
       int ec, dc;
       if (j.mode == 2) {
@@ -779,11 +791,17 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
       const T* bfc = bfc0 + c * chs;
       cd dm[4];
       cd tr = mk(0, 0);
+      const int l16c = opaque_i(lane & 15), gc = opaque_i(lane >> 4);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const int a = 4 * g + s, i = l16, e_ = a * 16 + i;
-        cd v = env[e_] * bf_at(bfc, rowlen, KN, N, k, a, i);
-        if (i == ec && a < N) v = v + (dc == 0 ? dex[e_] : dey[e_]) * ob[s];
+        const int a = 4 * gc + s, i = l16c, e_ = a * 16 + i;
+        const cd bv = bf_at(bfc, rowlen, KN, N, k, a, i);
+        cd v = env[e_] * bv;
+        if (i == ec && a < N) {
+          const cd de = dc == 0 ? dex[e_] : dey[e_];
+          v = v + de * ob[s];
+          if constexpr (KIND == 2) lapx = lapx + mi[s] * (2.0 * (de * bv));
+        }
         dm[s] = v;
         tr = tr + mi[s] * v;
       }
@@ -815,13 +833,10 @@ __global__ __launch_bounds__(512) void k_det_jet16(JetGeom j, DetParams p, const
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int a = 4 * g + s, i = l16, e_ = a * 16 + i;
-        const int ic = i < N ? i : N - 1;
-        cd v = lenv[e_] * ob[s] + env[e_] * bf_at(bfl, rowlen, KN, N, k, a, i);
-        v = v + 2.0 * (dex[e_] * bf_at(bfc0 + (1 + 2 * ic) * chs, rowlen, KN, N, k, a, i) +
-                       dey[e_] * bf_at(bfc0 + (2 + 2 * ic) * chs, rowlen, KN, N, k, a, i));
+        const cd v = lenv[e_] * ob[s] + env[e_] * bf_at(bfl, rowlen, KN, N, k, a, i);
         tr = tr + mi[s] * v;
       }
-      tr = wave_sum_cd(tr);
+      tr = wave_sum_cd(tr + lapx);
       if (lane == 0) res[k * rs_ + cl] = tr - hsum;
     }
   }
@@ -1072,13 +1087,6 @@ __device__ __forceinline__ void mfma_block_pair(const cd (&mi)[4][4], const cd (
 
 // One workgroup per configuration (1, 2 or 4 waves); envelope jet built once in LDS, then each
 // wave takes determinants k = wave, wave + nwaves, ... in registers.  KIND as k_det_jet16.
-// x, opaque to the optimiser: per-slot addresses built from it inside a loop stay inside the
-// loop instead of being hoisted (and held in registers) across the determinant's whole life
-__device__ __forceinline__ int opaque_i(int x) {
-  asm volatile("" : "+v"(x));
-  return x;
-}
-
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 2 ? 1 : 2)))
 void k_det_jet32(JetGeom j, DetParams p, const double* __restrict__ R,
