@@ -209,3 +209,28 @@ def test_attention_beyond_16_tokens(N, d, heads):
     assert np.max(np.abs(lpm(R, S).cpu().numpy().real - v.real)) < 2e-4
     em = make_local_energy(lpm, 1.0, 0.7, 0.2)(R[:1], S[:1]).cpu().numpy()
     assert abs(em[0] - e64[0]) < 5e-3 * max(1.0, abs(e64[0]))
+
+
+@pytest.mark.parametrize("N,extra", [(17, dict(n_dets=3, use_jastrow=True, learnable_env=True, n_knots=6)),
+                                     (32, dict(n_dets=2, use_jastrow=True, jastrow_kind="cusp"))])
+def test_register_det_head_beyond_16(N, extra):
+    """The register determinant head for 16 < N <= 32 (2 x 2 blocks of 16 x 16, padded with an
+    identity block at N = 17) with several determinants, a Jastrow factor and a learnable envelope:
+    float64 log psi (three walkers) and E_loc (one walker, kinetic + SOC + Zeeman) against the
+    oracle."""
+    L = 2.0 * np.sqrt(np.pi * N)
+    g = bands.rashba_gs(L, N, 1.0, 0.6, 0.3, n_cut=10, twist=(0.1, 0.17))
+    kw = dict(embed_dim=16, depth=1, n_heads=2, n_freq=2, **extra)
+    arch = opf.Arch(N, g["Gs"], g["W"], L, **kw)
+    theta = opf.init_theta(arch, 7, backflow_scale=0.05, scale_all=0.02)
+    R, S = _configs(3, N, L, 33)
+    lp64 = make_logpsi(ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, **kw), theta.cuda())
+    v = lp64(R, S).cpu().numpy()
+    ref = np.array([complex(opf.logpsi(arch, theta, torch.tensor(R[w]), torch.tensor(S[w]))) for w in range(3)])
+    assert np.max(np.abs(v.real - ref.real)) < 1e-10
+    dim = np.angle(np.exp(1j * (v.imag - ref.imag)))
+    assert np.max(np.abs(dim)) < 1e-10
+    e64 = make_local_energy(lp64, 1.0, 0.6, 0.3)(R[:1], S[:1]).cpu().numpy()
+    Eo = ole.make_local_energy(opf.make_logpsi(arch, theta), 1.0, 0.6, 0.3)
+    er = Eo(torch.tensor(R[0]), torch.tensor(S[0]))
+    assert abs(e64[0] - er) < 1e-8 * max(1.0, abs(er))
