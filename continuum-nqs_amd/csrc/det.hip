@@ -462,6 +462,29 @@ __device__ __forceinline__ int opaque_i(int x) {
 __device__ __forceinline__ cd shfl_cd(cd v, int src) { return mk(__shfl(v.re, src), __shfl(v.im, src)); }
 __device__ __forceinline__ cd wave_sum_cd(cd v) { return mk(wave_sum(v.re), wave_sum(v.im)); }
 
+// DPP moves inside each 16-lane row (VALU, no LDS traffic): row_newbcast:L (0x150 + L) gives every
+// lane the value of lane L of its row, row_ror:K (0x120 + K) the value of lane (i - K) mod 16.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+template <int L>
+__device__ __forceinline__ cd bcast16(cd v) {
+  return mk(dpp_d<0x150 + L>(v.re), dpp_d<0x150 + L>(v.im));
+}
+// the value of one lane, wave-uniform
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+
 struct Lu16 {
   cd m[4];      // row-chunk layout; after factor(): L below the diagonal (unit), U on and above
   int orig;     // original row index of the row now at this lane's position
@@ -482,48 +505,68 @@ __device__ __forceinline__ cd pick4(const cd (&m)[4], int s) {
   return mk(v[2 * s], v[2 * s + 1]);
 }
 
-__device__ __forceinline__ void lu16_factor(Lu16& f, int lane) {
+// Pivot step Q (compile time, so that row broadcasts are DPP row_newbcast:Q): the column and the
+// row swap move across rows with ds_bpermute; the pivot search (rotations), the pivot and the pivot
+// row stay inside each 16-lane row.  Branch-free: the matrix keeps one set of registers.
+template <int Q>
+__device__ __forceinline__ void lu16_step(Lu16& f, int lane) {
+  constexpr int SQ = Q & 3, GQ = Q >> 2;
   const int r = lane & 15, g = lane >> 4;
-  f.orig = r;
-  f.parity = 0;
-  f.zero_pivot = false;
-#pragma unroll 1
-  for (int q = 0; q < 16; ++q) {
-    const int src_col = r + 16 * (q >> 2);
-    cd cq = shfl_cd(pick4(f.m, q & 3), src_col);
-    double v = r >= q ? fabs(cq.re) + fabs(cq.im) : -1.0;
-    int br = r;
+  cd cq = shfl_cd(f.m[SQ], r + 16 * GQ);  // X[r][Q]
+  double v = r >= Q ? fabs(cq.re) + fabs(cq.im) : -1.0;
+  int br = r;
+#define NQS_LU_MAX(K)                                                        \
+  {                                                                          \
+    const double ov = dpp_d<0x120 + K>(v);                                   \
+    const int obr = dpp_i<0x120 + K>(br);                                    \
+    const bool take = ov > v || (ov == v && obr < br);                       \
+    v = take ? ov : v;                                                       \
+    br = take ? obr : br;                                                    \
+  }
+  NQS_LU_MAX(1)
+  NQS_LU_MAX(2)
+  NQS_LU_MAX(4)
+  NQS_LU_MAX(8)
+#undef NQS_LU_MAX
+  const int p = __builtin_amdgcn_readfirstlane(br);
+  f.zero_pivot = f.zero_pivot || __builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0) != 0;
+  f.parity ^= p != Q ? 1 : 0;
+  const int src = (r == Q ? p : (r == p ? Q : r)) + 16 * g;  // p == Q: every lane onto itself
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      const double ov = __shfl_xor(v, off);
-      const int obr = __shfl_xor(br, off);
-      const bool take = ov > v || (ov == v && obr < br);
-      v = take ? ov : v;
-      br = take ? obr : br;
-    }
-    const int p = __builtin_amdgcn_readfirstlane(br);
-    if (__builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0)) f.zero_pivot = true;
-    // swap rows p and q, branch-free (p == q shuffles every lane onto itself), so the matrix keeps
-    // one set of registers
-    f.parity ^= p != q ? 1 : 0;
-    const int src = (r == q ? p : (r == p ? q : r)) + 16 * g;
+  for (int s = 0; s < 4; ++s) f.m[s] = shfl_cd(f.m[s], src);
+  f.orig = __shfl(f.orig, src);
+  cq = shfl_cd(cq, src);
+  const cd pv = bcast16<Q>(cq);
+  const bool zero = pv.re == 0.0 && pv.im == 0.0;
+  const cd l = (r > Q && !zero) ? cdiv_(cq, pv) : mk(0, 0);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) f.m[s] = shfl_cd(f.m[s], src);
-    f.orig = __shfl(f.orig, src);
-    cq = shfl_cd(cq, src);
-    const cd pv = shfl_cd(cq, q);
-    const bool zero = pv.re == 0.0 && pv.im == 0.0;
-    const cd l = (r > q && !zero) ? cdiv_(cq, pv) : mk(0, 0);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const cd rq = shfl_cd(f.m[s], q + 16 * g);
-      const int col = 4 * g + s;
-      if (r > q) {
-        if (col == q) f.m[s] = l;
-        else if (col > q) f.m[s] = f.m[s] - l * rq;
-      }
+  for (int s = 0; s < 4; ++s) {
+    const cd rq = bcast16<Q>(f.m[s]);
+    const int col = 4 * g + s;
+    if (r > Q) {
+      if (col == Q) f.m[s] = l;
+      else if (col > Q) f.m[s] = f.m[s] - l * rq;
     }
   }
+}
+
+template <int Q>
+struct Lu16Steps {
+  static __device__ __forceinline__ void run(Lu16& f, int lane) {
+    lu16_step<Q>(f, lane);
+    Lu16Steps<Q + 1>::run(f, lane);
+  }
+};
+template <>
+struct Lu16Steps<16> {
+  static __device__ __forceinline__ void run(Lu16&, int) {}
+};
+
+__device__ __forceinline__ void lu16_factor(Lu16& f, int lane) {
+  f.orig = lane & 15;
+  f.parity = 0;
+  f.zero_pivot = false;
+  Lu16Steps<0>::run(f, lane);
 }
 
 // log det from the factors (principal branch), as lu_logdet for the first N pivots.
@@ -540,34 +583,52 @@ __device__ __forceinline__ cd lu16_logdet(const Lu16& f, int lane, int N) {
   return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
 }
 
-// Mi = U^-1 L^-1 P in the row-chunk layout.
-__device__ __forceinline__ void lu16_inverse(const Lu16& f, int lane, cd (&y)[4]) {
-  const int r = lane & 15, g = lane >> 4;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) y[s] = mk(4 * g + s == f.orig ? 1.0 : 0.0, 0);
-#pragma unroll 1
-  for (int q = 0; q < 16; ++q) {
-    const cd lq = shfl_cd(pick4(f.m, q & 3), r + 16 * (q >> 2));
+// Mi = U^-1 L^-1 P in the row-chunk layout.  Step Q at compile time: row Q of y is a DPP
+// broadcast, the column of L or U one ds_bpermute, U[Q][Q] one readlane.
+template <int Q>
+struct Lu16Inv {
+  static __device__ __forceinline__ void fwd(const Lu16& f, int lane, cd (&y)[4]) {
+    const int r = lane & 15;
+    const cd lq = shfl_cd(f.m[Q & 3], r + 16 * (Q >> 2));  // L[r][Q]
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const cd yq = shfl_cd(y[s], q + 16 * g);
-      if (r > q) y[s] = y[s] - lq * yq;
+      const cd yq = bcast16<Q>(y[s]);
+      if (r > Q) y[s] = y[s] - lq * yq;
     }
+    Lu16Inv<Q + 1>::fwd(f, lane, y);
   }
-#pragma unroll 1
-  for (int q = 15; q >= 0; --q) {
-    const cd mq = pick4(f.m, q & 3);
-    const cd uqq = shfl_cd(mq, q + 16 * (q >> 2));
-    if (r == q)
+  static __device__ __forceinline__ void bwd(const Lu16& f, int lane, cd (&y)[4]) {
+    const int r = lane & 15;
+    const cd mq = f.m[Q & 3];
+    constexpr int UL = Q + 16 * (Q >> 2);  // lane of U[Q][Q]
+    const cd uqq = mk(readlane_d(mq.re, UL), readlane_d(mq.im, UL));
+    if (r == Q)
 #pragma unroll
       for (int s = 0; s < 4; ++s) y[s] = cdiv_(y[s], uqq);
-    const cd urq = shfl_cd(mq, r + 16 * (q >> 2));
+    const cd urq = shfl_cd(mq, r + 16 * (Q >> 2));  // U[r][Q]
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const cd zq = shfl_cd(y[s], q + 16 * g);
-      if (r < q) y[s] = y[s] - urq * zq;
+      const cd zq = bcast16<Q>(y[s]);
+      if (r < Q) y[s] = y[s] - urq * zq;
     }
+    Lu16Inv<Q - 1>::bwd(f, lane, y);
   }
+};
+template <>
+struct Lu16Inv<16> {
+  static __device__ __forceinline__ void fwd(const Lu16&, int, cd (&)[4]) {}
+};
+template <>
+struct Lu16Inv<-1> {
+  static __device__ __forceinline__ void bwd(const Lu16&, int, cd (&)[4]) {}
+};
+
+__device__ __forceinline__ void lu16_inverse(const Lu16& f, int lane, cd (&y)[4]) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) y[s] = mk(4 * g + s == f.orig ? 1.0 : 0.0, 0);
+  Lu16Inv<0>::fwd(f, lane, y);
+  Lu16Inv<15>::bwd(f, lane, y);
 }
 
 template <typename T>
