@@ -460,7 +460,6 @@ __device__ __forceinline__ int opaque_i(int x) {
 }
 
 __device__ __forceinline__ cd shfl_cd(cd v, int src) { return mk(__shfl(v.re, src), __shfl(v.im, src)); }
-__device__ __forceinline__ cd wave_sum_cd(cd v) { return mk(wave_sum(v.re), wave_sum(v.im)); }
 
 // DPP moves inside each 16-lane row (VALU, no LDS traffic): row_newbcast:L (0x150 + L) gives every
 // lane the value of lane L of its row, row_ror:K (0x120 + K) the value of lane (i - K) mod 16.
@@ -483,6 +482,30 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
   const long long b = __builtin_bit_cast(long long, x);
   const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+// Sum over the wave: pairwise within quads (quad_perm [1,0,3,2], [2,3,0,1]), then quad with
+// quad (row_half_mirror: lane i with 7 - i), half-row with half-row (row_mirror: i with 15 - i),
+// all as DPP, then across the four rows with two shuffles.  Every step adds two lanes' values
+// in both lanes, so all 64 lanes end with the same bits.
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+__device__ __forceinline__ cd wave_sum_cd(cd v) { return mk(wave_sum_d(v.re), wave_sum_d(v.im)); }
+// product over the wave, same pattern
+__device__ __forceinline__ cd wave_prod_cd(cd v) {
+  v = v * mk(dpp_d<0xB1>(v.re), dpp_d<0xB1>(v.im));
+  v = v * mk(dpp_d<0x4E>(v.re), dpp_d<0x4E>(v.im));
+  v = v * mk(dpp_d<0x141>(v.re), dpp_d<0x141>(v.im));
+  v = v * mk(dpp_d<0x140>(v.re), dpp_d<0x140>(v.im));
+  v = v * mk(__shfl_xor(v.re, 16), __shfl_xor(v.im, 16));
+  v = v * mk(__shfl_xor(v.re, 32), __shfl_xor(v.im, 32));
+  return v;
 }
 
 struct Lu16 {
@@ -575,10 +598,8 @@ __device__ __forceinline__ cd lu16_logdet(const Lu16& f, int lane, int N) {
   const bool mine = (r >> 2) == g && r < N;
   const cd u = pick4(f.m, r & 3);
   const double au = mine ? cabs_(u) : 1.0;
-  double la = wave_sum(log(au));
-  cd ph = mine && au > 0 ? mk(u.re / au, u.im / au) : mk(1, 0);
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) ph = ph * shfl_cd(ph, lane ^ off);
+  double la = wave_sum_d(log(au));
+  cd ph = wave_prod_cd(mine && au > 0 ? mk(u.re / au, u.im / au) : mk(1, 0));
   if (f.parity) ph = mk(-ph.re, -ph.im);
   return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
 }
@@ -1026,12 +1047,10 @@ __device__ __forceinline__ cd lu32_logdet(const Lu32& f, int lane, int N) {
   const cd u0 = pick4(f.m[0], l & 3), u1 = pick4(f.m[3], l & 3);
   const bool m0 = on && l < N, m1 = on && 16 + l < N;
   const double a0 = m0 ? cabs_(u0) : 1.0, a1 = m1 ? cabs_(u1) : 1.0;
-  double la = wave_sum(log(a0) + log(a1));
+  double la = wave_sum_d(log(a0) + log(a1));
   cd ph0 = m0 && a0 > 0 ? mk(u0.re / a0, u0.im / a0) : mk(1, 0);
   cd ph1 = m1 && a1 > 0 ? mk(u1.re / a1, u1.im / a1) : mk(1, 0);
-  cd ph = ph0 * ph1;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) ph = ph * shfl_cd(ph, lane ^ off);
+  cd ph = wave_prod_cd(ph0 * ph1);
   if (f.parity) ph = mk(-ph.re, -ph.im);
   return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
 }
