@@ -942,93 +942,105 @@ struct Lu32 {
   bool zero_pivot;
 };
 
-// Pivot steps q = 16 BQ .. 16 BQ + 15 (column q in block column BQ, slot q & 3 of lanes
-// 16 ((q >> 2) & 3) + l).  With BQ = 1 rows 0..15 are final and the pivot row lies in block row 1,
-// so every block index is static.
-template <int BQ>
-__device__ __forceinline__ void lu32_half(Lu32& f, int lane) {
+// Pivot step q = 16 BQ + QL at compile time (column q in block column BQ, slot QL & 3 of lanes
+// 16 (QL >> 2) + l).  With BQ = 1 rows 0..15 are final and the pivot row lies in block row 1, so
+// every block index is static.  Broadcasts of row q and the pivot search stay inside the 16-lane
+// rows (DPP); the column and the rows of the (run-time) pivot move with ds_bpermute.
+template <int BQ, int QL>
+__device__ __forceinline__ void lu32_step(Lu32& f, int lane) {
+  constexpr int q = 16 * BQ + QL, gq = QL >> 2, sq = QL & 3, lq = QL;
   const int l = lane & 15, g = lane >> 4;
-#pragma unroll 1
-  for (int q4 = 16 * BQ; q4 < 16 * BQ + 16; q4 += 4) {
-    const int gq = (q4 >> 2) & 3;
-#pragma unroll
-    for (int sq = 0; sq < 4; ++sq) {
-      const int q = q4 + sq, lq = q & 15;
-      // column q of this lane's rows l (block row 0) and 16 + l (block row 1)
-      cd c0 = BQ == 0 ? shfl_cd(f.m[0][sq], l + 16 * gq) : mk(0, 0);
-      cd c1 = shfl_cd(f.m[2 + BQ][sq], l + 16 * gq);
-      const double v0 = (BQ == 0 && l >= q) ? fabs(c0.re) + fabs(c0.im) : -1.0;
-      const double v1 = 16 + l >= q ? fabs(c1.re) + fabs(c1.im) : -1.0;
-      double v = v1 > v0 ? v1 : v0;
-      int br = v1 > v0 ? 16 + l : l;
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int obr = __shfl_xor(br, off);
-        const bool take = ov > v || (ov == v && obr < br);
-        v = take ? ov : v;
-        br = take ? obr : br;
-      }
-      const int p = __builtin_amdgcn_readfirstlane(br);
-      if (__builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0)) f.zero_pivot = true;
-      const int bp = BQ == 1 ? 1 : p >> 4, lp = p & 15;
-      // swap rows p and q (branch-free: with p == q every write stores the value it replaces, and
-      // the matrix keeps one set of registers)
-      f.parity ^= p != q ? 1 : 0;
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          cd& mq = f.m[2 * BQ + bj][s];
-          const cd xp = shfl_cd(bp ? f.m[2 + bj][s] : f.m[bj][s], lp + 16 * g);
-          const cd xq = shfl_cd(mq, lq + 16 * g);
-          if (l == lq) mq = xp;
-          if (l == lp) {
-            if (bp) f.m[2 + bj][s] = xq;
-            else f.m[bj][s] = xq;
-          }
-          __builtin_amdgcn_sched_barrier(0);  // one slot's shuffles live at a time
-        }
-      const int op = __shfl(bp ? f.orig[1] : f.orig[0], lp + 16 * g);
-      const int oq = __shfl(f.orig[BQ], lq + 16 * g);
-      if (l == lq) f.orig[BQ] = op;
-      if (l == lp) {
-        if (bp) f.orig[1] = oq;
-        else f.orig[0] = oq;
-      }
-      const cd xpq = shfl_cd(bp ? c1 : c0, lp), xqq = shfl_cd(BQ ? c1 : c0, lq);
-      if (l == lq) {
-        if (BQ) c1 = xpq;
-        else c0 = xpq;
-      }
-      if (l == lp) {
-        if (bp) c1 = xqq;
-        else c0 = xqq;
-      }
-      const cd pv = xpq;
-      const bool zero = pv.re == 0.0 && pv.im == 0.0;
-      const cd l0 = (BQ == 0 && l > q && !zero) ? cdiv_(c0, pv) : mk(0, 0);
-      const cd l1 = (16 + l > q && !zero) ? cdiv_(c1, pv) : mk(0, 0);
-      // eliminate below row q (columns of block column 0 are final once q >= 16)
-#pragma unroll
-      for (int bj = BQ; bj < 2; ++bj)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const cd rq = shfl_cd(f.m[2 * BQ + bj][s], lq + 16 * g);
-          const int col = 16 * bj + 4 * g + s;
-          if (BQ == 0 && l > q) {
-            if (col == q) f.m[bj][s] = l0;
-            else if (col > q) f.m[bj][s] = f.m[bj][s] - l0 * rq;
-          }
-          if (16 + l > q) {
-            if (col == q) f.m[2 + bj][s] = l1;
-            else if (col > q) f.m[2 + bj][s] = f.m[2 + bj][s] - l1 * rq;
-          }
-          __builtin_amdgcn_sched_barrier(0);
-        }
-    }
+  // column q of this lane's rows l (block row 0) and 16 + l (block row 1)
+  cd c0 = BQ == 0 ? shfl_cd(f.m[0][sq], l + 16 * gq) : mk(0, 0);
+  cd c1 = shfl_cd(f.m[2 + BQ][sq], l + 16 * gq);
+  const double v0 = (BQ == 0 && l >= q) ? fabs(c0.re) + fabs(c0.im) : -1.0;
+  const double v1 = 16 + l >= q ? fabs(c1.re) + fabs(c1.im) : -1.0;
+  double v = v1 > v0 ? v1 : v0;
+  int br = v1 > v0 ? 16 + l : l;
+#define NQS_LU_MAX(K)                                                        \
+  {                                                                          \
+    const double ov = dpp_d<0x120 + K>(v);                                   \
+    const int obr = dpp_i<0x120 + K>(br);                                    \
+    const bool take = ov > v || (ov == v && obr < br);                       \
+    v = take ? ov : v;                                                       \
+    br = take ? obr : br;                                                    \
   }
+  NQS_LU_MAX(1)
+  NQS_LU_MAX(2)
+  NQS_LU_MAX(4)
+  NQS_LU_MAX(8)
+#undef NQS_LU_MAX
+  const int p = __builtin_amdgcn_readfirstlane(br);
+  f.zero_pivot = f.zero_pivot || __builtin_amdgcn_readfirstlane(v == 0.0 ? 1 : 0) != 0;
+  const int bp = BQ == 1 ? 1 : p >> 4, lp = p & 15;
+  // swap rows p and q (branch-free: with p == q every write stores the value it replaces, and
+  // the matrix keeps one set of registers)
+  f.parity ^= p != q ? 1 : 0;
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      cd& mq = f.m[2 * BQ + bj][s];
+      const cd xp = shfl_cd(bp ? f.m[2 + bj][s] : f.m[bj][s], lp + 16 * g);
+      const cd xq = bcast16<lq>(mq);
+      if (l == lq) mq = xp;
+      if (l == lp) {
+        if (bp) f.m[2 + bj][s] = xq;
+        else f.m[bj][s] = xq;
+      }
+    }
+  const int op = __shfl(bp ? f.orig[1] : f.orig[0], lp + 16 * g);
+  const int oq = dpp_i<0x150 + lq>(f.orig[BQ]);
+  if (l == lq) f.orig[BQ] = op;
+  if (l == lp) {
+    if (bp) f.orig[1] = oq;
+    else f.orig[0] = oq;
+  }
+  // c0 / c1 are the same in every row of 16 lanes: the pivot's and row q's values by readlane
+  const cd cp = bp ? c1 : c0, cqq = BQ ? c1 : c0;
+  const cd xpq = mk(readlane_d(cp.re, lp), readlane_d(cp.im, lp));
+  const cd xqq = mk(readlane_d(cqq.re, lq), readlane_d(cqq.im, lq));
+  if (l == lq) {
+    if (BQ) c1 = xpq;
+    else c0 = xpq;
+  }
+  if (l == lp) {
+    if (bp) c1 = xqq;
+    else c0 = xqq;
+  }
+  const cd pv = xpq;
+  const bool zero = pv.re == 0.0 && pv.im == 0.0;
+  const cd l0 = (BQ == 0 && l > q && !zero) ? cdiv_(c0, pv) : mk(0, 0);
+  const cd l1 = (16 + l > q && !zero) ? cdiv_(c1, pv) : mk(0, 0);
+  // eliminate below row q (columns of block column 0 are final once q >= 16)
+#pragma unroll
+  for (int bj = BQ; bj < 2; ++bj)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const cd rq = bcast16<lq>(f.m[2 * BQ + bj][s]);
+      const int col = 16 * bj + 4 * g + s;
+      if (BQ == 0 && l > q) {
+        if (col == q) f.m[bj][s] = l0;
+        else if (col > q) f.m[bj][s] = f.m[bj][s] - l0 * rq;
+      }
+      if (16 + l > q) {
+        if (col == q) f.m[2 + bj][s] = l1;
+        else if (col > q) f.m[2 + bj][s] = f.m[2 + bj][s] - l1 * rq;
+      }
+    }
 }
+
+template <int BQ, int QL>
+struct Lu32Steps {
+  static __device__ __forceinline__ void run(Lu32& f, int lane) {
+    lu32_step<BQ, QL>(f, lane);
+    Lu32Steps<BQ, QL + 1>::run(f, lane);
+  }
+};
+template <int BQ>
+struct Lu32Steps<BQ, 16> {
+  static __device__ __forceinline__ void run(Lu32&, int) {}
+};
 
 __device__ __forceinline__ void lu32_factor(Lu32& f, int lane) {
   const int l = lane & 15;
@@ -1036,8 +1048,8 @@ __device__ __forceinline__ void lu32_factor(Lu32& f, int lane) {
   f.orig[1] = 16 + l;
   f.parity = 0;
   f.zero_pivot = false;
-  lu32_half<0>(f, lane);
-  lu32_half<1>(f, lane);
+  Lu32Steps<0, 0>::run(f, lane);
+  Lu32Steps<1, 0>::run(f, lane);
 }
 
 // log det from the factors (principal branch), as lu_logdet for the first N pivots
@@ -1057,64 +1069,63 @@ __device__ __forceinline__ cd lu32_logdet(const Lu32& f, int lane, int N) {
 
 // Mi = U^-1 L^-1 P in the row-chunk layout.  Forward: rows below q minus L[r][q] row q;
 // backward: row q divided by U[q][q], rows above q minus U[r][q] row q.
-template <int BQ>
-__device__ __forceinline__ void lu32_fwd_half(const Lu32& f, int lane, cd (&y)[4][4]) {
-  const int l = lane & 15, g = lane >> 4;
-#pragma unroll 1
-  for (int q4 = 16 * BQ; q4 < 16 * BQ + 16; q4 += 4) {
-    const int gq = (q4 >> 2) & 3;
-    // a loop (not unrolled): the compiler would otherwise sink the updates of rows that no later
-    // step reads to the end of the group, holding every step's broadcast row in registers
-#pragma unroll 1
-    for (int sq = 0; sq < 4; ++sq) {
-      const int q = q4 + sq, lq = q & 15;
-      const cd c0 = BQ == 0 ? shfl_cd(pick4(f.m[0], sq), l + 16 * gq) : mk(0, 0);
-      const cd c1 = shfl_cd(pick4(f.m[2 + BQ], sq), l + 16 * gq);
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const cd yq = shfl_cd(y[2 * BQ + bj][s], lq + 16 * g);
-          if (BQ == 0 && l > q) y[bj][s] = y[bj][s] - c0 * yq;
-          if (16 + l > q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
-          __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-  }
-}
+// keeps x in registers at this point: the updates of rows that no later step of a half reads are
+// not sunk to its end (which would hold every step's broadcast row in registers)
+__device__ __forceinline__ void pin_cd(cd& x) { asm volatile("" : "+v"(x.re), "+v"(x.im)); }
 
-template <int BQ>
-__device__ __forceinline__ void lu32_bwd_half(const Lu32& f, int lane, cd (&y)[4][4]) {
-  const int l = lane & 15, g = lane >> 4;
-#pragma unroll 1
-  for (int q4 = 16 * BQ + 12; q4 >= 16 * BQ; q4 -= 4) {
-    const int gq = (q4 >> 2) & 3;
-#pragma unroll 1
-    for (int sq = 3; sq >= 0; --sq) {
-      const int q = q4 + sq, lq = q & 15;
-      const cd ru = cdiv_(mk(1, 0), shfl_cd(pick4(f.m[3 * BQ], sq), lq + 16 * gq));  // 1 / U[q][q]
+template <int BQ, int QL>
+struct Lu32Inv {
+  static constexpr int q = 16 * BQ + QL, gq = QL >> 2, sq = QL & 3, lq = QL;
+  static __device__ __forceinline__ void fwd(const Lu32& f, int lane, cd (&y)[4][4]) {
+    const int l = lane & 15;
+    const cd c0 = BQ == 0 ? shfl_cd(f.m[0][sq], l + 16 * gq) : mk(0, 0);
+    const cd c1 = shfl_cd(f.m[2 + BQ][sq], l + 16 * gq);
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          cd& yy = y[2 * BQ + bj][s];
-          const cd dv = yy * ru;
-          if (l == lq) yy = dv;
-        }
-      const cd c0 = shfl_cd(pick4(f.m[BQ], sq), l + 16 * gq);
-      const cd c1 = BQ == 1 ? shfl_cd(pick4(f.m[3], sq), l + 16 * gq) : mk(0, 0);
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const cd yq = shfl_cd(y[2 * BQ + bj][s], lq + 16 * g);
-          if (l < q) y[bj][s] = y[bj][s] - c0 * yq;
-          if (BQ == 1 && 16 + l < q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
-          __builtin_amdgcn_sched_barrier(0);
-        }
-    }
+      for (int s = 0; s < 4; ++s) {
+        const cd yq = bcast16<lq>(y[2 * BQ + bj][s]);
+        if (BQ == 0 && l > q) y[bj][s] = y[bj][s] - c0 * yq;
+        if (16 + l > q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
+        pin_cd(y[2 + bj][s]);
+      }
+    Lu32Inv<BQ, QL + 1>::fwd(f, lane, y);
   }
-}
+  static __device__ __forceinline__ void bwd(const Lu32& f, int lane, cd (&y)[4][4]) {
+    const int l = lane & 15;
+    constexpr int UL = lq + 16 * gq;  // lane of U[q][q]
+    const cd uqq = f.m[3 * BQ][sq];
+    const cd ru = cdiv_(mk(1, 0), mk(readlane_d(uqq.re, UL), readlane_d(uqq.im, UL)));  // 1 / U[q][q]
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        cd& yy = y[2 * BQ + bj][s];
+        const cd dv = yy * ru;
+        if (l == lq) yy = dv;
+      }
+    const cd c0 = shfl_cd(f.m[BQ][sq], l + 16 * gq);
+    const cd c1 = BQ == 1 ? shfl_cd(f.m[3][sq], l + 16 * gq) : mk(0, 0);
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const cd yq = bcast16<lq>(y[2 * BQ + bj][s]);
+        if (l < q) y[bj][s] = y[bj][s] - c0 * yq;
+        if (BQ == 1 && 16 + l < q) y[2 + bj][s] = y[2 + bj][s] - c1 * yq;
+        pin_cd(y[bj][s]);
+      }
+    Lu32Inv<BQ, QL - 1>::bwd(f, lane, y);
+  }
+};
+template <int BQ>
+struct Lu32Inv<BQ, 16> {
+  static __device__ __forceinline__ void fwd(const Lu32&, int, cd (&)[4][4]) {}
+};
+template <int BQ>
+struct Lu32Inv<BQ, -1> {
+  static __device__ __forceinline__ void bwd(const Lu32&, int, cd (&)[4][4]) {}
+};
 
 __device__ __forceinline__ void lu32_inverse(const Lu32& f, int lane, cd (&y)[4][4]) {
   const int g = lane >> 4;
@@ -1124,10 +1135,10 @@ __device__ __forceinline__ void lu32_inverse(const Lu32& f, int lane, cd (&y)[4]
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int s = 0; s < 4; ++s) y[2 * bi + bj][s] = mk(16 * bj + 4 * g + s == f.orig[bi] ? 1.0 : 0.0, 0);
-  lu32_fwd_half<0>(f, lane, y);
-  lu32_fwd_half<1>(f, lane, y);
-  lu32_bwd_half<1>(f, lane, y);
-  lu32_bwd_half<0>(f, lane, y);
+  Lu32Inv<0, 0>::fwd(f, lane, y);
+  Lu32Inv<1, 0>::fwd(f, lane, y);
+  Lu32Inv<1, 15>::bwd(f, lane, y);
+  Lu32Inv<0, 15>::bwd(f, lane, y);
 }
 
 // sum over the wave of tr(X Y) for X = A_IJ, Y = the accumulators of (A_JI)^T (f64 MFMA C
