@@ -736,12 +736,12 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
 }
 
 // ------------------------------------------------------------------ Cholesky (lower, float64)
-// Right-looking over outer panels of CH_NBO = 256 columns (rank-256 trailing updates on the f64
+// Right-looking over outer panels of CH_NBO = 512 columns (rank-512 trailing updates on the f64
 // MFMA GEMM, lower tiles only); inside an outer panel, left-looking over CH_NB = 64 columns:
 // factor the 64 x 64 diagonal block, solve the rows below it with its inverse, update the rest
 // of the outer panel.
 #define CH_NB 64
-#define CH_NBO 256
+#define CH_NBO 512
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
