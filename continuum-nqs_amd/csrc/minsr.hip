@@ -345,9 +345,10 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A,
 
 // 256 x 256 x 64 block tile on 8 waves (4 x 2), each owning a 64 x 128 sub-tile (4 x 8 blocks of
 // v_mfma_f32_16x16x32_bf16, 128 accumulators per lane: the same operand traffic per FLOP as
-// 32x32x16 blocks, in shorter MFMAs that interleave better with the fragment reads, measured 6%
-// faster); each thread stages 4 + 4 16-byte vectors per stage, so accumulators, staging and
-// fragments fit in registers at 2 waves per SIMD.
+// 32x32x16 blocks, in shorter MFMAs); the operand stages go from global memory straight to LDS
+// (double-buffered, 128 KB), so staging costs no registers.  On random data at the full K the
+// two changes together cut the Gram from 145 to 123 ms (the kernel is power-limited there, and
+// both spend less energy per FLOP).
 //
 // Tile order.  Every tile streams two 256-row panels over all of K, so HBM traffic is set by how
 // many concurrently running tiles share panels in an XCD's L2.  Tiles are grouped in 8 x 8
@@ -357,7 +358,6 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A,
 // block reading 12 panels, where a row-major walk of the triangle reads 33.
 #define G2_BM 256
 #define G2_BK 64
-#define G2_LD (G2_BK + 8)
 // Position t of the super-tile sequence -> tile (bm, bn) of the padded grid; positions whose tile
 // lies outside the lower triangle of the nt x nt grid are skipped by the caller.
 constexpr int SUPER = 8;
@@ -396,14 +396,17 @@ struct GramTail {
 };
 
 // one launch computes the Grams of up to kGramBatch systems of the same size (blockIdx.y = system)
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
 __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc, int64_t ntiles, int nt,
                                                        GramTail tail) {
   const __bf16* __restrict__ A = gb.A[blockIdx.y];
   const int64_t K = gb.K[blockIdx.y];
   float* __restrict__ C = gb.C[blockIdx.y];
+  // LDS: 2 stages x (row panel, column panel) x 256 rows x 128 bytes (64 bf16 of one k-step),
+  // the 16-byte slot s of row r holding k-chunk s ^ (r & 7) (conflict-free fragment reads)
   extern __shared__ __bf16 g2s[];
-  __bf16(*As)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s);
-  __bf16(*Bs)[G2_BM][G2_LD] = reinterpret_cast<__bf16(*)[G2_BM][G2_LD]>(g2s + 2 * G2_BM * G2_LD);
+  char* lds = reinterpret_cast<char*>(g2s);
   int64_t t, kb0 = 0, kb1 = K / G2_BK;
   if (tail.part == nullptr) {
     const int64_t per_xcd = (ntiles + 7) / 8;
@@ -424,67 +427,60 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc
   const int64_t m0 = (int64_t)bm * G2_BM, n0 = (int64_t)bn * G2_BM;
   // output origin: the tile in C, or the partial's own 256 x 256 square
   const int64_t om = tail.part == nullptr ? m0 : 0, on = tail.part == nullptr ? n0 : 0;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   f4v acc[4][8];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = (f4v)0.f;
-  // staging: thread covers row tid / 8 + 64 q (q = 0..3), 16-byte chunk tid % 8; A is in the
-  // blocked layout of k_center, so the k-step kb of a panel is the contiguous block
-  // A + (panel * nkb + kb) * 256 * 64
-  u4v ra[4], rb[4];  // ext vectors: HIP's uint4 struct would be demoted to scratch here
-  const int srow = tid >> 3, scol = (tid & 7) * 8;
+  // Staging by direct global -> LDS loads (global_load_lds_dwordx4: lane l's 16 bytes land at the
+  // wave's LDS base + 16 l), no registers in between.  A is in the blocked layout of k_center, so
+  // the k-step kb of a panel is the contiguous block A + (panel * nkb + kb) * 256 * 64; waves 0-3
+  // load the row panel, 4-7 the column panel, 64 rows each as 8 loads of 8 rows; lane l fetches the
+  // k-chunk (l & 7) ^ (l >> 3) of its row so that it lands in the swizzled slot.
   const int64_t nkb = K / G2_BK;
-  const __bf16* pa = A + ((int64_t)bm * nkb + kb0) * (G2_BM * G2_BK) + srow * G2_BK + scol;
-  const __bf16* pb = A + ((int64_t)bn * nkb + kb0) * (G2_BM * G2_BK) + srow * G2_BK + scol;
-  const int64_t rstep = 64 * G2_BK;
+  const int op = wave >> 2;
+  const __bf16* src = A + ((int64_t)(op ? bn : bm) * nkb + kb0) * (G2_BM * G2_BK) +
+                      ((wave & 3) * 64 + (lane >> 3)) * G2_BK + (((lane & 7) ^ (lane >> 3)) * 8);
+  const int dst0 = op * 32768 + (wave & 3) * 64 * 128;
   const int64_t nk = kb1 - kb0;  // >= 1: chunks are never empty
-  // Software pipeline over k-steps: the registers carry stage kb + 1 across the loop back edge,
-  // so its loads are issued a whole stage of MFMA work before they are stored to LDS (loads
-  // consumed within one iteration get sunk next to their stores by the compiler).
-#define NQS_G2_LOAD(kb_)                                                  \
-  {                                                                       \
-    const int64_t k0 = (int64_t)(kb_) * (G2_BM * G2_BK);                  \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                       \
-      ra[q] = *reinterpret_cast<const u4v*>(pa + q * rstep + k0);       \
-      rb[q] = *reinterpret_cast<const u4v*>(pb + q * rstep + k0);       \
-    }                                                                     \
+#define NQS_G2_DL(stage, kb_)                                                                           \
+  {                                                                                                     \
+    const __bf16* s_ = src + (int64_t)(kb_) * (G2_BM * G2_BK);                                          \
+    _Pragma("unroll") for (int j = 0; j < 8; ++j) __builtin_amdgcn_global_load_lds(                     \
+        s_ + j * 8 * G2_BK, (lds_ptr_t)(lds + (stage) * 65536 + dst0 + j * 1024), 16, 0, 0);             \
   }
-#define NQS_G2_STORE(buf)                                                        \
-  _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                \
-    *reinterpret_cast<u4v*>(&As[buf][srow + 64 * q][scol]) = ra[q];            \
-    *reinterpret_cast<u4v*>(&Bs[buf][srow + 64 * q][scol]) = rb[q];            \
-  }
-  NQS_G2_LOAD(0)
-  NQS_G2_STORE(0)
-  NQS_G2_LOAD(nk > 1 ? 1 : 0)
+  NQS_G2_DL(0, 0)
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   for (int64_t kb = 0; kb < nk; ++kb) {
     const int cur = kb & 1;
+    // stage kb + 1 into the other buffer (its last readers passed the barrier that ended kb - 1)
+    if (kb + 1 < nk) NQS_G2_DL(cur ^ 1, kb + 1)
+    const char* base = lds + cur * 65536;
 #pragma unroll
     for (int kk = 0; kk < G2_BK; kk += 32) {
+      const int slot = ((kk / 8 + (lane >> 4)) ^ (lane & 7)) * 16;
       bf8v a[4], b[8];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        a[i] = *reinterpret_cast<const bf8v*>(&As[cur][wm * 64 + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+        a[i] = *reinterpret_cast<const bf8v*>(base + (wm * 64 + i * 16 + (lane & 15)) * 128 + slot);
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        b[j] = *reinterpret_cast<const bf8v*>(&Bs[cur][wn * 128 + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+        b[j] = *reinterpret_cast<const bf8v*>(base + 32768 + (wn * 128 + j * 16 + (lane & 15)) * 128 + slot);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    // stage kb + 1 (loaded one iteration ago) into the idle buffer, then issue stage kb + 2
-    // (clamped: the tail re-reads the last stage, which is never consumed)
-    NQS_G2_STORE(cur ^ 1)
-    NQS_G2_LOAD(kb + 2 < nk ? kb + 2 : nk - 1)
+    // this wave's loads of stage kb + 1 have landed; the barrier makes every wave's visible and
+    // ends the reads of stage kb
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
   }
-#undef NQS_G2_LOAD
-#undef NQS_G2_STORE
+#undef NQS_G2_DL
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -533,7 +529,7 @@ int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hip
   }
   const int64_t nt = n / G2_BM;
   const int64_t tiles = supertile_count(nt);
-  const size_t lds = sizeof(__bf16) * 4 * G2_BM * G2_LD;
+  const size_t lds = 2 * 2 * G2_BM * G2_BK * sizeof(__bf16);  // 128 KB
   static bool attr = false;
   if (!attr) {
     NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_syrk_bf16_256, hipFuncAttributeMaxDynamicSharedMemorySize,
