@@ -237,7 +237,9 @@ __device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& 
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
 }
 
-constexpr int BF_BM = 128, BF_BK = 64, BF_LD = BF_BK + 8;
+constexpr int BF_BM = 128, BF_BK = 32, BF_LD = BF_BK + 8;
+// staging geometry: 8-float chunks per tile row, rows per pass of the 256 threads, passes
+constexpr int BF_CPR = BF_BK / 8, BF_RPP = 256 / BF_CPR, BF_NQ = BF_BM / BF_RPP;
 
 template <int SPLIT>
 constexpr size_t bf_lds_bytes() {
@@ -250,7 +252,7 @@ constexpr size_t bf_lds_bytes() {
 // One LDS buffer per operand and plane; the registers carry stage kb + 1 across the loop back
 // edge (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
 template <int SPLIT>
-__global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
+__global__ __launch_bounds__(256, 3) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
   constexpr int NP = SPLIT == 1 ? 1 : 2;
   constexpr int PLANE = BF_BM * BF_LD;  // A planes (hi, lo), then B planes
   extern __shared__ __bf16 bfs[];
@@ -269,16 +271,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  const int srow = tid >> 3, sk = (tid & 7) * 8;
-  const float* pa[4];
-  const float* pb[4];
+  const int srow = tid / BF_CPR, sk = (tid % BF_CPR) * 8;
+  const float* pa[BF_NQ];
+  const float* pb[BF_NQ];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int64_t gm = m0 + srow + 32 * q, gn = n0 + srow + 32 * q;
+  for (int q = 0; q < BF_NQ; ++q) {
+    const int64_t gm = m0 + srow + BF_RPP * q, gn = n0 + srow + BF_RPP * q;
     pa[q] = g.A + (gm < g.M ? gm : g.M - 1) * g.lda;
     pb[q] = g.B + (gn < g.N ? gn : g.N - 1) * g.ldb;
   }
-  f4v ra[4][2], rb[4][2];
+  f4v ra[BF_NQ][2], rb[BF_NQ][2];
   const f4v zero = (f4v)0.f;
   const int64_t nk = cdiv_dev(g.K, BF_BK);
   // K is a multiple of 4: each 4-wide half of a chunk is either wholly inside K or wholly out
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
   {                                                                             \
     const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                             \
     const int64_t c0 = g0 < g.K ? g0 : g.K - 4, c1 = g0 + 4 < g.K ? g0 + 4 : g.K - 4; \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                             \
+    _Pragma("unroll") for (int q = 0; q < BF_NQ; ++q) {                             \
       ra[q][0] = *reinterpret_cast<const f4v*>(pa[q] + c0);                     \
       ra[q][1] = *reinterpret_cast<const f4v*>(pa[q] + c1);                     \
       rb[q][0] = *reinterpret_cast<const f4v*>(pb[q] + c0);                     \
@@ -307,8 +309,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
   {                                                                                           \
     const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                                           \
     const bool ok0 = g0 < g.K, ok1 = g0 + 4 < g.K;                                            \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
-      const int row = srow + 32 * q;                                                          \
+    _Pragma("unroll") for (int q = 0; q < BF_NQ; ++q) {                                           \
+      const int row = srow + BF_RPP * q;                                                          \
       const bool am = m0 + row < g.M, bnn = n0 + row < g.N;                                   \
       const f4v a0 = (ok0 && am) ? ra[q][0] : zero, a1 = (ok1 && am) ? ra[q][1] : zero;       \
       const f4v b0 = (ok0 && bnn) ? rb[q][0] : zero, b1 = (ok1 && bnn) ? rb[q][1] : zero;     \
@@ -364,18 +366,22 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
 #undef NQS_BF_LOAD
 #undef NQS_BF_STORE
 #undef NQS_BF_SPLIT8
-  // two-phase epilogue as in k_gemm_nt_v2
+  // two-phase epilogue as in k_gemm_nt_v2, one 32-row block i at a time (32 operands live)
   const bool has_beta = g.beta != 0.f, has_bias = g.bias != nullptr;
-  float addv[2][2][16];
+  float bv[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < 2; ++j) {
+    const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+    bv[j] = has_bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    float addv[2][16];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) addv[i][j][r] = 0.f;
-  if (has_beta) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int r = 0; r < 16; ++r) addv[j][r] = 0.f;
+    if (has_beta) {
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -383,34 +389,28 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_bf(GemmArgs<float> g, int nb
           const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
           const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
           const int64_t rr = row < g.M ? row : g.M - 1, cc = col < g.N ? col : g.N - 1;
-          addv[i][j][r] = g.beta * g.C[rr * g.ldc + cc];
+          addv[j][r] = g.beta * g.C[rr * g.ldc + cc];
         }
-  }
-  if (has_bias) {
+    }
+    if (has_bias) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
-      const float bv = g.bias[col < g.N ? col : g.N - 1];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t row = (uint32_t)(m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3));
-          addv[i][j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv : 0.f;
+          addv[j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv[j] : 0.f;
         }
     }
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
-        const float v = g.alpha * acc[i][j][r] + addv[i][j][r];
+        const float v = g.alpha * acc[i][j][r] + addv[j][r];
         if (row < g.M && col < g.N) g.C[row * g.ldc + col] = v;
       }
+  }
 }
 
 template <int SPLIT>
