@@ -6,6 +6,8 @@
 // Cholesky (diagonal block factor + inverse in one workgroup, panel and trailing update on the
 // f64 MFMA GEMM), blocked triangular solves, one step of iterative refinement and the
 // push-through GEMV.  Also the median/MAD clip of the local energies (minsr.py:142-152).
+#include <mutex>
+
 #include "nqs_common.h"
 #include "kernels.h"
 #include "minsr.h"
@@ -747,8 +749,10 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 
 // Factor the 64 x 64 diagonal block at (j0, j0) in place (lower) and store its inverse (lower,
 // row-major) in Linv, with one wave: lane i keeps row i in registers; column k's multipliers
-// reach the other lanes by v_readlane, so no step needs a barrier.  The inverse is formed by
-// forward substitution, column c in lane c, reading L from LDS (the same address on every lane).
+// reach the other lanes by v_readlane, so no step needs a barrier.  The registers of the upper
+// triangle (k > i) are updated along with the rest but never read by another lane nor stored,
+// so the steps need no masks.  The inverse is formed by forward substitution, column c in lane c,
+// reading L and the reciprocal pivots from LDS (the same address on every lane).
 // info[0] = first non-positive pivot + 1 (the pivot is then taken as 1 and the factor goes on).
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0,
                                                    double* __restrict__ Linv, int* __restrict__ info, int64_t sA,
@@ -757,12 +761,14 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
   Linv += blockIdx.z * sL;
   info += blockIdx.z;
   __shared__ double Ls[CH_NB][CH_NB + 1];
+  __shared__ double rdiag[CH_NB];
   const int i = threadIdx.x;
   double a[CH_NB];
   double* row = A + (j0 + i) * lda + j0;
 #pragma unroll
   for (int k = 0; k < CH_NB; ++k) a[k] = row[k];
   int bad = 0;
+  double myinv = 0.0;
 #pragma unroll
   for (int k = 0; k < CH_NB; ++k) {
     double d = readlane_d(a[k], k);
@@ -771,12 +777,10 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
       d = 1.0;
     }
     const double lkk = sqrt(d), inv = 1.0 / lkk;
-    a[k] = i == k ? lkk : (i > k ? a[k] * inv : a[k]);
+    a[k] = i == k ? lkk : a[k] * inv;
+    myinv = i == k ? inv : myinv;
 #pragma unroll
-    for (int j = k + 1; j < CH_NB; ++j) {
-      const double ljk = readlane_d(a[k], j);
-      a[j] = i >= j ? a[j] - a[k] * ljk : a[j];
-    }
+    for (int j = k + 1; j < CH_NB; ++j) a[j] -= a[k] * readlane_d(a[k], j);
   }
   if (bad && i == 0 && info[0] == 0) info[0] = (int)(j0 + bad);
 #pragma unroll
@@ -785,6 +789,7 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
     row[k] = v;
     Ls[i][k] = v;
   }
+  rdiag[i] = myinv;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const int c = i;
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
     }
 #pragma unroll
     for (int k = r & ~3; k < r; ++k) s0 -= Ls[r][k] * x[k];
-    x[r] = ((s0 + s1) + (s2 + s3)) / Ls[r][r];
+    x[r] = ((s0 + s1) + (s2 + s3)) * rdiag[r];
   }
 #pragma unroll
   for (int r = 0; r < CH_NB; ++r) Linv[r * CH_NB + c] = x[r];
@@ -850,12 +855,47 @@ static GemmArgs<double> strided(GemmArgs<double> g, int64_t sA, int64_t sB, int6
   return g;
 }
 
+// The side stream of the Cholesky lookahead, one per device (default priority), and the two
+// events that order it against the caller's stream.  Calls take the mutex while they create
+// or enqueue, so concurrent callers never interleave their records and waits.
+struct CholSide {
+  hipStream_t s = nullptr;
+  hipEvent_t ready = nullptr, done = nullptr;
+};
+static std::mutex g_chol_mu;
+constexpr int kMaxDevices = 64;
+static CholSide g_chol_side[kMaxDevices];
+
+static int chol_side(CholSide** out) {
+  int dev = 0;
+  NQS_CHECK_HIP(hipGetDevice(&dev));
+  NQS_REQUIRE(dev >= 0 && dev < kMaxDevices, NQS_ERR_ARG, "device index out of range");
+  CholSide& c = g_chol_side[dev];
+  if (!c.s) {
+    NQS_CHECK_HIP(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+    NQS_CHECK_HIP(hipEventCreateWithFlags(&c.ready, hipEventDisableTiming));
+    NQS_CHECK_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+  }
+  *out = &c;
+  return NQS_OK;
+}
+
 // nb independent factorizations of the same size (system z at A + z sA, Linv + z sL,
 // panel + z sP, info + z) share every launch.
+//
+// Lookahead of one outer panel: after panel b is factored, its update of panel b + 1's columns
+// runs on the caller's stream, and its update of every column right of that (the bulk of the
+// flops) on the side stream, where it overlaps the factorization of panel b + 1 (disjoint
+// columns).  The caller's stream waits for that update before it applies panel b + 1 to the
+// columns it touched, and once more before returning.
 int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st, int nb,
              int64_t sA, int64_t sL, int64_t sP) {
   NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "Cholesky size must be a multiple of 64");
   NQS_REQUIRE(nb >= 1 && nb <= 65535, NQS_ERR_ARG, "bad batch count");
+  std::lock_guard<std::mutex> lock(g_chol_mu);
+  CholSide* side = nullptr;
+  NQS_TRY(chol_side(&side));
+  bool pending = false;  // a side-stream update the caller's stream has not waited for
   for (int64_t J0 = 0; J0 < n; J0 += CH_NBO) {
     const int64_t nbo = n - J0 < CH_NBO ? n - J0 : CH_NBO;
     for (int64_t j0 = J0; j0 < J0 + nbo; j0 += CH_NB) {
@@ -880,15 +920,30 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
                          m, CH_NB, A, lda, j0 + CH_NB, j0, sP, sA);
       NQS_LAUNCH_CHECK();
     }
-    // trailing update with the whole outer panel: A22 -= Lp Lp^T (lower tiles), K = nbo
-    const int64_t M = n - J0 - nbo;
-    if (M > 0) {
-      const double* Lp = A + (J0 + nbo) * lda + J0;
+    // trailing update with the whole outer panel, A22 -= Lp Lp^T (lower tiles), K = nbo: first the
+    // next outer panel's columns (here), then the columns right of it (side stream)
+    const int64_t J1 = J0 + nbo, M = n - J1;
+    if (M <= 0) continue;
+    const int64_t nb1 = M < CH_NBO ? M : CH_NBO, M2 = M - nb1;
+    const double* Lp = A + J1 * lda + J0;
+    if (pending) {
+      NQS_CHECK_HIP(hipStreamWaitEvent(st, side->done, 0));
+      pending = false;
+    }
+    NQS_TRY(gemm_nt<double>(strided(g64(Lp, Lp, A + J1 * lda + J1, M, nb1, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA),
+                            nb, st));
+    if (M2 > 0) {
+      const double* Lq = Lp + nb1 * lda;
+      NQS_CHECK_HIP(hipEventRecord(side->ready, st));
+      NQS_CHECK_HIP(hipStreamWaitEvent(side->s, side->ready, 0));
       NQS_TRY(gemm_nt<double>(
-          strided(g64(Lp, Lp, A + (J0 + nbo) * lda + J0 + nbo, M, M, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA), nb,
-          st));
+          strided(g64(Lq, Lq, A + (J1 + nb1) * lda + J1 + nb1, M2, M2, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA),
+          nb, side->s));
+      NQS_CHECK_HIP(hipEventRecord(side->done, side->s));
+      pending = true;
     }
   }
+  if (pending) NQS_CHECK_HIP(hipStreamWaitEvent(st, side->done, 0));
   return NQS_OK;
 }
 
@@ -896,57 +951,115 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
 // L y = b then L^T x = y, in super-blocks of TS_SB = 4 x 64 rows.  Per super-block one workgroup
 // solves the diagonal part (the four 64 x 64 blocks through their inverses Linv, with the
 // in-super-block updates between them), then a grid of workgroups applies the super-block to the
-// rest of the vector: forward, b[r] -= L[r, J0:J0+256] y[J0:J0+256] for the rows below (one wave
-// per row, lanes along the row); backward, x[c] -= sum_r L[J0 + r][c] x[J0 + r] for the columns
-// to the left (one thread per column, rows streamed).
+// rest of the vector: forward, b[r] -= L[r, J0:J0+256] y[J0:J0+256] for the rows below; backward,
+// x[c] -= sum_r L[J0 + r][c] x[J0 + r] for the columns to the left.
 #define TS_SB 256
 
+// The super-block's diagonal part touches at most ten 64 x 64 blocks: step s (s = 0..3, in solve
+// order) applies the diagonal inverse (offset g = 0) and then the off-diagonal blocks g = 1..3 - s
+// to the parts not yet solved.  16 waves = 4 groups of 4; block (s, g) belongs to group ts_grp(s, g) and
+// register slot ts_slot(s) (each group holds at most three blocks, every slot of a step is the same
+// compile-time index), and is loaded at kernel start, so the steps themselves only read LDS.
+// Thread (i = lane, p = wave & 3) of a group holds elements (i, 16p .. 16p + 15) of the block as
+// applied (row i of M for forward, column i of M for the transposed backward) and forms the partial
+// dot product of output i; the four partials are summed in LDS in a fixed order.
+__host__ __device__ constexpr int ts_grp(int s, int g) {
+  return s == 0 ? g : (s == 1 ? 1 + g : (s == 2 ? (g == 0 ? 2 : 0) : 3));
+}
+__host__ __device__ constexpr int ts_slot(int s) { return s < 2 ? s : 2; }
+// inverse of the two maps: (group, slot) -> step s (-1 when the slot is unused) and offset g
+__device__ __forceinline__ void ts_owner(int grp, int slot, int& s, int& g) {
+  s = slot == 0 ? 0 : (slot == 1 ? (grp == 0 ? -1 : 1) : (grp == 1 ? -1 : (grp == 3 ? 3 : 2)));
+  g = slot == 0 ? grp : (slot == 1 ? grp - 1 : (grp == 0 ? 1 : 0));
+}
+
+// address of element (i, k) of block (dq, tq) as applied: diagonal (tq == dq) through Linv,
+// off-diagonal through L (forward: L(tq, dq); backward: L(dq, tq) transposed)
+__device__ __forceinline__ const double* ts_elem_base(const double* A, int64_t lda, const double* Linv, int64_t J0,
+                                                      int dq, int tq, int trans, int i, int p, int64_t& step) {
+  if (dq == tq) {
+    const double* Li = Linv + ((J0 + 64 * dq) / CH_NB) * CH_NB * CH_NB;
+    step = trans ? CH_NB : 1;
+    return trans ? Li + (16 * p) * CH_NB + i : Li + i * CH_NB + 16 * p;
+  }
+  if (!trans) {
+    step = 1;
+    return A + (J0 + 64 * tq + i) * lda + J0 + 64 * dq + 16 * p;
+  }
+  step = lda;
+  return A + (J0 + 64 * dq + 16 * p) * lda + J0 + 64 * tq + i;
+}
+
 // sb = rows of this super-block (a multiple of 64, at most TS_SB; the last one may be short)
-__global__ __launch_bounds__(256) void k_trsv_super(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                    const double* __restrict__ Linv, double* __restrict__ b,
-                                                    int trans, int64_t sA, int64_t sL, int64_t sx) {
+__global__ __launch_bounds__(1024) void k_trsv_super(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
+                                                     const double* __restrict__ Linv, double* __restrict__ b,
+                                                     int trans, int64_t sA, int64_t sL, int64_t sx) {
   A += blockIdx.z * sA;
   Linv += blockIdx.z * sL;
   b += blockIdx.z * sx;
   __shared__ double x[TS_SB];
-  __shared__ double y[CH_NB];
-  const int t = threadIdx.x;
+  __shared__ double red[4][4][CH_NB];  // [group][p][i]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, grp = wave >> 2, p = wave & 3;
+  const int nblk = sb / CH_NB;
+  double m[3][16];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int s, g;
+    ts_owner(grp, k, s, g);
+    const bool have = s >= 0 && s + g < nblk;
+    const int dq = trans ? nblk - 1 - s : s, tq = trans ? dq - g : dq + g;
+    // an unused slot reads the first Linv block instead (never applied), so the loads need no branch
+    int64_t step = 1;
+    const double* e = have ? ts_elem_base(A, lda, Linv, J0, dq, tq, trans, lane, p, step) : Linv + lane;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) m[k][j] = e[j * step];
+  }
   if (t < sb) x[t] = b[J0 + t];
   __syncthreads();
-  const int nblk = sb / CH_NB;
-  for (int q = 0; q < nblk; ++q) {
-    const int blk = trans ? nblk - 1 - q : q;
-    const int o = blk * CH_NB;
-    const double* Li = Linv + ((J0 + o) / CH_NB) * CH_NB * CH_NB;
-    if (t < CH_NB) {
-      double s = 0;
-      for (int k = 0; k < CH_NB; ++k) s += (trans ? Li[k * CH_NB + t] : Li[t * CH_NB + k]) * x[o + k];
-      y[t] = s;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s >= nblk) break;
+    const int dq = trans ? nblk - 1 - s : s;
+    // phase A: x_dq <- Linv_dq (or its transpose) x_dq
+    if (grp == ts_grp(s, 0)) {
+      const double* v = x + 64 * dq + 16 * p;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc += m[ts_slot(s)][j] * v[j];
+      red[0][p][lane] = acc;
     }
     __syncthreads();
-    if (t < CH_NB) x[o + t] = y[t];
-    if (!trans) {
-      // rows below the block inside the super-block
-      if (t >= o + CH_NB && t < sb) {
-        double s = 0;
-        const double* row = A + (J0 + t) * lda + J0 + o;
-        for (int k = 0; k < CH_NB; ++k) s += row[k] * y[k];
-        x[t] -= s;
-      }
-    } else {
-      // columns left of the block inside the super-block
-      if (t < o) {
-        double s = 0;
-        for (int k = 0; k < CH_NB; ++k) s += A[(J0 + o + k) * lda + J0 + t] * y[k];
-        x[t] -= s;
-      }
-    }
+    if (t < CH_NB) x[64 * dq + t] = (red[0][0][t] + red[0][1][t]) + (red[0][2][t] + red[0][3][t]);
     __syncthreads();
+    // phase B: x_tq -= M(dq, tq) x_dq for the blocks not yet solved
+    if (s + 1 < nblk) {
+      int gb = 0;
+#pragma unroll
+      for (int g = 1; g + s < 4; ++g)
+        if (grp == ts_grp(s, g)) gb = g;
+      if (gb > 0 && s + gb < nblk) {
+        const double* v = x + 64 * dq + 16 * p;
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc += m[ts_slot(s)][j] * v[j];
+        red[gb][p][lane] = acc;
+      }
+      __syncthreads();
+      if (t < 3 * CH_NB) {
+        const int g = 1 + t / CH_NB, i = t % CH_NB;
+        if (s + g < nblk) {
+          const int tq = trans ? dq - g : dq + g;
+          x[64 * tq + i] -= (red[g][0][i] + red[g][1][i]) + (red[g][2][i] + red[g][3][i]);
+        }
+      }
+      __syncthreads();
+    }
   }
   if (t < sb) b[J0 + t] = x[t];
 }
 
-// forward update: rows r in [r0, n): b[r] -= L[r, J0:J0+sb] . b[J0:J0+sb]; 8 rows per wave
+// forward update: rows r in [r0, n): b[r] -= L[r, J0:J0+sb] . b[J0:J0+sb]; 8 rows per wave, the
+// row's loads all issued before the reductions
 __global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
                                                        int64_t r0, int64_t n, double* __restrict__ b, int64_t sA,
                                                        int64_t sx) {
@@ -957,35 +1070,52 @@ __global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict_
   z[t] = t < sb ? b[J0 + t] : 0.0;
   __syncthreads();
   const int64_t rbase = r0 + ((int64_t)blockIdx.x * 4 + wave) * 8;
+  double acc[8];
+#pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int64_t r = rbase + q;
-    if (r >= n) break;
+    const int64_t r = rbase + q < n ? rbase + q : n - 1;
     const double* row = A + r * lda + J0;
     double s = 0;
-    for (int c = lane; c < sb; c += 64) s += row[c] * z[c];
-    s = wave_sum(s);
-    if (lane == 0) b[r] -= s;
+#pragma unroll
+    for (int c4 = 0; c4 < TS_SB / 64; ++c4) {
+      const int c = lane + 64 * c4;
+      s += row[c < sb ? c : sb - 1] * z[c];  // z is zero past sb: clamped loads, no branches
+    }
+    acc[q] = s;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double s = wave_sum(acc[q]);
+    if (lane == 0 && rbase + q < n) b[rbase + q] -= s;
   }
 }
 
-// backward update: columns c < J0: x[c] -= sum_{r < sb} L[J0 + r][c] x[J0 + r]
+// backward update: columns c < J0: x[c] -= sum_{r < sb} L[J0 + r][c] x[J0 + r]; 64 columns per
+// workgroup, its four waves taking a quarter of the rows each, summed in a fixed order
 __global__ __launch_bounds__(256) void k_trsv_bwd_cols(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
                                                        double* __restrict__ x, int64_t sA, int64_t sx) {
   A += blockIdx.z * sA;
   x += blockIdx.z * sx;
   __shared__ double z[TS_SB];
-  z[threadIdx.x] = (int)threadIdx.x < sb ? x[J0 + threadIdx.x] : 0.0;
+  __shared__ double part[4][64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  z[t] = t < sb ? x[J0 + t] : 0.0;
   __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= J0) return;
-  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  for (int r = 0; r < sb; r += 4) {
-    s0 += A[(J0 + r) * lda + c] * z[r];
-    s1 += A[(J0 + r + 1) * lda + c] * z[r + 1];
-    s2 += A[(J0 + r + 2) * lda + c] * z[r + 2];
-    s3 += A[(J0 + r + 3) * lda + c] * z[r + 3];
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t cc = c < J0 ? c : J0 - 1;
+  const int rq = TS_SB / 4;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < rq; r += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int rr = wave * rq + r + u;
+      s[u] += A[(J0 + (rr < sb ? rr : sb - 1)) * lda + cc] * z[rr];  // z is zero past sb
+    }
   }
-  x[c] -= (s0 + s1) + (s2 + s3);
+  part[wave][lane] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+  __syncthreads();
+  if (wave == 0 && c < J0) x[c] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb,
@@ -995,7 +1125,7 @@ int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, doub
   const unsigned z = (unsigned)nb;
   for (int64_t J0 = 0; J0 < n; J0 += TS_SB) {
     const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 0, sA, sL, sx);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(1024), 0, st, L, lda, J0, sb, Linv, x, 0, sA, sL, sx);
     const int64_t r0 = J0 + sb;
     if (r0 < n)
       hipLaunchKernelGGL(k_trsv_fwd_rows, dim3((unsigned)cdiv(n - r0, 32), 1, z), dim3(256), 0, st, L, lda, J0, sb, r0,
@@ -1004,9 +1134,9 @@ int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, doub
   const int64_t last = ((n - 1) / TS_SB) * TS_SB;
   for (int64_t J0 = last; J0 >= 0; J0 -= TS_SB) {
     const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(TS_SB), 0, st, L, lda, J0, sb, Linv, x, 1, sA, sL, sx);
+    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(1024), 0, st, L, lda, J0, sb, Linv, x, 1, sA, sL, sx);
     if (J0 > 0)
-      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 256), 1, z), dim3(256), 0, st, L, lda, J0, sb, x, sA,
+      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 64), 1, z), dim3(256), 0, st, L, lda, J0, sb, x, sA,
                          sx);
   }
   NQS_LAUNCH_CHECK();
