@@ -33,7 +33,22 @@ __global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ O, in
   const int64_t r1 = min(half * W + W, r0 + len);
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   const T* p = O + r0 * ldo + c0;
-  for (int64_t r = r0; r < r1; ++r, p += ldo) {
+  int64_t r = r0;
+  // four rows' loads in flight per step; the sums keep the row order
+  for (; r + 4 <= r1; r += 4, p += 4 * ldo) {
+    v4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const v4*>(p + u * ldo);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double w = x ? x[r + u] : 1.0;
+      a0 += w * (double)v[u][0];
+      a1 += w * (double)v[u][1];
+      a2 += w * (double)v[u][2];
+      a3 += w * (double)v[u][3];
+    }
+  }
+  for (; r < r1; ++r, p += ldo) {
     const v4 v = *reinterpret_cast<const v4*>(p);
     const double w = x ? x[r] : 1.0;
     a0 += w * (double)v[0];
@@ -100,10 +115,13 @@ __global__ __launch_bounds__(1024) void k_mean_dot(const double* __restrict__ me
   }
 }
 
+#define CT_ROWS 128  // rows per k_center workgroup
+
 // Centring.  In place (O -= 1 mean^T per half) for the float64 / float32 Gram, which reads O;
 // for the bf16 Gram only the centred bf16 copy is written and O is left as it was (the
 // refinement and push-through apply the mean correction on the fly).  A thread takes 8
-// consecutive columns of one row (32 rows x 64 columns per workgroup).  With `blocked`, the
+// consecutive columns of one row, in turn for four groups of 32 rows (128 rows x 64 columns per
+// workgroup).  With `blocked`, the
 // bf16 copy is laid out in 256-row x 64-column blocks, [row panel][column block][256][64], each
 // 32 KB contiguous, so the Gram streams a panel slice as one contiguous read instead of 256
 // scattered 128-byte pieces.
@@ -112,39 +130,42 @@ __global__ __launch_bounds__(256) void k_center(T* __restrict__ O, int64_t ldo, 
                                                 const double* __restrict__ mean, __bf16* __restrict__ Ob,
                                                 int blocked, int inplace) {
   typedef T v4 __attribute__((ext_vector_type(4)));
-  const int64_t r = (int64_t)blockIdx.y * 32 + (threadIdx.x >> 3);
   const int64_t k0 = (int64_t)blockIdx.x * 64 + (threadIdx.x & 7) * 8;
-  if (r >= 2 * W || k0 >= ldo) return;
-  const int half = r >= W ? 1 : 0;
-  const double* mrow = mean + (int64_t)half * P;
-  T* orow = O + r * ldo + k0;
-  v4 x0 = *reinterpret_cast<const v4*>(orow), x1 = *reinterpret_cast<const v4*>(orow + 4);
-  float v[8];
+  if (k0 >= ldo) return;
+  const int64_t nkb = ldo / 64;
+  // the 8 means of this thread's columns for both halves (zero past P: padded columns stay zero)
+  double mu[2][8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int64_t k = k0 + e;
-    const T raw = e < 4 ? x0[e & 3] : x1[e & 3];
-    const T c = k < P ? (T)((double)raw - mrow[k < P ? k : 0]) : (T)0;
-    v[e] = (float)c;
-    if (e < 4) x0[e & 3] = c;
-    else x1[e & 3] = c;
-  }
-  if (inplace) {
-    *reinterpret_cast<v4*>(orow) = x0;
-    *reinterpret_cast<v4*>(orow + 4) = x1;
-  }
-  if (!Ob) return;
-  bf8v b;
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-  for (int e = 0; e < 8; ++e) b[e] = (__bf16)v[e];
-  int64_t off;
-  if (blocked) {
-    const int64_t nkb = ldo / 64;
-    off = (((r >> 8) * nkb + (k0 >> 6)) * 256 + (r & 255)) * 64 + (k0 & 63);
-  } else {
-    off = r * ldo + k0;
+    for (int e = 0; e < 8; ++e) mu[h][e] = k0 + e < P ? mean[(int64_t)h * P + k0 + e] : 0.0;
+#pragma unroll
+  for (int q = 0; q < CT_ROWS / 32; ++q) {
+    const int64_t r = (int64_t)blockIdx.y * CT_ROWS + q * 32 + (threadIdx.x >> 3);
+    if (r >= 2 * W) break;
+    const int half = r >= W ? 1 : 0;
+    T* orow = O + r * ldo + k0;
+    v4 x0 = *reinterpret_cast<const v4*>(orow), x1 = *reinterpret_cast<const v4*>(orow + 4);
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const T raw = e < 4 ? x0[e & 3] : x1[e & 3];
+      const T c = k0 + e < P ? (T)((double)raw - mu[half][e]) : (T)0;
+      v[e] = (float)c;
+      if (e < 4) x0[e & 3] = c;
+      else x1[e & 3] = c;
+    }
+    if (inplace) {
+      *reinterpret_cast<v4*>(orow) = x0;
+      *reinterpret_cast<v4*>(orow + 4) = x1;
+    }
+    if (!Ob) continue;
+    bf8v b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) b[e] = (__bf16)v[e];
+    const int64_t off = blocked ? (((r >> 8) * nkb + (k0 >> 6)) * 256 + (r & 255)) * 64 + (k0 & 63) : r * ldo + k0;
+    *reinterpret_cast<bf8v*>(Ob + off) = b;
   }
-  *reinterpret_cast<bf8v*>(Ob + off) = b;
 }
 
 bool gram_bf16_blocked(int64_t n, int64_t K) { return n % 256 == 0 && K % 64 == 0; }
@@ -163,8 +184,8 @@ int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* 
   NQS_LAUNCH_CHECK();
   if (!Ob && !inplace) return NQS_OK;
   const int blocked = gram_bf16_blocked(2 * W, ldo) ? 1 : 0;
-  hipLaunchKernelGGL(k_center<T>, dim3((unsigned)cdiv(ldo, 64), (unsigned)cdiv(2 * W, 32)), dim3(256), 0, st, O, ldo,
-                     W, P, mean, Ob, blocked, inplace);
+  hipLaunchKernelGGL(k_center<T>, dim3((unsigned)cdiv(ldo, 64), (unsigned)cdiv(2 * W, CT_ROWS)), dim3(256), 0, st,
+                     O, ldo, W, P, mean, Ob, blocked, inplace);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }
