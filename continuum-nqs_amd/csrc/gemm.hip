@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_v2(GemmArgs<T> g) {
 // (both round-to-nearest-even) as it is staged into LDS; SPLIT = 3 accumulates
 // hi*hi + hi*lo + lo*hi in float32 (products carry about 16 significant bits; the dropped lo*lo
 // term is below 2^-16 relative), SPLIT = 1 keeps hi*hi only (plain bf16).  Block tile
-// 128 x 128 x 64, 4 waves in 2 x 2, 64 x 64 per wave as 2 x 2 blocks of
+// 128 x 128 x 32, 4 waves in 2 x 2, 64 x 64 per wave as 2 x 2 blocks of
 // v_mfma_f32_32x32x16_bf16 (A[lane%32][8 (lane/32) + e], C row 8 (r/4) + 4 (lane/32) + r%4,
 // col lane%32).  Tiles are dealt XCD-aware: workgroup b runs on XCD b % 8, and the column
 // tiles of one row panel go to the same XCD back to back, so the panel is read from HBM once.
@@ -246,9 +246,10 @@ constexpr size_t bf_lds_bytes() {
   return sizeof(__bf16) * (size_t)2 * (SPLIT == 1 ? 1 : 2) * BF_BM * BF_LD;
 }
 
-// Staging: a thread moves rows tid / 8 + 32 q (q = 0..3), k-chunk tid % 8 (8 floats, two 16-byte
-// loads) and writes its 8 hi and 8 lo bf16 as 16-byte LDS stores; 8 lanes fill one 128-byte row,
-// and the 144-byte row pitch keeps both the stores and the fragment reads free of bank conflicts.
+// Staging: a thread moves rows tid / 4 + 64 q (q = 0..1), k-chunk tid % 4 (8 floats, two 16-byte
+// loads) and writes its 8 hi and 8 lo bf16 as 16-byte LDS stores; 4 lanes fill one 64-byte row,
+// stored at an 80-byte row pitch.  Three workgroups share a CU (the epilogue keeps one 32-row
+// block of addends live at a time, so the kernel fits 168 registers without spills).
 // One LDS buffer per operand and plane; the registers carry stage kb + 1 across the loop back
 // edge (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
 template <int SPLIT>
