@@ -238,6 +238,158 @@ def bench_c5(args):
             "roofline": None, "cpu_baseline": None}
 
 
+def _gpu_ms(fn, reps=3):
+    """Mean milliseconds per call of fn() over reps calls after one warm-up call (HIP events)."""
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _cpu_s(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def bench_next(args):
+    """The rows next to the step, at the headline workload (N = 16, 8192 walkers, mixed): the
+    first-order Adam step (surrogate gradient as one column GEMV over O, Adam in place, and a whole
+    train step), the spin estimators, E_loc with a skyrmion texture field, and the overlap /
+    rotation estimators.  Each row carries its GPU time, the roofline of the kernel that is its own
+    (the rest is log psi / E_loc work measured on the main line) and the oracle timed on a few
+    walkers of the same workload."""
+    from nqs.psiformer import LogPsi
+    from nqs.local_energy import make_local_energy
+    from nqs.ewald import make_ewald
+    from nqs.minsr import grad_params
+    from nqs.observables import make_spin_estimators
+    from nqs.manybody_chern import mc_overlap, ratio_stats
+    from nqs.topology import rotation_eigenvalue_vmc
+    from nqs import train as ntr, pinning as npn
+    from oracle import psiformer as opf, train as otr, observables as oob, local_energy as ole, pinning as opn
+
+    W = args.walkers
+    sysd = build_system("mixed", W, gemm=args.gemm)
+    model, L, R, S = sysd["model"], sysd["L"], sysd["R"], sysd["S"]
+    theta = sysd["theta"].cuda().contiguous()
+    _, _, coul = make_ewald(L, 1.0, n_real=4, n_recip=8)
+    lp = LogPsi(model, theta)
+    HBM = 8.0e12
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    arch = opf.Arch(N_E, sysd["Gs"], sysd["Wenv"], L, **ARCH)
+    th_cpu = sysd["theta"].cpu()
+    olp = opf.make_logpsi(arch, th_cpu)
+    Rc, Sc = R[:4].cpu().numpy(), S[:4].cpu().numpy().astype(np.int64)
+    rows = {}
+
+    # f1: surrogate gradient + Adam + whole first-order step
+    E = make_local_energy(lp, 1.0, LAM, HZ)
+    e = E(R, S) + coul(R)
+    O, _ = grad_params(lp, R, S)
+    t_g = _gpu_ms(lambda: ntr.vmc_gradient(O, e))
+    g_bytes = float(O.data.shape[0]) * O.data.shape[1] * O.data.element_size()
+    opt = ntr.Adam(1e-3)
+    th2 = theta.clone()
+    st = opt.init(th2)
+    part = opt.partner(model)
+    g = ntr.vmc_gradient(O, e)
+    P = theta.shape[0]
+    t_a = _gpu_ms(lambda: opt.update_(th2, g, st, part), reps=10)
+    a_bytes = 64.0 * P  # theta, g, mu, nu, partner read; theta, mu, nu written (8 bytes each)
+    del O
+    torch.cuda.empty_cache()
+    optm, step = ntr.make_train_step(model, L, N_E, 1.0, LAM, HZ, coul, ntr.Adam(1e-3), n_sweeps=args.sweeps,
+                                     step_size=0.3 * RS)
+    th3 = theta.clone()
+    ost = optm.init(th3)
+    R2, S2 = R.clone(), S.clone()
+    step(th3, ost, R2, S2, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        th3, ost, R2, S2 = step(th3, ost, R2, S2, 2 + k)[:4]
+    torch.cuda.synchronize()
+    t_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    nw = 2
+    c_g = _cpu_s(lambda: otr.surrogate_gradient(arch, th_cpu, Rc[:nw], Sc[:nw], e[:nw].cpu().numpy()))
+    c_a = _cpu_s(lambda: otr.adam_step(arch, th_cpu.numpy(), g.cpu().numpy(), None))
+    rows["f1_adam_step"] = {
+        "train_step_ms": t_step, "walker_steps_per_s": W / (t_step / 1e3),
+        "vmc_gradient_ms": t_g, "adam_update_ms": t_a,
+        "roofline": {"kernel": "nqs_vmc_gradient (column GEMV over O)", "bound": "hbm",
+                     "achieved": g_bytes / (t_g / 1e3) / 1e9, "peak": HBM / 1e9, "unit": "GB/s",
+                     "frac": g_bytes / (t_g / 1e3) / HBM, "algorithmic_bytes": g_bytes},
+        "adam_roofline": {"bound": "hbm", "achieved": a_bytes / (t_a / 1e3) / 1e9, "peak": HBM / 1e9,
+                          "unit": "GB/s", "frac": a_bytes / (t_a / 1e3) / HBM, "algorithmic_bytes": a_bytes},
+        "cpu": {"surrogate_gradient_s_per_walker": c_g / nw, "adam_step_s": c_a, "cores": cores, "kind": "port"}}
+
+    # f2: spin estimators (log psi of every walker and of its N single-spin flips)
+    est = make_spin_estimators(lp)
+    t_s = _gpu_ms(lambda: est(R, S))
+    c_s = _cpu_s(lambda: [oob.spin_estimators(olp, Rc[w], Sc[w]) for w in range(nw)])
+    rows["f2_spin_estimators"] = {
+        "ms": t_s, "walkers_per_s": W / (t_s / 1e3), "configurations_per_s": W * (N_E + 1) / (t_s / 1e3),
+        "roofline": None, "note": "N + 1 log psi forwards per walker; their kernels are the log psi kernels of the "
+                                  "main line",
+        "cpu": {"walkers_per_s": nw / c_s, "cores": cores, "kind": "port"}}
+
+    # f3: E_loc with the skyrmion texture (ext_field) against plain E_loc
+    q1 = np.array([2 * np.pi / L, 0.0])
+    q2 = np.array([-np.pi / L, np.sqrt(3) * np.pi / L])
+    Et = make_local_energy(lp, 1.0, LAM, HZ, ext_field=npn.make_pin("skx", q1, q2, 0.3))
+    t_e0 = _gpu_ms(lambda: E(R, S), reps=2)
+    t_e1 = _gpu_ms(lambda: Et(R, S), reps=2)
+    Eo = ole.make_local_energy(olp, 1.0, LAM, HZ, ext_field=opn.make_pin("skx", q1, q2, 0.3))
+    c_e = _cpu_s(lambda: [Eo(torch.tensor(Rc[w]), torch.tensor(Sc[w])) for w in range(1)])
+    rows["f3_texture_eloc"] = {
+        "eloc_ms": t_e1, "plain_eloc_ms": t_e0, "texture_overhead_ms": t_e1 - t_e0,
+        "walkers_per_s": W / (t_e1 / 1e3), "roofline": None,
+        "note": "the texture is a Fourier table evaluated inside the E_loc physics kernel",
+        "cpu": {"walkers_per_s": 1.0 / c_e, "cores": cores, "kind": "port"}}
+
+    # f4: overlap of two states and the C4 rotation eigenvalue
+    rng = np.random.default_rng(5)
+    lpb = LogPsi(model, (theta + 1e-3 * torch.tensor(rng.standard_normal(P), device=theta.device)).contiguous())
+    t_o = _gpu_ms(lambda: mc_overlap(lp, lpb, R, S))
+    t_r = _gpu_ms(lambda: rotation_eigenvalue_vmc(lp, R, S, 4, L))
+    la, lb = lp(R, S), lpb(R, S)
+    t_rs = _gpu_ms(lambda: ratio_stats(la, lb), reps=10)
+    olpb = opf.make_logpsi(arch, lpb.theta.cpu() if hasattr(lpb, "theta") else th_cpu)
+    def cpu_overlap():
+        a = [complex(olp(torch.tensor(Rc[w]), torch.tensor(Sc[w]))) for w in range(nw)]
+        b = [complex(olpb(torch.tensor(Rc[w]), torch.tensor(Sc[w]))) for w in range(nw)]
+        oob.ratio_stats(a, b)
+    c_o = _cpu_s(cpu_overlap)
+    rs_bytes = 32.0 * W
+    rows["f4_overlap_rotation"] = {
+        "mc_overlap_ms": t_o, "rotation_eigenvalue_ms": t_r, "ratio_stats_ms": t_rs,
+        "walkers_per_s": W / (t_o / 1e3),
+        "roofline": {"kernel": "nqs_ratio_stats", "bound": "hbm", "achieved": rs_bytes / (t_rs / 1e3) / 1e9,
+                     "peak": HBM / 1e9, "unit": "GB/s", "frac": rs_bytes / (t_rs / 1e3) / HBM,
+                     "algorithmic_bytes": rs_bytes, "note": "launch-bound at this size (256 KB)"},
+        "cpu": {"walkers_per_s": nw / c_o, "cores": cores, "kind": "port"}}
+
+    f1 = rows["f1_adam_step"]
+    return {"metric": "first-order (Adam) VMC walker-steps/s at N=16, and the next rows' kernels",
+            "value": f1["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": f1["train_step_ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+bf16+f64", "data": "synthetic",
+            "config": {"workload": "N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers=8192",
+                       "walkers": W, "params": P, "sweeps": args.sweeps},
+            "roofline": dict(f1["roofline"], traffic=None), "rows": rows,
+            "cpu_baseline": {"value": 1.0 / (f1["cpu"]["surrogate_gradient_s_per_walker"]), "unit": "walkers/s",
+                             "cores": cores, "kind": "port",
+                             "sample": f"the oracle's surrogate gradient (reverse mode) on {nw} walkers; each row "
+                                       "carries its own oracle timing"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,9 +405,10 @@ def main():
                     help="dense-layer MFMA mode of the mixed backbone")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps from a captured HIP graph")
-    ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5"],
+    ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5", "next"],
                     help="c3: the headline N=16 workload (default); c4: the N=32 large configuration; c5: the "
-                         "24-system uneven ensemble (each prints its own line)")
+                         "24-system uneven ensemble; next: the Adam step, spin estimators, texture E_loc and "
+                         "overlap / rotation estimators at the c3 workload (each prints its own line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the calibration kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -276,6 +429,9 @@ def main():
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args)))
+        return
+    if args.config == "next":
+        print(json.dumps(bench_next(args)))
         return
     from nqs import _lib
     from nqs.ewald import make_ewald
