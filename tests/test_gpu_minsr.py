@@ -98,6 +98,19 @@ def test_minsr_matches_oracle_and_paramsr(lam):
     assert float(torch.max(torch.abs(d_gpu_ps - d_ps))) / den < 1e-6
 
 
+@pytest.mark.parametrize("W", [160, 224, 416])
+def test_minsr_short_last_super_block(W):
+    """2W = 320, 448, 832: the triangular solves run full 256-row super-blocks and then a last one of
+    one, three and one 64-row blocks; the Cholesky's last outer panel is short too.  The float64
+    direction against the oracle's LU solve."""
+    O, eps = _minsr_case(W=W, seed=6)
+    d = nms.minsr_direction(O, eps, 1e-4, gram="fp64").cpu()
+    assert int(nms.minsr_direction.last_info.item()) == 0
+    d_lu = oms.minsr_direction(O.cpu(), eps, 1e-4)
+    den = float(torch.max(torch.abs(d_lu)))
+    assert float(torch.max(torch.abs(d - d_lu))) / den < 1e-6
+
+
 @pytest.mark.parametrize("gram,tol,W", [("fp32", 1e-3, 64), ("bf16", 5e-2, 64), ("fp32", 1e-3, 384),
                                         ("bf16", 5e-2, 384)])
 def test_low_precision_gram_close_to_fp64(gram, tol, W):
