@@ -361,7 +361,7 @@ def bench_next(args):
     t_r = _gpu_ms(lambda: rotation_eigenvalue_vmc(lp, R, S, 4, L))
     la, lb = lp(R, S), lpb(R, S)
     t_rs = _gpu_ms(lambda: ratio_stats(la, lb), reps=10)
-    olpb = opf.make_logpsi(arch, lpb.theta.cpu() if hasattr(lpb, "theta") else th_cpu)
+    olpb = opf.make_logpsi(arch, lpb.theta.cpu())
     def cpu_overlap():
         a = [complex(olp(torch.tensor(Rc[w]), torch.tensor(Sc[w]))) for w in range(nw)]
         b = [complex(olpb(torch.tensor(Rc[w]), torch.tensor(Sc[w]))) for w in range(nw)]
