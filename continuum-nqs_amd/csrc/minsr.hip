@@ -917,6 +917,16 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
   CholSide* side = nullptr;
   NQS_TRY(chol_side(&side));
   bool pending = false;  // a side-stream update the caller's stream has not waited for
+  // on every return, including an early error return, the caller's stream ends up ordered after
+  // the side stream's last update, so no update is left writing into A behind the caller's back
+  struct Join {
+    hipStream_t st;
+    CholSide* side;
+    bool* pending;
+    ~Join() {
+      if (*pending) (void)hipStreamWaitEvent(st, side->done, 0);
+    }
+  } join{st, side, &pending};
   for (int64_t J0 = 0; J0 < n; J0 += CH_NBO) {
     const int64_t nbo = n - J0 < CH_NBO ? n - J0 : CH_NBO;
     for (int64_t j0 = J0; j0 < J0 + nbo; j0 += CH_NB) {
@@ -964,7 +974,10 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
       pending = true;
     }
   }
-  if (pending) NQS_CHECK_HIP(hipStreamWaitEvent(st, side->done, 0));
+  if (pending) {
+    pending = false;
+    NQS_CHECK_HIP(hipStreamWaitEvent(st, side->done, 0));
+  }
   return NQS_OK;
 }
 
