@@ -95,6 +95,62 @@ __global__ __launch_bounds__(256) void k_wgrad_cols(WgradArgs<T> a, int ib) {
   }
 }
 
+// Float variant of the column form with two adjacent columns per thread and 8-byte stores, the
+// workgroup sized to the layer width (a 128-wide layer takes one wave, not half of four).  Taken
+// when every segment offset, the segment width and the strides are even, so a column pair never
+// straddles two leaves and the stores stay 8-byte aligned.  Each column's sum runs in the same
+// order as in k_wgrad_cols.
+__global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  extern __shared__ char smem_raw[];
+  float* Xt = reinterpret_cast<float*>(smem_raw);  // [ib][NP]
+  const int N = a.N, NP = (N + 3) & ~3;
+  const int64_t cfg = blockIdx.y >> 1;
+  const int c = blockIdx.y & 1;
+  const int i0 = blockIdx.z * ib;
+  const int nrow = min(ib, a.in - i0);
+  const float* X = a.X + cfg * (int64_t)N * a.ldx;
+  for (int e = threadIdx.x; e < ib * NP; e += blockDim.x) {
+    const int ii = e / NP, t = e % NP;
+    Xt[e] = (t < N && ii < nrow) ? X[(int64_t)t * a.ldx + i0 + ii] : 0.f;
+  }
+  __syncthreads();
+  const int j = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= a.out) return;
+  const float* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg + j;
+  float g0[32], g1[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    const f2 v = t < N ? *reinterpret_cast<const f2*>(G + (int64_t)t * a.ldg) : (f2)0.f;
+    g0[t] = v[0];
+    g1[t] = v[1];
+  }
+  const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
+  const int sg = j / a.seg_w, jj = j % a.seg_w;
+  float* O = a.O + row * a.ldo + a.seg_base[sg] + jj;
+  for (int ii = 0; ii < nrow; ++ii) {
+    const float* xr = Xt + ii * NP;
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+    for (int t = 0; t < 32; t += 4) {
+      if (t < NP) {
+        s0 += xr[t] * g0[t] + xr[t + 1] * g0[t + 1] + xr[t + 2] * g0[t + 2] + xr[t + 3] * g0[t + 3];
+        s1 += xr[t] * g1[t] + xr[t + 1] * g1[t + 1] + xr[t + 2] * g1[t + 2] + xr[t + 3] * g1[t + 3];
+      }
+    }
+    *reinterpret_cast<f2*>(O + (int64_t)(i0 + ii) * a.seg_w) = (f2){s0, s1};
+  }
+}
+
+template <typename T>
+static bool wgrad_pairs_ok(const WgradArgs<T>& a) {
+  if (sizeof(T) != 4 || a.seg_w % 2 || a.ldg % 2 || a.ldo % 2 || ((uintptr_t)a.O & 7) || ((uintptr_t)a.G & 7))
+    return false;
+  for (int s = 0; s < a.out / a.seg_w; ++s)
+    if (a.seg_base[s] % 2) return false;
+  return true;
+}
+
 // Output columns go in chunks of ob: all of them when the N x out cotangent tile takes at most
 // 96 KB of LDS, else the largest multiple of 64 that keeps the tile within 64 KB.
 template <typename T>
@@ -105,6 +161,16 @@ int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
     const int NP = (a.N + 3) & ~3;
     int ib = (int)std::min<int64_t>(a.in, (48 * 1024) / ((int64_t)NP * sizeof(T)));
     const size_t lds = sizeof(T) * (size_t)ib * NP;
+    if constexpr (sizeof(T) == 4) {
+      if (wgrad_pairs_ok(a)) {
+        const int threads = (int)std::min<int64_t>(256, round_up(cdiv(a.out, 2), 64));
+        hipLaunchKernelGGL(k_wgrad_cols2, dim3((unsigned)cdiv(a.out, 2 * threads), (unsigned)(a.n_cfg * 2),
+                                               (unsigned)cdiv(a.in, ib)),
+                           dim3(threads), lds, st, a, ib);
+        NQS_LAUNCH_CHECK();
+        return NQS_OK;
+      }
+    }
     hipLaunchKernelGGL(k_wgrad_cols<T>, dim3((unsigned)cdiv(a.out, 256), (unsigned)(a.n_cfg * 2),
                                              (unsigned)cdiv(a.in, ib)),
                        dim3(256), lds, st, a, ib);
