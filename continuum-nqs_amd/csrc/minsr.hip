@@ -493,10 +493,14 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc
 #pragma unroll
       for (int j = 0; j < 8; ++j)
         b[j] = *reinterpret_cast<const bf8v*>(base + 32768 + (wn * 128 + j * 16 + (lane & 15)) * 128 + slot);
+      // raised wave priority while this wave issues its MFMA burst: the waves that are between
+      // bursts (LDS reads, staging) yield issue slots to it
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_s_setprio(0);
     }
     // this wave's loads of stage kb + 1 have landed; the barrier makes every wave's visible and
     // ends the reads of stage kb
