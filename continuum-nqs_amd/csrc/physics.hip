@@ -229,7 +229,9 @@ __global__ void k_mcmc_accept_pos(int64_t W, int N, double* __restrict__ R, cons
   }
 }
 
-// Spin flip proposal: S' = S with s_i flipped (stored in Sp); accept as above.
+// Spin flip proposal: S' = S with s_i flipped (stored in Sp); accept as above.  An index outside
+// [0, N) proposes nothing (Sp = S) and the accept kernel rejects it, so a caller's bad stream can
+// neither write outside the walker's spins nor count as a move.
 __global__ void k_mcmc_flip_propose(int64_t W, int N, const int32_t* __restrict__ S,
                                     const int32_t* __restrict__ fi, int32_t* __restrict__ Sp) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -237,7 +239,7 @@ __global__ void k_mcmc_flip_propose(int64_t W, int N, const int32_t* __restrict_
   const int64_t w = q / N;
   const int i = (int)(q % N);
   const int s = S[q];
-  Sp[q] = (i == fi[w]) ? 1 - s : s;
+  Sp[q] = (i == fi[w]) ? 1 - s : s;   // never true for fi[w] outside [0, N)
 }
 
 __global__ void k_mcmc_accept_spin(int64_t W, int N, int32_t* __restrict__ S, const int32_t* __restrict__ fi,
@@ -247,8 +249,9 @@ __global__ void k_mcmc_accept_spin(int64_t W, int N, int32_t* __restrict__ S, co
   if (w >= W) return;
   const double lpn = 2.0 * lp_new[2 * w];
   const double d = lpn - logp[w];
+  const int i = fi[w];
+  if ((unsigned)i >= (unsigned)N) return;   // out-of-range proposal: rejected
   if (u[w] < exp(d > 0.0 ? 0.0 : d)) {
-    const int i = fi[w];
     S[w * N + i] = 1 - S[w * N + i];
     logp[w] = lpn;
     if (acc) acc[w] += 1;

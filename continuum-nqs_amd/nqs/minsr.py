@@ -191,6 +191,13 @@ class VMCStep:
         self.timers = None
         self.graph = graph      # replay the sweeps from a captured HIP graph (nqs.graphs.GraphSweeps)
         self._gs = None
+        self.failed = torch.zeros((), dtype=torch.int64, device=_lib.device())   # skipped updates
+
+    @property
+    def failed_steps(self):
+        """Number of steps whose update was skipped (non-positive-definite Gram or non-finite
+        direction); reading it synchronises with the device."""
+        return int(self.failed.item())
 
     def _ev(self):
         e = torch.cuda.Event(enable_timing=True)
@@ -222,10 +229,12 @@ class VMCStep:
         d = minsr_direction(self.O, e_clip, self.lam, self.gram, self.refine)
         info = minsr_direction.last_info
         if t: t.append(self._ev())
-        if int(info.item()) != 0 or not bool(torch.isfinite(d).all()):
-            # the shifted Gram was not positive definite in this precision: no update this step
-            self.failed_steps = getattr(self, "failed_steps", 0) + 1
-            d = torch.zeros_like(d)
+        # A step whose shifted Gram was not positive definite in this precision (info != 0) or whose
+        # direction is not finite leaves theta unchanged; the decision stays on the device (no host
+        # sync) and is counted in self.failed (device int64), which train_minsr reports.
+        ok = (info[0] == 0) & torch.isfinite(d).all()
+        d = torch.where(ok, d, torch.zeros_like(d))
+        self.failed += (~ok).to(torch.int64)
         self.theta -= self.lr * d
         if self.graph:
             # same buffers: the captured sweeps keep reading them
@@ -266,14 +275,20 @@ def train_minsr(model, params, L, n_elec, key, *, m_star=1.0, lambda_r=1.0, h_z=
     if R0 is None and burn > 0:
         st.chain.run(burn, step_size, 2, key=seed)
     hist = history if history is not None else {"E": [], "Eerr": []}
+    hist.setdefault("failed_steps", [])
     for it in range(n_steps):
         stats = st.step(key=seed + 1000 + it)
         s = stats.cpu().numpy()
         hist["E"].append(float(s[0]))
         hist["Eerr"].append(float(s[1]))
+        hist["failed_steps"].append(st.failed_steps)
         if log_every and (it % log_every == 0 or it == n_steps - 1):
             print(f"    minsr {it:4d}: E={s[0]:+.6f} +/- {s[1]:.1e} acc={float(st.last['acc'].mean()):.2f}",
                   flush=True)
+    if st.failed_steps:
+        import warnings
+        warnings.warn(f"train_minsr: {st.failed_steps} of {n_steps} updates skipped (the shifted Gram was not "
+                      f"positive definite in the {st.gram} precision, or the direction was not finite)")
     out = unravel(st.theta.cpu())
     if return_walkers:
         return out, hist, st.chain.R, st.chain.S

@@ -54,8 +54,6 @@ class Chain:
         xi, up, fi, uf = _dev_streams(st, dev)
         assert xi.shape == (n_sweeps, W, N, 2) and up.shape == (n_sweeps, W)
         assert fi.shape == (n_sweeps, n_spin_flips, W) and uf.shape == (n_sweeps, n_spin_flips, W)
-        if n_spin_flips and (int(fi.min()) < 0 or int(fi.max()) >= N):
-            raise ValueError("flip indices out of range")
         acc_p = torch.zeros(W, dtype=torch.int32, device=dev)
         acc_s = torch.zeros(W, dtype=torch.int32, device=dev)
         m = self.logpsi.model

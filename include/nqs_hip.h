@@ -160,7 +160,8 @@ int nqs_ewald(const nqs_ewald_desc* e, const double* R_dev, int64_t n_walkers, i
 /* ---- Metropolis sweeps over positions and spins with caller-supplied streams
  * xi (n_sweeps, W, N, 2) f64, u_pos (n_sweeps, W) f64, flip_i (n_sweeps, n_flip, W) i32,
  * u_flip (n_sweeps, n_flip, W) f64.  R, S, logp (W) = 2 Re log psi of the current state, are
- * updated in place; acc_pos (W), acc_spin (W) count accepted moves (int32, accumulated). */
+ * updated in place; acc_pos (W), acc_spin (W) count accepted moves (int32, accumulated).  A flip
+ * index outside [0, N) is a rejected proposal (no spin changes, no count). */
 int nqs_mcmc_sweeps(const nqs_model_desc* m, const void* packed_dev, double* R_dev, int32_t* S_dev,
                     double* logp_dev, int64_t n_walkers, int32_t n_sweeps, int32_t n_flip,
                     double step, const double* xi_dev, const double* u_pos_dev,

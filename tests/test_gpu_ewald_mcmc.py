@@ -117,6 +117,32 @@ def test_metropolis_decisions_bit_exact():
     assert np.array_equal(ch.R.cpu().numpy(), R)          # bit for bit
 
 
+def test_metropolis_out_of_range_flip_indices_are_rejected():
+    """Flip indices outside [0, N) in a caller's stream are rejected proposals inside the kernels (no
+    host-side check runs): the result equals the oracle's with those proposals forced to reject."""
+    N, L, arch, theta, lp = _mcmc_case("base")
+    Wk, ns, nf = 32, 2, 2
+    rng = np.random.default_rng(17)
+    R0 = rng.uniform(0, L, (Wk, N, 2))
+    S0 = rng.integers(0, 2, (Wk, N))
+    st = osa.make_streams(ns, Wk, N, nf, seed=18)
+    bad = rng.random(st["flip_i"].shape) < 0.4
+    st["flip_i"] = np.where(bad, rng.choice([-1, N, N + 7, -(2 ** 31)], st["flip_i"].shape), st["flip_i"])
+    ref = dict(st, flip_i=np.where(bad, 0, st["flip_i"]), u_flip=np.where(bad, 2.0, st["u_flip"]))
+    logpsi_b = lambda R, S: opf.batched_logpsi(arch, theta, torch.tensor(R), torch.tensor(S)).numpy()
+    R, S, lpc = R0.copy(), S0.copy(), None
+    accs_o = np.zeros(Wk, int)
+    for s in range(ns):
+        R, S, _, asp, lpc = osa.sweep(logpsi_b, R, S, np.eye(2) * L, 0.5, ref["xi"][s], ref["u_pos"][s],
+                                      ref["flip_i"][s], ref["u_flip"][s], lp_cur=lpc)
+        accs_o += asp.sum(0)
+    ch = Chain(lp, R0, S0)
+    _, asp = ch.run(ns, 0.5, nf, streams=st)
+    assert np.array_equal(np.rint(asp.cpu().numpy() * ns * nf).astype(int), accs_o)
+    assert np.array_equal(ch.S.cpu().numpy(), S)
+    assert np.array_equal(ch.R.cpu().numpy(), R)
+
+
 def test_sampler_density_uniform_and_energy():
     """test_nqs_sampler.py:31-56 on the HIP path (plane-wave determinant: uniform density)."""
     from nqs.local_energy import make_local_energy

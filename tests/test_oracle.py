@@ -102,7 +102,7 @@ def test_oracle_psiformer_zero_backflow_is_slater_and_antisymmetric():
 
 
 def test_oracle_library_ops_match_torch_definitions():
-    """LayerNorm / tanh-GELU / softmax restatements against torch.nn's implementations of the same
+    """LayerNorm / tanh-GELU restatements against torch.nn.functional's implementations of the same
     definitions (LayerNorm: biased variance; eps 1e-6)."""
     x = torch.randn(7, 16, dtype=torch.float64)
     p = {"n/scale": torch.randn(16, dtype=torch.float64), "n/bias": torch.randn(16, dtype=torch.float64)}
@@ -110,6 +110,136 @@ def test_oracle_library_ops_match_torch_definitions():
     assert torch.max(torch.abs(opf.layer_norm(x, p, "n") - ref)) < 1e-12
     ref = torch.nn.functional.gelu(x, approximate="tanh")
     assert torch.max(torch.abs(opf.gelu_tanh(x) - ref)) < 1e-14
+
+
+def _torch_block(p, b, d, h):
+    """torch.nn.TransformerEncoderLayer with the reference block's weights copied in: pre-norm
+    (norm_first), eps 1e-6, feed-forward d -> 2d -> d with tanh-GELU, no dropout.  torch's
+    layer splits heads as contiguous slices of width d/h and scales QK^T by (d/h)^-1/2."""
+    gelu = lambda t: torch.nn.functional.gelu(t, approximate="tanh")
+    layer = torch.nn.TransformerEncoderLayer(d, h, dim_feedforward=2 * d, dropout=0.0, activation=gelu,
+                                             layer_norm_eps=1e-6, batch_first=True, norm_first=True,
+                                             dtype=torch.float64)
+    a = b + "/attn"
+    with torch.no_grad():
+        layer.self_attn.in_proj_weight.copy_(torch.cat([p[a + "/q/kernel"].T, p[a + "/k/kernel"].T,
+                                                        p[a + "/v/kernel"].T]))
+        layer.self_attn.in_proj_bias.copy_(torch.cat([p[a + "/q/bias"], p[a + "/k/bias"], p[a + "/v/bias"]]))
+        layer.self_attn.out_proj.weight.copy_(p[a + "/o/kernel"].T)
+        layer.self_attn.out_proj.bias.copy_(p[a + "/o/bias"])
+        layer.linear1.weight.copy_(p[b + "/Dense_0/kernel"].T)
+        layer.linear1.bias.copy_(p[b + "/Dense_0/bias"])
+        layer.linear2.weight.copy_(p[b + "/Dense_1/kernel"].T)
+        layer.linear2.bias.copy_(p[b + "/Dense_1/bias"])
+        layer.norm1.weight.copy_(p[b + "/LayerNorm_0/scale"])
+        layer.norm1.bias.copy_(p[b + "/LayerNorm_0/bias"])
+        layer.norm2.weight.copy_(p[b + "/LayerNorm_1/scale"])
+        layer.norm2.bias.copy_(p[b + "/LayerNorm_1/bias"])
+    return layer.eval()
+
+
+@pytest.mark.parametrize("d,h,depth", [(128, 8, 4), (32, 4, 2), (16, 4, 0)])
+def test_oracle_backbone_matches_torch_transformer_layers(d, h, depth):
+    """The backbone (src/nqs/psiformer.py:67-81,118-136) against torch's own library layers with the
+    same weights: embed Dense, then per block torch.nn.TransformerEncoderLayer(norm_first=True), then
+    torch.nn.LayerNorm.  Post-norm placement, a missing final LayerNorm, interleaved heads, erf-GELU
+    or a different attention scale each move the result by far more than the tolerance.  Every leaf
+    is perturbed so that LayerNorm scales and biases are not the identity."""
+    g = _rashba(N=16, L=21.2694, lr=0.8, hz=0.05)
+    a = opf.Arch(16, g["Gs"], g["W"], 21.2694, embed_dim=d, depth=depth, n_heads=h, n_freq=8, n_dets=8)
+    th = opf.init_theta(a, 11, backflow_scale=0.05, scale_all=0.1)
+    p = opf.unravel(a, th)
+    rng = np.random.default_rng(12)
+    R, S = torch.tensor(rng.uniform(0, 21.2694, (16, 2))), torch.tensor(rng.integers(0, 2, 16))
+    x = opf.backbone(a, p, R, S)
+    # library path: features by an explicit loop over (axis, sin/cos, harmonic)
+    s = R / 21.2694
+    cols = [f(2 * math.pi * m * s[:, ax]) for ax in range(2) for f in (torch.sin, torch.cos) for m in range(1, 9)]
+    tok = torch.cat([torch.stack(cols, -1), torch.nn.functional.one_hot(S, 2).double()], -1)
+    emb = torch.nn.Linear(34, d, dtype=torch.float64)
+    with torch.no_grad():
+        emb.weight.copy_(p["params/embed/kernel"].T)
+        emb.bias.copy_(p["params/embed/bias"])
+        y = emb(tok)[None]
+        for l in range(depth):
+            y = _torch_block(p, f"params/TransformerBlock_{l}", d, h)(y)
+        y = torch.nn.functional.layer_norm(y[0], (d,), p["params/LayerNorm_0/scale"], p["params/LayerNorm_0/bias"],
+                                           eps=1e-6)
+    assert torch.max(torch.abs(x - y)) < 1e-12 * max(1.0, float(torch.max(torch.abs(y))))
+    # the planted mistakes above are visible: post-norm at the same weights differs at O(1)
+    if depth:
+        with torch.no_grad():
+            yb = emb(tok)[None]
+            blk = _torch_block(p, "params/TransformerBlock_0", d, h)
+            blk.norm_first = False
+            yb = torch.nn.functional.layer_norm(blk(yb)[0], (d,), eps=1e-6)
+            x1 = opf.layer_norm(x, {"n/scale": torch.ones(d, dtype=torch.float64),
+                                    "n/bias": torch.zeros(d, dtype=torch.float64)}, "n")
+        assert torch.max(torch.abs(yb - x1)) > 1e-2
+
+
+def test_oracle_mha_and_softmax_match_torch_functional():
+    """mha (psiformer.py:50-64) against torch.nn.functional.multi_head_attention_forward with the
+    packed q/k/v projection, and the softmax step against torch.softmax."""
+    d, h, N = 128, 8, 16
+    rng = np.random.default_rng(5)
+    t = lambda *s: torch.tensor(rng.standard_normal(s) / math.sqrt(s[0]), dtype=torch.float64)
+    p = {f"m/{n}/kernel": t(d, d) for n in "qkvo"}
+    p.update({f"m/{n}/bias": t(d) for n in "qkvo"})
+    x = torch.tensor(rng.standard_normal((N, d)), dtype=torch.float64)
+    ours = opf.mha(x, p, "m", h)
+    ref, _ = torch.nn.functional.multi_head_attention_forward(
+        x[:, None], x[:, None], x[:, None], d, h,
+        torch.cat([p["m/q/kernel"].T, p["m/k/kernel"].T, p["m/v/kernel"].T]),
+        torch.cat([p["m/q/bias"], p["m/k/bias"], p["m/v/bias"]]), None, None, False, 0.0,
+        p["m/o/kernel"].T, p["m/o/bias"], training=False, need_weights=False)
+    assert torch.max(torch.abs(ours - ref[:, 0])) < 1e-12
+    # interleaved heads (head j = columns j, j+h, ...) would be a different function
+    perm = torch.arange(d).reshape(d // h, h).T.reshape(-1)
+    p2 = dict(p)
+    for n in "qkv":
+        p2[f"m/{n}/kernel"], p2[f"m/{n}/bias"] = p[f"m/{n}/kernel"][:, perm], p[f"m/{n}/bias"][perm]
+    p2["m/o/kernel"] = p["m/o/kernel"][perm]
+    assert torch.max(torch.abs(opf.mha(x, p2, "m", h) - ref[:, 0])) > 1e-3
+    s = torch.tensor(rng.standard_normal((h, N, N)) * 4, dtype=torch.float64)
+    z = s - s.max(-1, keepdim=True).values
+    assert torch.max(torch.abs(torch.exp(z) / torch.exp(z).sum(-1, keepdim=True) - torch.softmax(s, -1))) < 1e-15
+
+
+def test_oracle_backflow_layout():
+    """bf[k, a, i] = out[i, k n_orb + a] (psiformer.py:139-143: reshape (N, K, n_orb), transpose
+    (1, 2, 0)), checked by perturbing one backflow bias entry at a time with K != N: only the
+    elements of row a of determinant k change, M_k[a, i] = env[a, i] (1 + out[i, k n_orb + a]); the
+    finite difference of the oracle's log psi is compared with tr(M_k^-1 dM_k) weighted by the
+    determinant's share of the sum, built here independently of the oracle's reshape."""
+    g = _rashba()
+    a = opf.Arch(5, g["Gs"], g["W"], 7.0, embed_dim=16, depth=1, n_heads=4, n_freq=4, n_dets=3)
+    th = opf.init_theta(a, 4, backflow_scale=0.05)
+    p = opf.unravel(a, th)
+    rng = np.random.default_rng(6)
+    R, S = torch.tensor(rng.uniform(0, 7, (5, 2))), torch.tensor(rng.integers(0, 2, 5))
+    base = opf.logpsi_from_params(a, p, R, S)
+    # a bias on output column c = k*N + a multiplies row a of determinant k by (1 + bf + delta):
+    # with det_logw = 0, d logpsi / d bias_c = w_k * sum_i (M_k^-1)[i, a] env[a, i]
+    x = opf.backbone(a, p, R, S).to(torch.complex128)
+    bf = (x @ p["params/backflow/kernel"] + p["params/backflow/bias"])
+    Gs, W = torch.tensor(a.Gs), torch.tensor(a.W)
+    env = torch.einsum("gi,agi->ai", torch.exp(1j * (Gs @ R.T).to(torch.complex128)), W[:, :, S.long()])
+    Ms = [torch.stack([env[aa] * (1 + bf[:, k * 5 + aa]) for aa in range(5)]) for k in range(3)]
+    logd = torch.stack([torch.linalg.slogdet(M)[1] + torch.log(torch.linalg.slogdet(M)[0]) for M in Ms])
+    z = torch.exp(logd - logd.real.max())
+    wk = z / z.sum()
+    h = 1e-7
+    for c in (0, 4, 6, 14):
+        k, aa = divmod(c, 5)
+        p2 = dict(p)
+        b2 = p["params/backflow/bias"].clone()
+        b2[c] += h
+        p2["params/backflow/bias"] = b2
+        fd = (opf.logpsi_from_params(a, p2, R, S) - base) / h
+        Minv = torch.linalg.inv(Ms[k])
+        an = wk[k] * (Minv[:, aa] * env[aa]).sum()
+        assert abs(complex(fd - an)) < 1e-5 * max(1.0, abs(complex(an))), (c, fd, an)
 
 
 def test_oracle_backbone_permutation_equivariant():
