@@ -211,15 +211,16 @@ class VMCStep:
             e = e + self.coulomb(self.chain.R)
         return e
 
-    def step(self, key=0):
+    def step(self, key=0, streams=None):
+        """One step; `streams` (sampler.make_streams layout) replaces the sweeps' random draws."""
         t = [self._ev()] if self.timers is not None else None
         if self.graph:
             if self._gs is None:
                 from .graphs import GraphSweeps
                 self._gs = GraphSweeps(self.chain, self.n_sweeps, self.step_size, self.n_spin_flips)
-            ap, _ = self._gs.run(key=key)
+            ap, asp = self._gs.run(key=key, streams=streams)
         else:
-            ap, _ = self.chain.run(self.n_sweeps, self.step_size, self.n_spin_flips, key=key)
+            ap, asp = self.chain.run(self.n_sweeps, self.step_size, self.n_spin_flips, key=key, streams=streams)
         if t: t.append(self._ev())
         e = self.energies()
         if t: t.append(self._ev())
@@ -247,7 +248,7 @@ class VMCStep:
         if t:
             t.append(self._ev())
             self.timers.append(t)
-        self.last = dict(e=e, stats=stats, acc=ap, dtheta=d)
+        self.last = dict(e=e, stats=stats, acc=ap, acc_spin=asp, dtheta=d)
         return stats
 
 
