@@ -184,48 +184,3 @@ def slater_spin_density(Gs, W, A, ngrid=24):
     cud = np.sum(np.conj(up) * dn, axis=-1)
     mz = np.sum(np.abs(up) ** 2 - np.abs(dn) ** 2, axis=-1)
     return np.stack([2 * np.real(cud), 2 * np.imag(cud), mz], -1)
-
-
-def classify_state(logpsi, R_batch, S_batch, A, ngrid=12, nshell=4, q1=None, q2=None, nharm=1):
-    """Texture + skyrmion number + structure-factor peak + a label (observables.py:172-220).  The
-    spin estimators are evaluated once and shared by the three measurements."""
-    _check_logpsi(logpsi)
-    sp, R, _ = _real_spins(logpsi, R_batch, S_batch)
-    tex = _texture(sp, R, A, ngrid)
-    if q1 is not None:
-        G = skx_arms(q1, q2, nharm)
-        mG, _, _ = _fourier(sp, R, G)
-        ng = max(20, ngrid)
-        Q = float(berg_luscher(_reconstruct(mG, G, A, ng)))
-        order_amp = float(np.mean(np.linalg.norm(mG[:, 1:], axis=0)))
-    else:
-        Q = berg_luscher(tex["m"])
-        order_amp = float("nan")
-    mn, qs = _recip_grid(A, nshell)
-    _, szz, sperp = _fourier(sp, R, qs)
-    qn = np.linalg.norm(qs, axis=1)
-    fin = qn > 1e-9
-    tot = szz + sperp
-    gamma = float(tot[~fin].sum())
-    if fin.any():
-        kpk = int(np.argmax(tot * fin))
-        peak_frac, peak_val = mn[kpk], float(tot[kpk])
-        zfrac = float(szz[kpk] / (tot[kpk] + 1e-12))
-    else:
-        peak_frac, peak_val, zfrac = (0, 0), 0.0, 0.0
-    mz, mxy = abs(tex["mbar"][2]), float(np.hypot(*tex["mbar"][:2]))
-    finite_q_order = peak_val > 1.5 * (gamma / max(1, int((~fin).sum())))
-    ordered = (order_amp > 0.05) if order_amp == order_amp else finite_q_order
-    if not ordered:
-        label = ("ferromagnet (m_z)" if mz > 0.3 else
-                 "in-plane / Rashba-FM" if mxy > 0.3 else
-                 "fluid / disordered")
-    else:
-        label = ("SKYRMION (Q!=0, ordered)" if abs(Q) > 0.4 else
-                 "z-SDW / collinear (Overhauser)" if zfrac > 0.6 else
-                 "spiral / in-plane (1Q)")
-    sf = dict(q=qs, qfrac=mn, Szz=szz, Sperp=sperp)
-    fou = None if q1 is None else dict(Q=Q, order_amp=order_amp, mG=mG)
-    return dict(skyrmion_Q=float(Q), order_amp=order_amp, mbar=tex["mbar"].tolist(), mabs=tex["mabs"],
-                peak_qfrac=peak_frac, peak_val=peak_val, gamma_weight=gamma, peak_is_z=zfrac, label=label,
-                _tex=tex, _sf=sf, _fou=fou)

@@ -23,12 +23,6 @@ def bands_k(kx, ky, m_star=1.0, lambda_r=1.0, h_z=0.0):
     return float(w[0]), float(w[1]), v[:, 0].copy(), v[:, 1].copy()
 
 
-def bands_analytic(kx, ky, m_star=1.0, lambda_r=1.0, h_z=0.0):
-    k2 = kx * kx + ky * ky
-    dm = np.sqrt(lambda_r ** 2 * k2 + 0.25 * h_z ** 2)
-    return k2 / (2.0 * m_star) - dm, k2 / (2.0 * m_star) + dm
-
-
 def allowed_k(L, n_cut, twist=(0.0, 0.0)):
     n = np.arange(-n_cut, n_cut + 1)
     nx, ny = np.meshgrid(n, n, indexing="ij")
@@ -82,30 +76,3 @@ def planewave_envelope(Kvecs, Chi):
     W = np.zeros((n, n, 2), dtype=complex)
     W[np.arange(n), np.arange(n), :] = Chi
     return Kvecs.copy(), W
-
-
-def cell_rashba_envelope(A, N, m_star=1.0, lambda_r=1.0, h_z=0.0, n_cut=8, twist=(0.0, 0.0)):
-    """General cell A (columns = lattice vectors): momenta B n + A^-T twist, B = 2 pi A^-T."""
-    A = np.array([[float(A), 0.0], [0.0, float(A)]]) if np.ndim(A) == 0 else np.asarray(A, float)
-    Ainv = np.linalg.inv(A)
-    B = 2.0 * np.pi * Ainv.T
-    kt = Ainv.T @ np.asarray(twist, float)
-    n = np.arange(-n_cut, n_cut + 1)
-    ks = [B @ np.array([float(a), float(b)]) + kt for a in n for b in n]
-    lv = _sorted_levels(ks, m_star, lambda_r, h_z)
-    occ = lv[:N]
-    Kvecs = np.array([[t["kx"], t["ky"]] for t in occ])
-    Chi = np.array([t["chi"] for t in occ])
-    Gs, W = planewave_envelope(Kvecs, Chi)
-    area = float(abs(np.linalg.det(A)))
-    return dict(Gs=Gs, W=W, E_gs=float(sum(t["E"] for t in occ)), gap=float(lv[N]["E"] - lv[N - 1]["E"]),
-                Kvecs=Kvecs, Chi=Chi, area=area, N=N, density=N / area)
-
-
-def ring_minimum(m_star=1.0, lambda_r=1.0, h_z=0.0):
-    if h_z == 0.0:
-        return lambda_r * m_star, -0.5 * lambda_r ** 2 * m_star
-    ks = np.linspace(0.0, 3.0 * lambda_r * m_star + 1.0, 200001)
-    E = ks ** 2 / (2 * m_star) - np.sqrt(lambda_r ** 2 * ks ** 2 + 0.25 * h_z ** 2)
-    i = int(np.argmin(E))
-    return float(ks[i]), float(E[i])

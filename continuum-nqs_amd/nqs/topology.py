@@ -1,11 +1,16 @@
 """C_n rotation eigenvalue of a many-body state on the HIP path (call surface of
-src/nqs/topology.py:84-150).
+src/nqs/topology.py:100-122).
 
-[R_n psi](r_1 s_1, ...) = prod_i exp(-i theta (1 - 2 s_i)/2) psi(R_n^-1 r_1, s_1; ...) with
-theta = 2 pi/n, so <R_n> = < phase(S) psi(R_n^-1 R, S)/psi(R, S) >_{|psi|^2}: two batched log psi
-evaluations (nqs_psiformer_fwd) and one nqs_ratio_stats reduction (mean and standard error).
-The exact single-plane-wave Slater value det(m) and the position map are host numpy.
+The rotation acts on positions and spins together: [R_n psi](X) = U(S) psi(R_n^-1 X), where every
+electron's position is turned by -2 pi/n about `center` (then wrapped into the cell) and every spin
+picks up exp(-i (pi/n) sigma_z), sigma_z = +1 for s = 0.  For a configuration the spin factor is
+therefore exp(-i (pi/n) (n_up - n_down)), with n_up - n_down = N - 2 sum_i s_i.  The estimator
+<R_n> = < U(S) psi(R_n^-1 R, S) / psi(R, S) >_{|psi|^2} is two batched log psi evaluations
+(nqs_psiformer_fwd) and one nqs_ratio_stats reduction; the position map and the phase are computed
+on the device in torch.
 """
+import math
+
 import numpy as np
 import torch
 
@@ -14,54 +19,30 @@ from .psiformer import LogPsi, as_walkers
 from .manybody_chern import ratio_stats
 
 
-def _rot2d(theta):
-    c, s = np.cos(theta), np.sin(theta)
-    return np.array([[c, -s], [s, c]])
-
-
 def rotate_positions(R, n, L, center=0.0):
-    """R_n^-1 r = Rot(-2 pi/n)(r - center) + center, wrapped to [0, L)^2 (topology.py:100-105)."""
-    th = 2.0 * np.pi / n
-    Rm = _rot2d(-th)
-    c = np.asarray(center, float) if np.ndim(center) else np.array([center, center], float)
-    return np.mod((np.asarray(R, float) - c) @ Rm.T + c, L)
+    """Positions R (..., 2) mapped by R_n^-1 (a turn by -2 pi/n about `center`), wrapped to [0, L)^2.
+    Returns a tensor on R's device (numpy input goes to the current GPU)."""
+    R = torch.as_tensor(np.asarray(R, float) if not torch.is_tensor(R) else R, dtype=torch.float64)
+    if not R.is_cuda:
+        R = R.to(_lib.device())
+    c = torch.as_tensor(np.broadcast_to(np.asarray(center, float), (2,)).copy(), dtype=torch.float64,
+                        device=R.device)
+    z = torch.view_as_complex((R - c).contiguous()) * complex(math.cos(2 * math.pi / n), -math.sin(2 * math.pi / n))
+    return torch.remainder(torch.view_as_real(z) + c, L)
 
 
 def spin_rotation_phase(S, n):
-    """prod_i exp(-i (2 pi/n)(1 - 2 s_i)/2) per configuration (topology.py:108-111)."""
-    th = 2.0 * np.pi / n
-    return np.exp(-1j * th * np.sum(1.0 - 2.0 * np.asarray(S), axis=-1) / 2.0)
+    """exp(-i (pi/n)(N - 2 sum_i s_i)) per configuration, S (..., N) in {0, 1}."""
+    S = torch.as_tensor(np.asarray(S) if not torch.is_tensor(S) else S)
+    m = S.shape[-1] - 2 * S.to(torch.float64).sum(-1)                   # n_up - n_down
+    return torch.polar(torch.ones_like(m), -(math.pi / n) * m)
 
 
 def rotation_eigenvalue_vmc(logpsi, R, S, n, L, center=0.0):
-    """(<R_n>, standard error) over walkers R (W, N, 2), S (W, N) (topology.py:114-122);
-    logpsi is a batched LogPsi."""
+    """(<R_n>, standard error) over walkers R (W, N, 2), S (W, N); logpsi is a batched LogPsi."""
     if not isinstance(logpsi, LogPsi):
         raise TypeError("rotation_eigenvalue_vmc needs a LogPsi")
-    R = R.cpu().numpy() if torch.is_tensor(R) else np.asarray(R, float)
-    S = S.cpu().numpy() if torch.is_tensor(S) else np.asarray(S)
-    Rrot = rotate_positions(R, n, L, center)
-    sp = spin_rotation_phase(S, n)
     Rd, Sd, _ = as_walkers(R, S)
-    Rr, _, _ = as_walkers(Rrot, S)
-    st = ratio_stats(logpsi(Rd, Sd), logpsi(Rr, Sd), phase=torch.tensor(sp, device=Rd.device), shift=False)
-    W = R.shape[0]
-    return complex(st[0], st[1]), complex(st[5] / np.sqrt(W))
-
-
-def rotation_eigenvalue_exact_singleG(Gs, Chi, n, L):
-    """Exact <R_n> = det(m) for single-plane-wave spinor orbitals exp(i Gs[a].r) Chi[a] on a
-    C_n-closed G set (topology.py:125-150)."""
-    Gs = np.asarray(Gs, float)
-    Chi = np.asarray(Chi, complex)
-    N = len(Gs)
-    th = 2.0 * np.pi / n
-    Rp = _rot2d(th)
-    phase = np.array([np.exp(-1j * th * (1 - 2 * s) / 2.0) for s in (0, 1)])
-    m = np.zeros((N, N), dtype=complex)
-    for b in range(N):
-        RGb = Rp @ Gs[b]
-        for a in range(N):
-            if np.allclose(Gs[a], RGb, atol=1e-8):
-                m[a, b] = np.sum(np.conj(Chi[a]) * phase * Chi[b])
-    return complex(np.linalg.det(m))
+    Rr = rotate_positions(Rd, n, L, center).contiguous()
+    st = ratio_stats(logpsi(Rd, Sd), logpsi(Rr, Sd), phase=spin_rotation_phase(Sd, n), shift=False)
+    return complex(st[0], st[1]), complex(st[5] / math.sqrt(Rd.shape[0]))

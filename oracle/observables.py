@@ -6,6 +6,8 @@ Restates, for one configuration at a time or in plain numpy:
   spin_fourier     observables.py:94-121 (rho_a(q) per walker, m_a(q), S_zz, S_perp)
   berg_luscher     observables.py:79-91 (explicit plaquette loops)
   ratio_stats      manybody_chern.py:23-34 and topology.py:114-122 (mean, mean |r|^2, ess, std)
+  rotation_overlap_planewaves  the exact <R_n> of a single-plane-wave spinor Slater determinant, the
+                   closed form topology.py:125-150 computes, derived here from the orbital overlaps
 Pins (tests/test_oracle.py): a spin-independent trial is exactly +x polarised (m_x = 1, m_y = 0);
 uncorrelated spins give a flat S_zz; a triple-Q skyrmion texture has Berg-Luscher charge 1 and a
 uniform one 0; the overlap of two plane-wave determinants is det(Wa^* Wb^T).
@@ -86,3 +88,17 @@ def ratio_stats(la, lb, phase=None, shift=True):
     a = np.abs(r)
     return np.array([r.real.mean(), r.imag.mean(), (a ** 2).mean(), a.sum() ** 2 / (a ** 2).sum() / len(r), s,
                      np.std(r)])
+
+
+def rotation_overlap_planewaves(Gs, Chi, n):
+    """<Psi| R_n |Psi> for the Slater determinant of phi_a(r, s) = exp(i G_a.r) Chi[a, s] (orthonormal
+    orbitals spanning an R_n-invariant space).  R_n maps phi_b to the plane wave at Rot(2 pi/n) G_b
+    with the spinor U Chi[b], U = diag(exp(-i pi/n), exp(+i pi/n)) (spin s = 0 first), so the orbital
+    overlap matrix is m[a, b] = [G_a == Rot G_b] Chi[a]^* . U Chi[b] and <R_n> = det m."""
+    Gs = np.asarray(Gs, float)
+    Chi = np.asarray(Chi, complex)
+    t = 2.0 * np.pi / n
+    rot = Gs @ np.array([[np.cos(t), np.sin(t)], [-np.sin(t), np.cos(t)]])      # row b = Rot(t) G_b
+    same = np.linalg.norm(Gs[:, None, :] - rot[None, :, :], axis=-1) < 1e-8 * (1.0 + np.abs(Gs).max())
+    spin = np.conj(Chi) @ (np.array([np.exp(-1j * t / 2), np.exp(1j * t / 2)])[:, None] * Chi.T)
+    return complex(np.linalg.det(np.where(same, spin, 0.0)))

@@ -68,8 +68,6 @@ def test_spin_independent_state_is_x_polarised():
     sf = nob.structure_factor(lp, R, S, A, nshell=3)
     qn = np.linalg.norm(sf["q"], axis=1)
     assert sf["Sperp"][qn < 1e-9][0] > 3.0 * sf["Sperp"][qn > 1e-9].max()
-    cl = nob.classify_state(lp, R, S, A, ngrid=10, nshell=3)
-    assert "in-plane" in cl["label"] or "Rashba" in cl["label"], cl["label"]
 
 
 def test_texture_fourier_and_ratio_kernels_match_numpy():
@@ -147,7 +145,7 @@ def test_rotation_c4_and_c2_of_an_exact_eigenstate():
     R = rng.uniform(0, L, (40, 4, 2))
     S = rng.integers(0, 2, (40, 4))
     for n in (4, 2):
-        exact = ntp.rotation_eigenvalue_exact_singleG(Gs, Chi, n, L)
+        exact = oob.rotation_overlap_planewaves(Gs, Chi, n)
         vmc, err = ntp.rotation_eigenvalue_vmc(lp, R, S, n, L)
         assert abs(exact) > 0.999
         assert err.real < 1e-7 and err.imag < 1e-7
@@ -157,7 +155,7 @@ def test_rotation_c4_and_c2_of_an_exact_eigenstate():
 def test_rotation_general_state_matches_exact_by_mcmc():
     L = 7.0
     Gs, Chi = _c4_state(L, None, symmetric=False)
-    exact = ntp.rotation_eigenvalue_exact_singleG(Gs, Chi, 4, L)
+    exact = oob.rotation_overlap_planewaves(Gs, Chi, 4)
     lp = make_logpsi_general(*planewave_envelope(Gs, Chi), L)
     R, S, _, _ = run_chain(lp, L, 4, 0, n_walkers=4000, n_sweeps=200, burn=150, step=0.6)
     vmc, err = ntp.rotation_eigenvalue_vmc(lp, R, S, 4, L)

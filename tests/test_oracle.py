@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import bands, ewald as oew, local_energy as ole, minsr as oms, psiformer as opf, sampler as osa
+from oracle import observables as oob
 
 
 def _rashba(N=5, L=7.0, lr=1.0, hz=0.5):
@@ -637,3 +638,40 @@ def test_oracle_adam_pins():
             gg = g[off:off + n]
             assert np.max(np.abs(th[off:off + n] - (theta.numpy()[off:off + n] - lr * gg / (np.abs(gg) + 1e-8)))) < 1e-15
             assert np.array_equal(th[off + n:off + 2 * n], theta.numpy()[off + n:off + 2 * n])
+
+
+def test_oracle_rotation_overlap_is_the_zero_variance_ratio():
+    """For a C4-closed single-plane-wave state whose spinors follow the rotation (an eigenstate of
+    R_4 and of R_2), psi(R_n^-1 X) U(S) / psi(X), evaluated with the oracle's Slater log psi at
+    random configurations, equals rotation_overlap_planewaves at every configuration; a state that
+    is not an eigenstate gives a ratio that varies."""
+    L = 7.0
+    g = 2 * np.pi / L
+    Gs = g * np.array([[1, 0], [0, 1], [-1, 0], [0, -1]], float)
+    U4 = np.diag([np.exp(-1j * np.pi / 4), np.exp(1j * np.pi / 4)])
+    c0 = np.array([1.0, 0.4 + 0.2j]) / np.linalg.norm([1.0, 0.4 + 0.2j])
+    rng = np.random.default_rng(9)
+    for sym in (True, False):
+        if sym:
+            Chi = np.stack([np.linalg.matrix_power(U4, k) @ c0 for k in range(4)])
+        else:
+            Chi = rng.normal(size=(4, 2)) + 1j * rng.normal(size=(4, 2))
+            Chi /= np.linalg.norm(Chi, axis=1, keepdims=True)
+        W = np.zeros((4, 4, 2), complex)
+        W[np.arange(4), np.arange(4)] = Chi
+        for n in (4, 2):
+            t = 2 * np.pi / n
+            rinv = np.array([[np.cos(t), np.sin(t)], [-np.sin(t), np.cos(t)]])      # turn by -t
+            vals = []
+            for _ in range(5):
+                R, S = rng.uniform(0, L, (4, 2)), rng.integers(0, 2, 4)
+                Rr = np.mod(R @ rinv.T, L)
+                lr = complex(opf.slater_logpsi(Gs, W, torch.tensor(Rr), torch.tensor(S)))
+                l0 = complex(opf.slater_logpsi(Gs, W, torch.tensor(R), torch.tensor(S)))
+                vals.append(np.exp(lr - l0 - 1j * (np.pi / n) * (4 - 2 * S.sum())))
+            exact = oob.rotation_overlap_planewaves(Gs, Chi, n)
+            if sym:
+                assert abs(abs(exact) - 1) < 1e-12
+                assert np.max(np.abs(np.array(vals) - exact)) < 1e-10, (n, vals, exact)
+            else:
+                assert np.ptp(np.abs(vals)) > 1e-3
