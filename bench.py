@@ -9,15 +9,19 @@ Rashba SOC, Zeeman) + Ewald Coulomb + median/MAD clip + the per-walker gradient 
 backward) + MinSR (Gram on MFMA, float64 Cholesky, one refinement step, push-through) + the
 parameter update.
 
---precision mixed: float32 backbone activations whose dense layers run as bf16x3 on MFMA
-(float32 operands split into bf16 hi/lo pairs, float32 accumulation), bf16 MFMA Gram with
-float32 accumulation, float64 determinants / Ewald / Cholesky.  --precision fp64: everything
-float64 (the plain FP64 GPU path).  Prints one JSON line; --gpus > 1 is refused (one GPU only).
+--precision mixed (default): float32 backbone activations whose dense layers run as bf16x3 on MFMA
+(float32 operands split into bf16 hi/lo pairs, float32 accumulation), bf16 MFMA Gram with float32
+accumulation, float64 determinants / Ewald / Cholesky.  --precision fp64: everything float64 (the
+plain FP64 GPU path).  Prints one JSON line; --gpus > 1 is refused (one GPU only).
 
-For --precision mixed the line also carries "accuracy": after the timed steps, the mixed path
-and the float64 path are evaluated on the same walkers with the same parameters (outside the
-timed region): per-step energy difference, log psi, E_loc and Ewald differences, and the relative
-difference of the MinSR direction d theta.
+The default run times the mixed path ("value"), then --fp64-steps steps of the plain float64 path
+from the same parameters and burnt-in walkers ("fp64_value", "ratio_mixed_over_fp64").  Outside the
+timed regions it records "accuracy": the mixed path against the float64 GPU path on every walker
+(per-step energy, log psi, E_loc, MinSR direction) and both against the float64 oracle on a subset
+(log psi, E_loc, Ewald), and "cpu_baseline": the oracle's step measured on a few walkers and
+extrapolated to the full walker count.  "roofline" is the Gram (the dominant kernel), timed with
+hipEvents around its dispatches; "step_roofline" is the whole step's algorithmic floor over its
+measured time.
 """
 import argparse
 import ctypes
@@ -62,8 +66,13 @@ def build_system(precision, W, seed=0, gemm="bf16x3"):
     return dict(model=model, theta=theta, R=R, S=S, L=L, Gs=Gs, Wenv=Wenv)
 
 
-def cpu_baseline(sysd, n_walkers=8, seed=0):
-    """The oracle's step (torch float64 on the host cores) on a bounded sample of walkers."""
+def cpu_baseline(sysd, n_walkers=8, seed=0, W_full=8192):
+    """The oracle's step (torch float64 on the host cores): one full step measured on a bounded
+    sample of walkers, and the headline step extrapolated from measured pieces.  Walker-parallel
+    stages (sweeps, dense-Hessian E_loc, reverse-mode O) scale linearly in W; the oracle's MinSR
+    (Otil Otil^T, LU solve, push-through) is timed on its own at 2 W_g = 256 rows of random data with
+    the real P and scaled as W^2 P for the Gram, W P for the push-through and W^3 for an LU timed at
+    4096^2."""
     from oracle import psiformer as opf, ewald as oew, cpu_step
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
@@ -77,28 +86,116 @@ def cpu_baseline(sysd, n_walkers=8, seed=0):
     cpu_step.vmc_step(a, th, R, S, lambda_r=LAM, h_z=HZ, coulomb=ce, n_sweeps=10, step=0.3 * RS, lam=1e-4,
                       lr=0.05, seed=seed, timings=tm)
     dt = time.perf_counter() - t0
-    return {"value": n_walkers / dt, "unit": "walker-steps/s", "cores": cores, "kind": "port",
-            "sample": f"one full step of the float64 torch oracle on {n_walkers} of the walkers "
-                      f"(10 sweeps, dense-Hessian E_loc, reverse-mode O, LU MinSR); "
-                      f"stage seconds {json.dumps({k: round(v, 3) for k, v in tm.items()})}"}
+    # MinSR pieces at scale
+    P = th.shape[0]
+    Wg = 128
+    gen = torch.Generator().manual_seed(3)
+    Ot = torch.randn(2 * Wg, P, generator=gen, dtype=torch.float64)
+    t1 = time.perf_counter()
+    T = Ot @ Ot.T
+    t_gram = time.perf_counter() - t1
+    y = torch.randn(2 * Wg, generator=gen, dtype=torch.float64)
+    t1 = time.perf_counter()
+    Ot.T @ y
+    t_push = time.perf_counter() - t1
+    n_lu = 4096
+    B = torch.randn(n_lu, n_lu, generator=gen, dtype=torch.float64)
+    B = B @ B.T + n_lu * torch.eye(n_lu, dtype=torch.float64)
+    t1 = time.perf_counter()
+    torch.linalg.solve(B, torch.randn(n_lu, 1, generator=gen, dtype=torch.float64))
+    t_lu = time.perf_counter() - t1
+    del Ot, T, B
+    per_walker = (tm["sample"] + tm["eloc"] + tm["grad"]) / n_walkers
+    ext = {"walker_stages_s": per_walker * W_full, "gram_s": t_gram * (W_full / Wg) ** 2,
+           "pushthrough_s": t_push * (W_full / Wg), "lu_s": t_lu * (2 * W_full / n_lu) ** 3}
+    t_full = sum(ext.values())
+    return {"value": W_full / t_full, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+            "sample": f"the float64 torch oracle's step (10 sweeps with log psi recomputed per proposal, "
+                      f"dense-Hessian E_loc, reverse-mode O, LU MinSR): value = {W_full} walkers / the "
+                      f"extrapolated step time; measured: one full step on {n_walkers} walkers "
+                      f"({n_walkers / dt:.3f} walker-steps/s, stage seconds "
+                      f"{json.dumps({k: round(v, 3) for k, v in tm.items()})}), the Gram Otil Otil^T on "
+                      f"{2 * Wg} x {P} ({t_gram:.3f} s), the push-through ({t_push:.4f} s), a {n_lu}^2 LU solve "
+                      f"({t_lu:.3f} s)",
+            "measured_sample_walker_steps_per_s": n_walkers / dt,
+            "extrapolated_s": {k: round(v, 1) for k, v in ext.items()}}
 
 
-def gram_pmc(precision, field):
-    """A rocprofv3 PMC measurement of the Gram kernel on this workload (traffic_bytes: FETCH_SIZE /
-    WRITE_SIZE scaled by a plain copy kernel; mfma_util_pct: MfmaUtil); None when not measured."""
+def _kernel_sha(path):
+    import hashlib
+    with open(path, "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
+
+
+def gram_pmc(precision):
+    """rocprofv3 PMC figures of the Gram on this workload (traffic_bytes: FETCH_SIZE / WRITE_SIZE
+    scaled by a plain copy kernel; mfma_util_pct: MfmaUtil), read from the profile summary they were
+    written to.  Returned only when the summary names the sha256 of the Gram's source file that is
+    built now, with that provenance; otherwise None."""
     f = os.path.join(ROOT, "profiles", "gram_hbm_traffic_n16_w8192.json")
     try:
         with open(f) as fh:
-            return float(json.load(fh)[precision][field])
+            js = json.load(fh)
+        rec = js[precision]
+        src = os.path.join(ROOT, "continuum-nqs_amd", "csrc", "minsr.hip")
+        if js.get("minsr_hip_sha256_16") != _kernel_sha(src):
+            return None
+        return {"traffic": float(rec["traffic_bytes"]), "mfma_util_pct": float(rec["mfma_util_pct"]),
+                "source": "profiles/gram_hbm_traffic_n16_w8192.json (rocprofv3 --pmc on this source revision, "
+                          "minsr.hip sha256 " + js["minsr_hip_sha256_16"] + ")"}
     except (OSError, KeyError, ValueError):
         return None
 
 
-def accuracy_vs_fp64(vs, sysd, coul, gemm):
-    """Mixed vs float64 GPU path on the walkers and parameters of the last timed step."""
+def f_gemm(N=N_E, d=ARCH["embed_dim"], depth=ARCH["depth"], F=ARCH["n_freq"], K=ARCH["n_dets"]):
+    """Dense-layer + attention FLOP of one log psi forward (one channel): embed, per block QKV, output
+    projection and the d -> 2d -> d MLP (16 N d^2) plus QK^T and PV (4 N^2 d), complex backflow (two
+    real GEMMs, 4 N d K N)."""
+    return 2 * N * (4 * F + 2) * d + depth * (16 * N * d * d + 4 * N * N * d) + 4 * N * d * K * N
+
+
+def step_roofline(precision, W, P, ldo, ms, n_sweeps=10, n_flip=2, refine=1):
+    """Whole-step speed-of-light: sum over the step's stages of max(flop / peak, bytes / HBM), each
+    term from the algorithm (not from counters), over the measured step time.  Stages: walker GEMMs
+    (n_sweeps (1 + n_flip) forwards, E_loc jets with 2N + 2 channels for the Laplacian and 3 per
+    flipped configuration for the SOC term, about 4 forwards for the backward) at the bf16 MFMA peak
+    x 3 (bf16x3 runs three bf16 products per float32 product) or the f64 peak; determinant LUs and
+    inverses (f64); O written once; the Gram operand (bf16: O read once, bf16 copy written; float64:
+    centred in place, read + write); the Gram (2W)(2W+1)P at bf16 / f64 MFMA peak; the Cholesky
+    (2W)^3/3 at the f64 peak; the refinement residual and the push-through (three reads of O)."""
+    HBM = 8.0e12
+    N, K = N_E, ARCH["n_dets"]
+    mixed = precision == "mixed"
+    b = 4 if mixed else 8
+    gemm_peak = PEAK["bf16"] / 3 if mixed else PEAK["fp64"]
+    walker_flop = W * f_gemm() * (n_sweeps * (1 + n_flip) + (2 * N + 2) + 3 * N + 4)
+    lu = 8.0 * N ** 3 / 3
+    det_flop = W * K * (n_sweeps * (1 + n_flip) * lu + (lu + 8 * N ** 3 + 2 * N * 8 * N ** 3)
+                        + N * (lu + 8 * N ** 3 + 2 * 8 * N ** 3))
+    o_bytes = 2.0 * W * ldo * b
+    n = 2 * W
+    terms = {
+        "walker_gemms": walker_flop / gemm_peak,
+        "determinants": det_flop / PEAK["fp64"],
+        "O_write": o_bytes / HBM,
+        "gram_operand": (o_bytes + 2.0 * W * ldo * 2) / HBM if mixed else 2 * o_bytes / HBM,
+        "gram": float(n) * (n + 1) * P / (PEAK["bf16"] if mixed else PEAK["fp64"]),
+        "cholesky": float(n) ** 3 / 3 / PEAK["fp64"],
+        "refine_pushthrough": (2 * refine + 1) * o_bytes / HBM,
+    }
+    floor = sum(terms.values())
+    return {"floor_ms": 1e3 * floor, "measured_ms": ms, "frac": 1e3 * floor / ms,
+            "terms_ms": {k: round(1e3 * v, 2) for k, v in terms.items()}}
+
+
+def accuracy(vs, sysd, coul, gemm, n_oracle=2, n_ewald=64):
+    """Mixed vs float64: the float64 GPU path on every walker (same walkers and parameters as the
+    last timed step), and the float64 oracle on a subset (log psi of 8 walkers, E_loc of n_oracle
+    walkers, Ewald of n_ewald walkers) for both GPU paths."""
     from nqs.psiformer import ContinuumSpinorPsiformer, LogPsi
     from nqs.local_energy import make_local_energy
     from nqs.minsr import grad_params, minsr_direction, clip_energies
+    from oracle import psiformer as opf, local_energy as ole, ewald as oew
     m64 = ContinuumSpinorPsiformer(n_elec=N_E, Gs=sysd["Gs"], W=sysd["Wenv"], L=sysd["L"], precision="fp64", **ARCH)
     R, S, theta = vs.chain.R, vs.chain.S, vs.theta
     lps = {"mixed": LogPsi(vs.model, theta), "fp64": LogPsi(m64, theta)}
@@ -113,11 +210,34 @@ def accuracy_vs_fp64(vs, sysd, coul, gemm):
     de = res["mixed"]["e"] - res["fp64"]["e"]
     E_m = float((res["mixed"]["e"].real + ec).mean())
     E_d = float((res["fp64"]["e"].real + ec).mean())
-    out = {"reference": "float64 GPU path, same walkers and parameters", "gemm": gemm,
-           "walkers": int(R.shape[0]), "E_step_fp64": E_d, "dE_step": abs(E_m - E_d),
+    out = {"reference": "float64 GPU path on all walkers (same walkers and parameters); the float64 oracle on "
+                        "the subset in 'oracle'",
+           "gemm": gemm, "walkers": int(R.shape[0]), "E_step_fp64": E_d, "dE_step": abs(E_m - E_d),
            "logpsi_re_max_abs": float(dlp.real.abs().max()), "logpsi_im_max_abs": float(dim.abs().max()),
-           "eloc_max_abs": float(de.abs().max()), "eloc_rms": float(de.abs().pow(2).mean().sqrt()),
-           "ewald_max_abs": 0.0}
+           "eloc_max_abs": float(de.abs().max()), "eloc_rms": float(de.abs().pow(2).mean().sqrt())}
+    # the oracle on a subset of the same walkers
+    a = opf.Arch(N_E, sysd["Gs"], sysd["Wenv"], sysd["L"], **ARCH)
+    th = theta.cpu()
+    Rc, Sc = R[:max(8, n_oracle)].cpu(), S[:max(8, n_oracle)].cpu().long()
+    lo = torch.stack([opf.logpsi(a, th, Rc[w], Sc[w]) for w in range(8)]).numpy()
+    Eo = ole.make_local_energy(opf.make_logpsi(a, th), 1.0, LAM, HZ)
+    eo = np.array([Eo(Rc[w], Sc[w]) for w in range(n_oracle)])
+    _, _, ceo = oew.make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
+    ewo = np.array([ceo(r) for r in R[:n_ewald].cpu().numpy()])
+    ewg = ec[:n_ewald].cpu().numpy()
+    orc = {"logpsi_walkers": 8, "eloc_walkers": n_oracle, "ewald_walkers": n_ewald,
+           "ewald_max_abs": float(np.max(np.abs(ewg - ewo))), "ewald_max_rel": float(np.max(np.abs(ewg - ewo) / np.abs(ewo)))}
+    for k in ("mixed", "fp64"):
+        lg = res[k]["lp"][:8].cpu().numpy()
+        d = lg - lo
+        orc[f"logpsi_{k}_max_abs"] = float(max(np.abs(d.real).max(),
+                                               np.abs((d.imag + np.pi) % (2 * np.pi) - np.pi).max()))
+        orc[f"eloc_{k}_max_abs"] = float(np.abs(res[k]["e"][:n_oracle].cpu().numpy() - eo).max())
+    # bound on the mixed per-step energy against the oracle: |E_mixed - E_fp64| over all walkers plus
+    # the float64 path's largest per-walker deviation from the oracle on the subset
+    orc["dE_step_bound"] = out["dE_step"] + orc["eloc_fp64_max_abs"] + orc["ewald_max_abs"]
+    out["oracle"] = orc
+    out["ewald_max_abs"] = orc["ewald_max_abs"]
     # MinSR direction from each path's own O, clip and Gram
     dth = {}
     for k, lp in lps.items():
@@ -128,6 +248,46 @@ def accuracy_vs_fp64(vs, sysd, coul, gemm):
         torch.cuda.empty_cache()
     out["dtheta_rel"] = float((dth["mixed"] - dth["fp64"]).norm() / dth["fp64"].norm())
     return out
+
+
+def time_path(model, theta, R, S, coul, args, gram, steps, warmup, graph=False, burn=0):
+    """Warm-up and timed VMC steps of one precision; returns (VMCStep, ms per step, stage means,
+    MinSR sub-stage means)."""
+    from nqs import _lib
+    from nqs.minsr import VMCStep
+    vs = VMCStep(model, theta, R, S, m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul, lr=0.05, lam=1e-4,
+                 n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1, graph=graph)
+    if burn:
+        vs.chain.run(burn, 0.3 * RS, 2, key=7)
+    lib = _lib.load()
+    for k in range(warmup):
+        vs.step(key=100 + k)
+    torch.cuda.synchronize()
+    lib.nqs_minsr_profile(1, None)
+    vs.timers = []
+    gram_ms = []
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for k in range(steps):
+        vs.step(key=1000 + k)
+        ms5 = (ctypes.c_float * 5)()
+        _lib.check(lib.nqs_minsr_profile(-1, ms5), "nqs_minsr_profile")
+        gram_ms.append(list(ms5))
+    t1.record()
+    t1.synchronize()
+    lib.nqs_minsr_profile(0, None)
+    ms = t0.elapsed_time(t1) / steps
+    stages = np.array([[a.elapsed_time(b) for a, b in zip(t[:-1], t[1:])] for t in vs.timers]).mean(0)
+    vs.timers = None
+    return vs, ms, stages, np.array(gram_ms).mean(0)
+
+
+def _stage_dict(stages, mins):
+    return {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
+            "minsr": float(stages[3]), "update_repack": float(stages[4]),
+            "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]), "minsr_cholesky": float(mins[2]),
+            "minsr_solve_refine": float(mins[3]), "minsr_pushthrough": float(mins[4])}
 
 
 C4 = dict(N=32, rs=5.0, lam=0.5, hz=0.0, arch=dict(embed_dim=128, depth=4, n_heads=8, n_freq=8, n_dets=16))
@@ -404,6 +564,8 @@ def main():
     ap.add_argument("--gemm", default="bf16x3", choices=["bf16x3", "fp32", "bf16"],
                     help="dense-layer MFMA mode of the mixed backbone")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-float64 accuracy leg")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="timed steps of the plain float64 path after the mixed run (0 = skip; mixed only)")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps from a captured HIP graph")
     ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5", "next"],
                     help="c3: the headline N=16 workload (default); c4: the N=32 large configuration; c5: the "
@@ -435,67 +597,66 @@ def main():
         return
     from nqs import _lib
     from nqs.ewald import make_ewald
-    from nqs.minsr import VMCStep
+    from nqs.psiformer import ContinuumSpinorPsiformer
 
     W = args.walkers
     sysd = build_system(args.precision, W, gemm=args.gemm)
     _, _, coul = make_ewald(sysd["L"], 1.0, n_real=4, n_recip=8)
-    gram = "bf16" if args.precision == "mixed" else "fp64"
-    vs = VMCStep(sysd["model"], sysd["theta"], sysd["R"], sysd["S"], m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul,
-                 lr=0.05, lam=1e-4, n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1,
-                 graph=args.graph)
+    # burnt-in walkers shared by both precisions
+    from nqs.psiformer import LogPsi
+    from nqs.sampler import Chain
+    ch = Chain(LogPsi(sysd["model"], sysd["theta"].cuda()), sysd["R"], sysd["S"])
     if args.burn:
-        vs.chain.run(args.burn, 0.3 * RS, 2, key=7)
-    lib = _lib.load()
-    for k in range(args.warmup):
-        vs.step(key=100 + k)
-    torch.cuda.synchronize()
-    lib.nqs_minsr_profile(1, None)
-    vs.timers = []
-    gram_ms = []
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for k in range(args.steps):
-        vs.step(key=1000 + k)
-        ms5 = (ctypes.c_float * 5)()
-        _lib.check(lib.nqs_minsr_profile(-1, ms5), "nqs_minsr_profile")
-        gram_ms.append(list(ms5))
-    t1.record()
-    t1.synchronize()
-    lib.nqs_minsr_profile(0, None)
-    ms = t0.elapsed_time(t1) / args.steps
-    stages = np.array([[a.elapsed_time(b) for a, b in zip(t[:-1], t[1:])] for t in vs.timers]).mean(0)
-    mins = np.array(gram_ms).mean(0)
+        ch.run(args.burn, 0.3 * RS, 2, key=7)
+    R0, S0 = ch.R.clone(), ch.S.clone()
+    del ch
+    gram = "bf16" if args.precision == "mixed" else "fp64"
+    vs, ms, stages, mins = time_path(sysd["model"], sysd["theta"].clone(), R0, S0, coul, args, gram, args.steps,
+                                     args.warmup, graph=args.graph)
     P = sysd["model"].n_params()
+    ldo = vs.O.ldo
     n = 2 * W
     gram_flops = float(n) * (n + 1) * P          # n(n+1)/2 outputs of the symmetric rank-P update, 2P flops each
     achieved = gram_flops / (mins[1] / 1e3)
     peak = PEAK["bf16" if gram == "bf16" else "fp64"]
     stats = vs.last["stats"].cpu().numpy()
+    pmc = gram_pmc(args.precision) if W == 8192 else None
     out = {"metric": METRIC, "value": W / (ms / 1000.0), "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32+bf16+f64" if args.precision == "mixed" else "f64",
            "data": "synthetic",
-           "config": {"workload": "N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers=8192",
+           "config": {"workload": f"N=16 rs=3 lambda=0.8 h_z=0.05 d=128 layers=4 heads=8 K=8 F=8 walkers={W}",
                       "walkers": W, "params": P, "sweeps": args.sweeps, "precision": args.precision, "gram": gram,
                       "gemm": args.gemm if args.precision == "mixed" else "fp64", "graph_sweeps": bool(args.graph)},
            "roofline": {"bound": "mfma", "kernel": "Gram (k_syrk_bf16)" if gram == "bf16" else "Gram (k_gemm_nt<double>)",
                         "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / peak,
-                        "traffic": gram_pmc(args.precision, "traffic_bytes") if W == 8192 else None,
-                        "mfma_util_pct": gram_pmc(args.precision, "mfma_util_pct") if W == 8192 else None,
-                        "gram_ms": float(mins[1])},
-           "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
-                        "minsr": float(stages[3]), "update_repack": float(stages[4]),
-                        "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
-                        "minsr_cholesky": float(mins[2]), "minsr_solve_refine": float(mins[3]),
-                        "minsr_pushthrough": float(mins[4])},
-           "E_mean": float(stats[0]), "E_err": float(stats[1])}
+                        "frac": achieved / peak, "traffic": pmc["traffic"] if pmc else None,
+                        "algorithmic_flop": gram_flops, "gram_ms": float(mins[1]),
+                        "mfma_util_pct": pmc["mfma_util_pct"] if pmc else None,
+                        "pmc_source": pmc["source"] if pmc else "not measured for this source revision"},
+           "step_roofline": step_roofline(args.precision, W, P, ldo, ms, args.sweeps),
+           "stage_ms": _stage_dict(stages, mins),
+           "E_mean": float(stats[0]), "E_err": float(stats[1]), "failed_steps": vs.failed_steps}
     if args.precision == "mixed" and not args.no_accuracy:
-        out["accuracy"] = accuracy_vs_fp64(vs, sysd, coul, args.gemm)
+        out["accuracy"] = accuracy(vs, sysd, coul, args.gemm)
+    if args.precision == "mixed" and args.fp64_steps > 0:
+        vs.O = None
+        _lib.release_workspaces()
+        torch.cuda.empty_cache()
+        m64 = ContinuumSpinorPsiformer(n_elec=N_E, Gs=sysd["Gs"], W=sysd["Wenv"], L=sysd["L"], precision="fp64",
+                                       **ARCH)
+        v64, ms64, st64, mi64 = time_path(m64, sysd["theta"].clone(), R0, S0, coul, args, "fp64", args.fp64_steps, 1)
+        out["fp64"] = {"value": W / (ms64 / 1000.0), "ms_per_step": ms64, "steps": args.fp64_steps, "warmup": 1,
+                       "stage_ms": _stage_dict(st64, mi64), "failed_steps": v64.failed_steps,
+                       "gram_frac_of_f64_peak": gram_flops / (mi64[1] / 1e3) / PEAK["fp64"],
+                       "step_roofline": step_roofline("fp64", W, P, v64.O.ldo, ms64, args.sweeps)}
+        out["fp64_value"] = out["fp64"]["value"]
+        out["ratio_mixed_over_fp64"] = out["value"] / out["fp64_value"]
+        del v64
+        _lib.release_workspaces()
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sysd, args.cpu_walkers)
+        out["cpu_baseline"] = cpu_baseline(sysd, args.cpu_walkers, W_full=W)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -150,3 +150,8 @@ def workspace(nbytes, tag="default"):
         buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
         _WS[tag] = buf
     return buf
+
+
+def release_workspaces():
+    """Drop every cached workspace buffer (their memory returns to torch's caching allocator)."""
+    _WS.clear()
