@@ -1,6 +1,7 @@
 // extern "C" entry points of libnqs_hip.so (declared in include/nqs_hip.h).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 #include "model.h"
 #include "physics.h"
@@ -43,8 +44,7 @@ static ElocPlan eloc_plan(const Model& md, int64_t W) {
 
 template <typename T>
 static int fwd_logpsi(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t W, double* out,
-                      char* ws, hipStream_t st) {
-  const int64_t cw = chunk_walkers(md, W, 1, 1);
+                      char* ws, hipStream_t st, int64_t cw) {
   char* p = ws;
   FwdBufs<T> b = carve_fwd<T>(md, p, cw, 1, 0);
   for (int64_t w0 = 0; w0 < W; w0 += cw) {
@@ -581,8 +581,58 @@ int nqs_psiformer_fwd(const nqs_model_desc* m, const void* packed, const double*
   NQS_TRY(build_model(m, md));
   NQS_REQUIRE(packed && R && S && logpsi && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
   NQS_TRY(check_ws(m, NQS_OP_LOGPSI, W, ws_bytes));
-  if (md.tsize == 8) return fwd_logpsi<double>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream);
-  return fwd_logpsi<float>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream);
+  const int64_t cw = chunk_walkers(md, W, 1, 1);
+  if (md.tsize == 8) return fwd_logpsi<double>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream, cw);
+  return fwd_logpsi<float>(md, packed, R, S, W, logpsi, (char*)ws, (hipStream_t)stream, cw);
+}
+
+// ---- allocation-free log psi apply.  The activations live in `scratch`, a device array in this
+// library's code object (each GPU holds its own copy until the process ends).  Host code never
+// passes the array's name to a kernel: its device address is looked up once per device with
+// hipGetSymbolAddress (that device current) and kept in g_scratch.  Calls that overlap on one GPU
+// share the array and race.
+static constexpr int64_t kScratchBytes = 64LL << 20;
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static char* g_scratch[kMaxDevices] = {};
+
+static int scratch_address(char** out) {
+  int dev = -1;
+  const hipError_t e = hipGetDevice(&dev);
+  NQS_REQUIRE(e == hipSuccess, NQS_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  NQS_REQUIRE(dev >= 0 && dev < kMaxDevices, NQS_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    NQS_REQUIRE(r == hipSuccess && p, NQS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+    g_scratch[dev] = (char*)p;
+  }
+  *out = g_scratch[dev];
+  return NQS_OK;
+}
+
+int nqs_psiformer_fwd_noalloc(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S,
+                              int64_t W, double* logpsi, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && logpsi && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  // the largest walker chunk whose activations fit the scratch array
+  int64_t lo = 0, hi = chunk_walkers(md, W, 1, 1);
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (fwd_bytes(md, mid, 1, 0) <= kScratchBytes) lo = mid;
+    else hi = mid - 1;
+  }
+  NQS_REQUIRE(lo >= 1, NQS_ERR_WORKSPACE,
+              "nqs_psiformer_fwd_noalloc: one walker's activations need " + std::to_string(fwd_bytes(md, 1, 1, 0)) +
+                  " bytes, beyond the library's " + std::to_string(kScratchBytes) + "-byte scratch");
+  if (W == 0) return NQS_OK;
+  char* ws = nullptr;
+  NQS_TRY(scratch_address(&ws));
+  if (md.tsize == 8) return fwd_logpsi<double>(md, packed, R, S, W, logpsi, ws, (hipStream_t)stream, lo);
+  return fwd_logpsi<float>(md, packed, R, S, W, logpsi, ws, (hipStream_t)stream, lo);
 }
 
 int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void* packed, const double* R,

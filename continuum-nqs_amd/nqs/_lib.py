@@ -57,6 +57,7 @@ SIGNATURES = [
     ("nqs_pack_params", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("nqs_workspace_bytes", c_i64, [ctypes.POINTER(ModelDesc), c_i32, c_i64]),
     ("nqs_psiformer_fwd", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_psiformer_fwd_noalloc", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     ("nqs_eloc", c_i32, [ctypes.POINTER(ModelDesc), ctypes.POINTER(HamDesc), c_vp, c_vp, c_vp, c_i64, c_vp,
                          c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_psiformer_grad", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp,

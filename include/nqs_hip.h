@@ -135,6 +135,16 @@ int nqs_psiformer_fwd(const nqs_model_desc* m, const void* packed_dev, const dou
                       const int32_t* S_dev, int64_t n_walkers, double* logpsi_dev, void* ws_dev,
                       int64_t ws_bytes, void* stream);
 
+/* ---- log psi apply that allocates nothing and takes no workspace.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and result as
+ * nqs_psiformer_fwd without the workspace: the activations live in a 64 MiB scratch array inside
+ * the library (one per GPU, held until the process ends), and walkers are processed in chunks that
+ * fit it.  Returns NQS_ERR_WORKSPACE when one walker's activations exceed 64 MiB.  Calls that
+ * overlap on one GPU (other streams or threads) share the scratch array and race. */
+int nqs_psiformer_fwd_noalloc(const nqs_model_desc* m, const void* packed_dev, const double* R_dev,
+                              const int32_t* S_dev, int64_t n_walkers, double* logpsi_dev, void* stream);
+
 /* ---- local energy (kinetic by forward Laplacian + Rashba SOC + Zeeman + ext field).
  * eloc_dev (W) complex; logpsi_dev (W) complex or NULL; grad_dev (W, 2N) complex or NULL;
  * lap_dev (W) complex or NULL (the Laplacian of log psi). */
