@@ -92,8 +92,12 @@ static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void*
       NQS_TRY(run_forward<T>(md, packed, jg, R, S, w0, bb, st));
     }
   }
-  // pass 2: every single-spin flip, value + the two tangents of the flipped electron
-  {
+  // pass 2: every single-spin flip, value + the two tangents of the flipped electron.  Only the
+  // Rashba term and the in-plane field components use the flipped configurations: without them
+  // (lambda_R = 0, no ext field, no pinning spin fields) their contribution is exactly zero and
+  // the pass is skipped.
+  const bool flips = h->lambda_r != 0.0 || h->n_field > 0 || h->n_pin > 0;
+  if (flips) {
     char* q = fwd_base;
     FwdBufs<T> b = carve_fwd<T>(md, q, pl.c2 * N, 3, 2);
     for (int64_t w0 = 0; w0 < W; w0 += pl.c2) {
@@ -122,8 +126,8 @@ static int eloc_impl(const Model& md, const nqs_hamiltonian_desc* h, const void*
   a.lp = lp;
   a.grad = grad;
   a.lap = lap;
-  a.lpf = lpf;
-  a.tanf = tanf;
+  a.lpf = flips ? lpf : nullptr;
+  a.tanf = flips ? tanf : nullptr;
   a.out = eloc;
   return launch_eloc_assemble(a, st);
 }
@@ -257,7 +261,7 @@ static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
   w.r = (double*)grab(n * 8);
   w.u = (double*)grab(d->ldo * 8);
   w.v = (double*)grab(n * 8);
-  w.ts = (double*)grab(2 * n * 8);
+  w.ts = (double*)grab(defl_ts_elems(n) * 8);
   w.sc = (double*)grab(8 * 8);
   w.part = (double*)grab(colsum_part_elems(d->ldo) * 8);
   w.red = (double*)grab(4 * 8);
@@ -385,7 +389,7 @@ static MinsrBatchWs minsr_batch_ws(const nqs_minsr_desc* d, int nsys, char* base
     q.et = (double*)grab(n * 8);
     q.u = (double*)grab(d[s].ldo * 8);
     q.v = (double*)grab(n * 8);
-    q.ts = (double*)grab(2 * n * 8);
+    q.ts = (double*)grab(defl_ts_elems(n) * 8);
     q.sc = (double*)grab(8 * 8);
     q.part = (double*)grab(colsum_part_elems(d[s].ldo) * 8);
     q.red = (double*)grab(4 * 8);

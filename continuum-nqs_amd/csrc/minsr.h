@@ -35,6 +35,9 @@ int gram_bf16_batched(const GramBatch& gb, int nsys, int64_t n, int64_t ldc, hip
                       int64_t scratch_elems = 0);
 template <typename T>
 int to_f64_shift(const T* Tin, int64_t ldt, int64_t n, double lam, double* A, int64_t lda, hipStream_t st);
+// ts must hold defl_ts_elems(2W) doubles: the two projections T u, T v (2 x 2W) then the per-slab
+// partial column sums.
+int64_t defl_ts_elems(int64_t n);
 int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, double* sc, hipStream_t st);
 // nb same-size systems share each launch: system z uses A + z sA, Linv + z sL, panel + z sP,
 // info + z (x + z sx in the solve)

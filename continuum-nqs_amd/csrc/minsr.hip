@@ -673,12 +673,14 @@ __global__ __launch_bounds__(256) void k_defl_rows(const double* __restrict__ A,
   }
 }
 // ... plus the column part (r > c): one thread per column and a 256-row slab per blockIdx.y
-// (coalesced over consecutive columns), slabs combined with float64 atomics
-__global__ void k_defl_cols(const double* __restrict__ A, int64_t lda, int64_t W, double* __restrict__ ts) {
+// (coalesced over consecutive columns).  Each slab writes its partial sums (zeros where it holds
+// no rows below the diagonal) to part[slab][2n]; k_defl_cols_sum adds them to ts in slab order, so
+// the result does not depend on the order in which workgroups run.
+__global__ void k_defl_cols(const double* __restrict__ A, int64_t lda, int64_t W, double* __restrict__ part) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = 2 * W;
   const int64_t r0 = (int64_t)blockIdx.y * 256;
-  if (c >= n || r0 + 255 <= c) return;
+  if (c >= n) return;
   double su = 0, sv = 0;
   const int64_t r1 = r0 + 256 < n ? r0 + 256 : n;
   for (int64_t r = (c + 1 > r0 ? c + 1 : r0); r < r1; ++r) {
@@ -686,9 +688,17 @@ __global__ void k_defl_cols(const double* __restrict__ A, int64_t lda, int64_t W
     if (r < W) su += a; else sv += a;
   }
   const double s = 1.0 / sqrt((double)W);
-  if (su != 0.0) atomicAdd(&ts[c], su * s);
-  if (sv != 0.0) atomicAdd(&ts[n + c], sv * s);
+  part[blockIdx.y * 2 * n + c] = su * s;
+  part[blockIdx.y * 2 * n + n + c] = sv * s;
 }
+__global__ void k_defl_cols_sum(const double* __restrict__ part, int64_t n_slabs, int64_t n, double* __restrict__ ts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * n) return;
+  double t = 0.0;
+  for (int64_t y = 0; y < n_slabs; ++y) t += part[y * 2 * n + i];
+  ts[i] += t;
+}
+int64_t defl_ts_elems(int64_t n) { return 2 * n + cdiv(n, 256) * 2 * n; }
 // scalars: alpha = u.Tu, beta = u.Tv, gamma = v.Tv, c = mean diagonal (one workgroup)
 __global__ void k_defl_scalars(const double* __restrict__ A, int64_t lda, int64_t W, const double* __restrict__ ts,
                                double* __restrict__ sc) {
@@ -749,6 +759,9 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
   hipLaunchKernelGGL(k_defl_rows, dim3((unsigned)n), dim3(256), 0, st, A, lda, W, ts);
   NQS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_defl_cols, dim3((unsigned)cdiv(n, 256), (unsigned)cdiv(n, 256)), dim3(256), 0, st, A, lda, W,
+                     ts + 2 * n);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_defl_cols_sum, dim3((unsigned)cdiv(2 * n, 256)), dim3(256), 0, st, ts + 2 * n, cdiv(n, 256), n,
                      ts);
   NQS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_defl_scalars, dim3(1), dim3(1024), 0, st, A, lda, W, ts, sc);

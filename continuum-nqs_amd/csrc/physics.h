@@ -19,7 +19,8 @@ struct EAssembleArgs {
   const double* lp;     // (W) complex: log psi
   const double* grad;   // (W, 2N) complex
   const double* lap;    // (W) complex
-  const double* lpf;    // (W N) complex: log psi of the flipped configurations
+  const double* lpf;    // (W N) complex: log psi of the flipped configurations (NULL: none, the
+                        // Rashba and in-plane field terms are zero)
   const double* tanf;   // (W N, 2) complex: d_{x_i}, d_{y_i} log psi of flipped configuration i
   double* out;          // (W) complex
 };

@@ -165,6 +165,7 @@ __global__ void k_eloc_assemble(EAssembleArgs a) {
       if (a.n_pin > 0) pins_at(a.pin, a.n_pin, a.A, x, y, &epin, f);
       ez -= f[2] * (1.0 - 2.0 * (double)s);
     }
+    if (!a.lpf) continue;   // no flipped configurations: no Rashba term, no in-plane fields
     const cd Lf = mk(a.lpf[2 * cfg], a.lpf[2 * cfg + 1]);
     const cd ratio = cexp_(Lf - L0);
     cd ax = ratio * mk(a.tanf[(cfg * 2 + 0) * 2], a.tanf[(cfg * 2 + 0) * 2 + 1]);

@@ -12,8 +12,9 @@ library's own:
              d Im log psi / d theta] (a reverse sweep that recomputes each transformer block's
              activations from its input instead of keeping them for every walker); the vector-
              Jacobian product is g_re^T O_re + g_im^T O_im.
-  d/dR     : the forward-mode tangent channels of nqs_eloc give grad log psi per coordinate;
-             the product is g_re Re grad + g_im Im grad.
+  d/dR     : the forward-mode tangent channels of nqs_eloc (called without Rashba term or fields,
+             so no spin-flipped configuration is evaluated) give grad log psi per coordinate; the
+             product is g_re Re grad + g_im Im grad.
 
 S (integer spins) is not differentiable.  theta must be the flat real_ravel vector (float64).
 """
@@ -49,7 +50,9 @@ class PsiformerLogPsi(torch.autograd.Function):
             coef = torch.cat([g[:, 0], g[:, 1]]).to(O.data.dtype)
             d_theta = (coef @ O.data)[:P].to(torch.float64)
         if ctx.needs_input_grad[1]:
-            _, _, grad, _ = LocalEnergy(ctx.lp).parts(Rw, Sw)    # (W, 2N) complex
+            # lambda_R = 0 and no fields: the library runs only the forward-mode pass (value, 2N
+            # tangents, Laplacian) and skips the spin-flipped configurations
+            _, _, grad, _ = LocalEnergy(ctx.lp, lambda_r=0.0).parts(Rw, Sw)    # (W, 2N) complex
             d = g[:, :1] * grad.real + g[:, 1:] * grad.imag
             d_R = d.reshape(ctx.r_shape)
         return d_theta, d_R, None, None

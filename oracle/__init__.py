@@ -20,9 +20,11 @@ How each function is pinned (beyond retyping the reference):
   * bands: E_gs against the analytic band formula and brute-force eigh of the 2x2 Bloch
     Hamiltonian; spiral basis against its own boundary-leakage certificate.
   * psiformer: zero backflow == Slater determinant (closed form); antisymmetry; permutation
-    equivariance of the backbone; LayerNorm / GELU / softmax checked against torch.nn
-    reference implementations of the same library definitions (flax LayerNorm eps=1e-6 with
-    E[x^2]-E[x]^2 variance, tanh-GELU, max-shifted softmax).
+    equivariance of the backbone; the whole backbone (embed, pre-norm blocks, final LayerNorm)
+    equal to torch.nn.TransformerEncoderLayer(norm_first=True) blocks and torch.nn.LayerNorm with
+    the same weights at d = 128 / 8 heads, d = 32 / 4 heads and depth 0; mha equal to
+    torch.nn.functional.multi_head_attention_forward; the backflow reshape against finite
+    differences of a matrix built independently (tests/test_oracle.py).
   * local_energy: zero-variance certificates E_loc == E_gs (Rashba, polarized, spiral), which
     fail for a wrong sign / index in kinetic, SOC or Zeeman.
   * ewald: eta independence, periodicity, evenness, V(r)*r -> alpha, alpha linearity,
@@ -30,9 +32,12 @@ How each function is pinned (beyond retyping the reference):
   * sampler: accept rule checked on hand-built cases; uniform density of a plane-wave det.
   * minsr: push-through identity (sample space == parameter space); O against central
     finite differences.
-  * ed: non-interacting limit equals the closed-form band energy; Coulomb matrix elements
-    against direct quadrature.
-Parity of the backbone's floating-point values with flax itself is not pinned by any test
-the reference holds (the reference's tests multiply the backbone by a zero backflow); the
-restatement follows the flax 0.8.5 / jax 0.4.31 library definitions named above.
+  * ed: the non-interacting limit equals the closed-form band energy; at lambda_R = h_z = 0 the
+    antisymmetrised spinful ground state equals the independently built relative-coordinate
+    problem; the spectrum is invariant under the sign convention of the Rashba term; zeta_M
+    matches the Ewald restatement.
+The backbone's values are pinned against torch's library transformer layers (same definitions
+as flax 0.8.5 LayerNorm / Dense and jax.nn.gelu's tanh form), not against flax itself, which is
+not installable here.  cpu_step.py composes these pieces into the reference's step; the
+two-electron trajectory it writes (tests/golden/) is the GPU path's whole-step parity target.
 """

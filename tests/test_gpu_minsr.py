@@ -111,11 +111,12 @@ def test_minsr_short_last_super_block(W):
     assert float(torch.max(torch.abs(d - d_lu))) / den < 1e-6
 
 
-@pytest.mark.parametrize("gram,tol,W", [("fp32", 1e-3, 64), ("bf16", 5e-2, 64), ("fp32", 1e-3, 384),
-                                        ("bf16", 5e-2, 384)])
+@pytest.mark.parametrize("gram,tol,W", [("fp32", 5e-4, 64), ("bf16", 5e-3, 64), ("fp32", 5e-4, 384),
+                                        ("bf16", 5e-3, 384)])
 def test_low_precision_gram_close_to_fp64(gram, tol, W):
     """bf16 / f32 MFMA Gram (f32 accumulate) + float64 Cholesky + one refinement step from the
-    float32 O: update vector within tol (relative, max norm) of the float64 MinSR.  W = 384
+    float32 O: update vector within tol (relative, max norm) of the float64 MinSR; the bounds are
+    about ten times the measured errors (fp32 2.2e-5 / 4.8e-5, bf16 4.1e-5 / 4.8e-4).  W = 384
     (2W = 768 = 3 x 256) takes the 256-tile bf16 Gram on the blocked operand layout."""
     g, arch, theta, m64 = _pair("base", seed=5)
     _, _, _, m32 = _pair("base", precision="mixed", seed=5)
@@ -126,7 +127,25 @@ def test_low_precision_gram_close_to_fp64(gram, tol, W):
     d64 = nms.minsr_direction(O64, eps, 1e-3, gram="fp64").cpu()
     d = nms.minsr_direction(O32, eps, 1e-3, gram=gram, refine=1).cpu()
     rel = float(torch.max(torch.abs(d - d64)) / torch.max(torch.abs(d64)))
+    print(f"{gram} Gram, W={W}: dtheta rel {rel:.3e} (bound {tol:.0e})", flush=True)
     assert rel < tol, rel
+
+
+@pytest.mark.parametrize("gram", ["fp64", "fp32", "bf16"])
+def test_minsr_direction_is_deterministic(gram):
+    """Two identical calls give bit-identical directions: every reduction of the solve (column sums,
+    Gram tail, centring deflation, Cholesky, triangular solves, refinement) sums in a fixed order.
+    W = 1024 gives the deflation's column part four 256-row slabs per column."""
+    g, arch, theta, m = _pair("base", precision="fp64" if gram == "fp64" else "mixed", seed=7)
+    R, S = _configs(1024, 5, 7.0, 7)
+    eps = torch.tensor(np.random.default_rng(7).normal(size=1024) + 1j * np.random.default_rng(8).normal(size=1024))
+    O, _ = nms.grad_params(make_logpsi(m, theta.cuda()), R, S)
+    base = O.data.clone()
+    d1 = nms.minsr_direction(O, eps, 1e-3, gram=gram)
+    O.data.copy_(base)
+    d2 = nms.minsr_direction(O, eps, 1e-3, gram=gram)
+    assert int(nms.minsr_direction.last_info.item()) == 0
+    assert torch.equal(d1, d2)
 
 
 def test_bf16_gram_tail_split_matches_unsplit():
