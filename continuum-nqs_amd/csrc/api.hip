@@ -559,6 +559,10 @@ int64_t nqs_workspace_bytes(const nqs_model_desc* m, int32_t op, int64_t W) {
     }
     case NQS_OP_GRAD:
       return grad_bytes(md, grad_chunk(md, W));
+    case NQS_OP_VJP: {
+      const int64_t c = grad_chunk(md, W);
+      return grad_bytes(md, c) + vjp_bytes(md, c);
+    }
     case NQS_OP_MCMC: {
       McmcPlan p = mcmc_plan(md, W);
       return al(p.fwd) + al(W * md.N * 16) + al(W * md.N * 4) + al(W * 16);
@@ -662,6 +666,33 @@ int nqs_spin_estimators(const nqs_model_desc* m, const void* packed, const doubl
   NQS_TRY(check_ws(m, NQS_OP_SPIN, W, ws_bytes));
   if (md.tsize == 8) return spin_impl<double>(md, packed, R, S, W, spins, (char*)ws, (hipStream_t)stream);
   return spin_impl<float>(md, packed, R, S, W, spins, (char*)ws, (hipStream_t)stream);
+}
+
+// g = 0 by a kernel (no memset node, so the call may be captured into a graph)
+__global__ void k_zero_f64(double* __restrict__ g, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) g[q] = 0.0;
+}
+
+int nqs_psiformer_vjp(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,
+                      const double* cot, double* g, double* logpsi, void* ws, int64_t ws_bytes, void* stream) {
+  Model md;
+  NQS_TRY(build_model(m, md));
+  NQS_REQUIRE(packed && R && S && cot && g && W >= 0, NQS_ERR_ARG, "null pointer or negative walker count");
+  NQS_TRY(check_ws(m, NQS_OP_VJP, W, ws_bytes));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)cdiv(std::max<int64_t>(md.P, 1), 256)), dim3(256), 0, st, g, md.P);
+  NQS_LAUNCH_CHECK();
+  const int64_t cw = grad_chunk(md, W);
+  for (int64_t w0 = 0; w0 < W; w0 += cw) {
+    const int64_t n = std::min(cw, W - w0);
+    double* lp = logpsi ? logpsi + 2 * w0 : nullptr;
+    if (md.tsize == 8)
+      NQS_TRY(run_backward<double>(md, packed, R, S, w0, n, W, nullptr, 0, lp, (char*)ws, st, cot, g));
+    else
+      NQS_TRY(run_backward<float>(md, packed, R, S, w0, n, W, nullptr, 0, lp, (char*)ws, st, cot, g));
+  }
+  return NQS_OK;
 }
 
 int nqs_psiformer_grad(const nqs_model_desc* m, const void* packed, const double* R, const int32_t* S, int64_t W,

@@ -432,12 +432,137 @@ int launch_attn_bwd(int64_t n_cfg, int N, int d, int nh, const T* qkv, const T* 
   return NQS_OK;
 }
 
+// ------------------------------------------------------------------ vector-Jacobian product
+template <typename T>
+__global__ void k_vjp_fold(int64_t n_cfg, int64_t per_cfg, int N, int64_t w0, const double* __restrict__ cot,
+                           T* __restrict__ ybar) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_cfg * per_cfg) return;
+  const int64_t cfg = q / per_cfg, r = q % per_cfg;
+  T* y0 = ybar + cfg * 2 * per_cfg + r;
+  T* y1 = y0 + per_cfg;
+  const double gr = cot[2 * (w0 + cfg)], gi = cot[2 * (w0 + cfg) + 1];
+  *y0 = (T)(gr * (double)*y0 + gi * (double)*y1);
+  *y1 = (T)0;
+}
+
+template <typename T>
+int launch_vjp_fold(int64_t n_cfg, int N, int width, int64_t w0, const double* cot, T* ybar, hipStream_t st) {
+  const int64_t per = (int64_t)N * width;
+  if (n_cfg * per == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_vjp_fold<T>, dim3((unsigned)cdiv(n_cfg * per, 256)), dim3(256), 0, st, n_cfg, per, N, w0, cot,
+                     ybar);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+int64_t vjp_slices(int64_t n_cfg) { return std::max<int64_t>(1, std::min<int64_t>(256, n_cfg)); }
+
+// part[slice][i][j] = sum over cfg in the slice, t of X[cfg][t][i] G[cfg][0][t][j]; one thread per
+// output column j and 16 rows i (X values are wave-uniform and come through the scalar cache).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vjp_wsum(WgradArgs<T> a, T* __restrict__ part, int64_t per_slice) {
+  constexpr int IB = 16;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int i0 = blockIdx.y * IB;
+  const int64_t s = blockIdx.z;
+  const int64_t c0 = s * per_slice, c1 = std::min(a.n_cfg, c0 + per_slice);
+  const int N = a.N;
+  const int in = a.X ? a.in : 1;
+  T acc[IB];
+#pragma unroll
+  for (int ii = 0; ii < IB; ++ii) acc[ii] = (T)0;
+  if (j < a.out) {
+    for (int64_t cfg = c0; cfg < c1; ++cfg) {
+      const T* G = a.G + (cfg * 2 * N) * a.ldg + j;
+      const T* X = a.X ? a.X + (cfg * N) * a.ldx + i0 : nullptr;
+      for (int t = 0; t < N; ++t) {
+        const T gv = G[(int64_t)t * a.ldg];
+        if (X) {
+#pragma unroll
+          for (int ii = 0; ii < IB; ++ii)
+            if (i0 + ii < in) acc[ii] += X[(int64_t)t * a.ldx + ii] * gv;
+        } else {
+          acc[0] += gv;
+        }
+      }
+    }
+  }
+  if (j >= a.out) return;
+#pragma unroll
+  for (int ii = 0; ii < IB; ++ii)
+    if (i0 + ii < in) part[(s * in + i0 + ii) * (int64_t)a.out + j] = acc[ii];
+}
+
+template <typename T>
+__global__ void k_vjp_wsum_reduce(WgradArgs<T> a, const T* __restrict__ part, int64_t n_slices, double* __restrict__ g) {
+  const int in = a.X ? a.in : 1;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)in * a.out) return;
+  const int i = (int)(q / a.out), j = (int)(q % a.out);
+  double t = 0.0;
+  for (int64_t s = 0; s < n_slices; ++s) t += (double)part[(s * in + i) * (int64_t)a.out + j];
+  const int sg = j / a.seg_w, jj = j % a.seg_w;
+  g[a.seg_base[sg] + (int64_t)i * a.seg_w + jj] += t;
+}
+
+template <typename T>
+int launch_vjp_wsum(const WgradArgs<T>& a, T* part, int64_t n_slices, double* g, hipStream_t st) {
+  if (a.n_cfg == 0) return NQS_OK;
+  NQS_REQUIRE(a.out % a.seg_w == 0 && a.out / a.seg_w <= 3, NQS_ERR_ARG, "bad output segmentation");
+  const int in = a.X ? a.in : 1;
+  const int64_t per = cdiv(a.n_cfg, n_slices);
+  const int64_t ns = cdiv(a.n_cfg, per);
+  hipLaunchKernelGGL(k_vjp_wsum<T>, dim3((unsigned)cdiv(a.out, 256), (unsigned)cdiv(in, 16), (unsigned)ns), dim3(256),
+                     0, st, a, part, per);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_vjp_wsum_reduce<T>, dim3((unsigned)cdiv((int64_t)in * a.out, 256)), dim3(256), 0, st, a, part,
+                     ns, g);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+template <typename T>
+__global__ void k_vjp_osm_reduce(const T* __restrict__ osm, int64_t ldsm, int64_t n_cfg, OsmMap map,
+                                 const double* __restrict__ cot, int64_t w0, double* __restrict__ g) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= map.start[map.n]) return;
+  int sgm = 0;
+  while (sgm + 1 < map.n && q >= map.start[sgm + 1]) ++sgm;
+  double t = 0.0;
+  for (int64_t cfg = 0; cfg < n_cfg; ++cfg) {
+    const double v0 = (double)osm[cfg * ldsm + q];
+    if (cot) {
+      const double v1 = (double)osm[(n_cfg + cfg) * ldsm + q];
+      t += cot[2 * (w0 + cfg)] * v0 + cot[2 * (w0 + cfg) + 1] * v1;
+    } else {
+      t += v0;
+    }
+  }
+  g[map.dst[sgm] + q - map.start[sgm]] += t;
+}
+
+template <typename T>
+int launch_vjp_osm_reduce(const T* osm, int64_t ldsm, int64_t n_cfg, const OsmMap& map, const double* cot,
+                          int64_t w0, double* g, hipStream_t st) {
+  const int64_t ncol = map.n > 0 ? map.start[map.n] : 0;
+  if (n_cfg == 0 || ncol == 0) return NQS_OK;
+  hipLaunchKernelGGL(k_vjp_osm_reduce<T>, dim3((unsigned)cdiv(ncol, 64)), dim3(64), 0, st, osm, ldsm, n_cfg, map, cot,
+                     w0, g);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
 #define NQS_INST_B(T)                                                                                          \
   template int launch_wgrad<T>(const WgradArgs<T>&, hipStream_t);                                            \
   template int launch_bgrad<T>(const WgradArgs<T>&, hipStream_t);                                            \
   template int launch_ln_bwd<T>(const LnBwdArgs<T>&, hipStream_t);                                           \
   template int launch_gelu_bwd<T>(int64_t, int64_t, const T*, const T*, T*, hipStream_t);                    \
-  template int launch_attn_bwd<T>(int64_t, int, int, int, const T*, const T*, T*, hipStream_t);
+  template int launch_attn_bwd<T>(int64_t, int, int, int, const T*, const T*, T*, hipStream_t);                \
+  template int launch_vjp_fold<T>(int64_t, int, int, int64_t, const double*, T*, hipStream_t);               \
+  template int launch_vjp_wsum<T>(const WgradArgs<T>&, T*, int64_t, double*, hipStream_t);                   \
+  template int launch_vjp_osm_reduce<T>(const T*, int64_t, int64_t, const OsmMap&, const double*, int64_t,   \
+                                        double*, hipStream_t);
 NQS_INST_B(double)
 NQS_INST_B(float)
 

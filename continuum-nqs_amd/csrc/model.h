@@ -61,10 +61,14 @@ template <typename T>
 FwdBufs<T> carve_fwd(const Model& md, char*& p, int64_t n_cfg, int C, int nt);
 
 // Per-walker parameter gradient: O rows (Re at w, Im at W_total + w) for walkers
-// [w0, w0 + n_cfg), plus log psi of those walkers.
+// [w0, w0 + n_cfg), plus log psi of those walkers.  With vjp_g != NULL it computes instead the
+// vector-Jacobian product vjp_g += sum_w (cot[2w] d Re L_w / d theta + cot[2w+1] d Im L_w / d theta)
+// over these walkers (O and ldo unused; the workspace needs vjp_bytes more).
 int64_t grad_bytes(const Model& md, int64_t n_cfg);
+int64_t vjp_bytes(const Model& md, int64_t n_cfg);
 template <typename T>
 int run_backward(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t w0,
-                 int64_t n_cfg, int64_t W_total, T* O, int64_t ldo, double* out_lp, char* ws, hipStream_t st);
+                 int64_t n_cfg, int64_t W_total, T* O, int64_t ldo, double* out_lp, char* ws, hipStream_t st,
+                 const double* vjp_cot = nullptr, double* vjp_g = nullptr);
 
 }  // namespace nqs

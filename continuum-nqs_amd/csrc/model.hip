@@ -416,9 +416,26 @@ int64_t grad_bytes(const Model& md, int64_t n_cfg) {
   return b;
 }
 
+// Extra workspace of the vector-Jacobian product: the slice partials of the weight sums and the
+// compact per-walker rows of the LayerNorm and determinant-head leaves.
+static int64_t vjp_det_cols(const Model& md) {
+  const Leaf* jl = find_leaf(md, "params/jas_c_same");
+  const Leaf* jc = find_leaf(md, "params/jastrow_loga");
+  return (md.K > 1 ? 2LL * md.K : 0) + (find_leaf(md, "params/dW") ? 2LL * md.N * md.n_g * 2 : 0) +
+         (jl ? 2 * jl->size : 0) + (jc ? 1 : 0);
+}
+int64_t vjp_bytes(const Model& md, int64_t n_cfg) {
+  const int64_t d = md.d, ffd = (int64_t)md.ff * md.d;
+  const int64_t maxio = std::max<int64_t>({(int64_t)md.kin * d, d * 2LL * md.KN, ffd * d, 3 * d * d});
+  return al(vjp_slices(n_cfg) * maxio * md.tsize) + al(2 * n_cfg * std::max<int64_t>(1, vjp_det_cols(md)) * md.tsize) +
+         al(2 * n_cfg * 2 * d * md.tsize);
+}
+
 template <typename T>
 int run_backward(const Model& md, const void* packed, const double* R, const int32_t* S, int64_t w0, int64_t n_cfg,
-                 int64_t W_total, T* O, int64_t ldo, double* out_lp, char* ws, hipStream_t st) {
+                 int64_t W_total, T* O, int64_t ldo, double* out_lp, char* ws, hipStream_t st, const double* vjp_cot,
+                 double* vjp_g) {
+  const bool vjp = vjp_g != nullptr;
   const T* pk = (const T*)packed;
   const int N = md.N, d = md.d, ffd = md.ff * md.d, KN = md.KN, depth = md.depth;
   const int64_t rows = n_cfg * N;
@@ -444,6 +461,21 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
   T* gb = grab(2 * rows * ffd);
   T* hb = grab(2 * rows * d);
   T* qb = grab(2 * rows * std::max(3 * d, 2 * KN));
+  // vector-Jacobian product: slice partials and compact small-leaf rows after the backward buffers
+  const int64_t n_slices = vjp_slices(n_cfg);
+  const int64_t ld_det = std::max<int64_t>(1, vjp_det_cols(md)), ld_ln = 2LL * d;
+  T* part = nullptr;
+  T* osm_det = nullptr;
+  T* osm_ln = nullptr;
+  if (vjp) {
+    const int64_t maxio = std::max<int64_t>({(int64_t)md.kin * d, (int64_t)d * 2 * KN, (int64_t)ffd * d, 3LL * d * d});
+    part = grab(n_slices * maxio);
+    osm_det = grab(2 * n_cfg * ld_det);
+    osm_ln = grab(2 * n_cfg * ld_ln);
+  }
+  // the O-row kernels index rows c * W_total + w0 + cfg; a compact buffer is addressed through a
+  // base shifted back by w0 rows, with W_total = n_cfg
+  auto shifted = [&](T* buf, int64_t ld) { return (T*)((uintptr_t)buf - (uintptr_t)(w0 * ld * (int64_t)sizeof(T))); };
   const JetGeom jg{n_cfg, N, 1, 0, 0, 0};
   const int64_t per = N, blen = N;
   auto leaf = [&](const std::string& n) -> int64_t {
@@ -471,7 +503,7 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
     a.seg_w = seg_w;
     return a;
   };
-  auto lnb = [&](const T* x, const T* ybar, T* xbar, const T* scale, int acc, const std::string& name) {
+  auto lnb = [&](const T* x, const T* ybar, T* xbar, const T* scale, int acc, const std::string& name) -> int {
     LnBwdArgs<T> a;
     a.x = x;
     a.ybar = ybar;
@@ -482,12 +514,32 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
     a.accumulate = acc;
     a.n_cfg = n_cfg;
     a.w0 = w0;
-    a.W_total = W_total;
-    a.O = O;
-    a.ldo = ldo;
-    a.off_scale = leaf(name + "/scale");
-    a.off_bias = leaf(name + "/bias");
-    return launch_ln_bwd<T>(a, st);
+    a.W_total = vjp ? n_cfg : W_total;
+    a.O = vjp ? shifted(osm_ln, ld_ln) : O;
+    a.ldo = vjp ? ld_ln : ldo;
+    a.off_scale = vjp ? 0 : leaf(name + "/scale");
+    a.off_bias = vjp ? d : leaf(name + "/bias");
+    NQS_TRY(launch_ln_bwd<T>(a, st));
+    if (vjp) {
+      OsmMap mp;
+      mp.n = 2;
+      mp.start[0] = 0;
+      mp.start[1] = d;
+      mp.start[2] = 2 * d;
+      mp.dst[0] = leaf(name + "/scale");
+      mp.dst[1] = leaf(name + "/bias");
+      NQS_TRY(launch_vjp_osm_reduce<T>(osm_ln, ld_ln, n_cfg, mp, nullptr, w0, vjp_g, st));
+    }
+    return NQS_OK;
+  };
+  // weight and bias gradients: O rows, or (vector-Jacobian product) sums over the rows into g
+  auto wgrad = [&](const WgradArgs<T>& a, hipStream_t) -> int {
+    return vjp ? launch_vjp_wsum<T>(a, part, n_slices, vjp_g, st) : launch_wgrad<T>(a, st);
+  };
+  auto bgrad = [&](WgradArgs<T> a, hipStream_t) -> int {
+    if (!vjp) return launch_bgrad<T>(a, st);
+    a.X = nullptr;
+    return launch_vjp_wsum<T>(a, part, n_slices, vjp_g, st);
   };
   auto gemm = [&](const T* A, const T* B, T* C, int64_t M, int64_t Nn, int64_t K, const T* bias, T beta) {
     return gemm_nt<T>(gargs<T>(md, A, B, C, M, Nn, K, K, K, Nn, bias, per, blen, beta), 1, st);
@@ -525,11 +577,40 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
   off.jas_opp = leaf("params/jas_c_opp");
   off.jas_loga = leaf("params/jastrow_loga");
   const DetParams dp = det_params(md, packed);
-  NQS_TRY(launch_det_bwd<T>(n_cfg, dp, R, S, w0, bfo, qb, out_lp, O, ldo, W_total, off, st));
+  if (!vjp) {
+    NQS_TRY(launch_det_bwd<T>(n_cfg, dp, R, S, w0, bfo, qb, out_lp, O, ldo, W_total, off, st));
+  } else {
+    // the determinant-head leaves into compact rows (both channels), weighted by the cotangent
+    DetBwdOffsets so;
+    OsmMap mp;
+    mp.n = 0;
+    int64_t c = 0;
+    auto seg = [&](int64_t orig, int64_t len, int64_t& slot) {
+      if (orig < 0) {
+        slot = -1;
+        return;
+      }
+      slot = c;
+      mp.start[mp.n] = c;
+      mp.dst[mp.n] = orig;
+      ++mp.n;
+      c += len;
+    };
+    const Leaf* lj = find_leaf(md, "params/jas_c_same");
+    seg(off.dlw, 2LL * md.K, so.dlw);
+    seg(off.dW, 2LL * N * md.n_g * 2, so.dW);
+    seg(off.jas_same, lj ? lj->size : 0, so.jas_same);
+    seg(off.jas_opp, lj ? lj->size : 0, so.jas_opp);
+    seg(off.jas_loga, 1, so.jas_loga);
+    mp.start[mp.n] = c;
+    NQS_TRY(launch_det_bwd<T>(n_cfg, dp, R, S, w0, bfo, qb, out_lp, shifted(osm_det, ld_det), ld_det, n_cfg, so, st));
+    NQS_TRY(launch_vjp_osm_reduce<T>(osm_det, ld_det, n_cfg, mp, vjp_cot, w0, vjp_g, st));
+    NQS_TRY(launch_vjp_fold<T>(n_cfg, N, 2 * KN, w0, vjp_cot, qb, st));
+  }
   // ---- backflow Dense (2KN real outputs = [Re | Im]; theta: re block then im block)
   const int64_t bk = leaf("params/backflow/kernel"), bb = leaf("params/backflow/bias");
-  NQS_TRY(launch_wgrad<T>(wg(xf, d, d, qb, 2 * KN, {bk, bk + (int64_t)d * KN}, KN), st));
-  NQS_TRY(launch_bgrad<T>(wg(xf, d, d, qb, 2 * KN, {bb}, 2 * KN), st));
+  NQS_TRY(wgrad(wg(xf, d, d, qb, 2 * KN, {bk, bk + (int64_t)d * KN}, KN), st));
+  NQS_TRY(bgrad(wg(xf, d, d, qb, 2 * KN, {bb}, 2 * KN), st));
   NQS_TRY(gemm(qb, pk + md.bfW, hb, 2 * rows, d, 2 * KN, nullptr, (T)0));
   NQS_TRY(lnb(xs[depth], hb, xb, pk + md.lnf_s, 0, "params/LayerNorm_0"));
   // ---- blocks in reverse, recomputing each block's activations from its checkpoint
@@ -539,35 +620,35 @@ int run_backward(const Model& md, const void* packed, const double* R, const int
     NQS_TRY(block_fwd(l, xs[l], nullptr));
     // MLP out: x_out = xmid + Gv W1 + b1
     NQS_TRY(gemm(xb, pk + P.w1, gb, 2 * rows, ffd, d, nullptr, (T)0));
-    NQS_TRY(launch_wgrad<T>(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/kernel")}, d), st));
-    NQS_TRY(launch_bgrad<T>(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/bias")}, d), st));
+    NQS_TRY(wgrad(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/kernel")}, d), st));
+    NQS_TRY(bgrad(wg(Gv, ffd, ffd, xb, d, {leaf(b + "Dense_1/bias")}, d), st));
     NQS_TRY(launch_gelu_bwd<T>(n_cfg, (int64_t)N * ffd, U, gb, gb, st));
     NQS_TRY(gemm(gb, pk + P.w0, hb, 2 * rows, d, ffd, nullptr, (T)0));
-    NQS_TRY(launch_wgrad<T>(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/kernel")}, ffd), st));
-    NQS_TRY(launch_bgrad<T>(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/bias")}, ffd), st));
+    NQS_TRY(wgrad(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/kernel")}, ffd), st));
+    NQS_TRY(bgrad(wg(hB, d, d, gb, ffd, {leaf(b + "Dense_0/bias")}, ffd), st));
     NQS_TRY(lnb(xmid, hb, xb, pk + P.ln1_s, 1, b + "LayerNorm_1"));
     // attention out: xmid = x_in + attA Wo + bo
     NQS_TRY(gemm(xb, pk + P.o, hb, 2 * rows, d, d, nullptr, (T)0));
-    NQS_TRY(launch_wgrad<T>(wg(attA, d, d, xb, d, {leaf(b + "attn/o/kernel")}, d), st));
-    NQS_TRY(launch_bgrad<T>(wg(attA, d, d, xb, d, {leaf(b + "attn/o/bias")}, d), st));
+    NQS_TRY(wgrad(wg(attA, d, d, xb, d, {leaf(b + "attn/o/kernel")}, d), st));
+    NQS_TRY(bgrad(wg(attA, d, d, xb, d, {leaf(b + "attn/o/bias")}, d), st));
     NQS_TRY(launch_attn_bwd<T>(n_cfg, N, d, md.nh, qkv, hb, qb, st));
     NQS_TRY(gemm(qb, pk + P.qkv, hb, 2 * rows, d, 3 * d, nullptr, (T)0));
-    NQS_TRY(launch_wgrad<T>(wg(hA, d, d, qb, 3 * d,
+    NQS_TRY(wgrad(wg(hA, d, d, qb, 3 * d,
                                {leaf(b + "attn/q/kernel"), leaf(b + "attn/k/kernel"), leaf(b + "attn/v/kernel")}, d),
                             st));
-    NQS_TRY(launch_bgrad<T>(
+    NQS_TRY(bgrad(
         wg(hA, d, d, qb, 3 * d, {leaf(b + "attn/q/bias"), leaf(b + "attn/k/bias"), leaf(b + "attn/v/bias")}, d), st));
     NQS_TRY(lnb(xs[l], hb, xb, pk + P.ln0_s, 1, b + "LayerNorm_0"));
   }
   // ---- embedding
-  NQS_TRY(launch_wgrad<T>(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/kernel")}, d), st));
-  NQS_TRY(launch_bgrad<T>(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/bias")}, d), st));
+  NQS_TRY(wgrad(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/kernel")}, d), st));
+  NQS_TRY(bgrad(wg(X0, md.kpad, md.kin, xb, d, {leaf("params/embed/bias")}, d), st));
   return NQS_OK;
 }
 
 template int run_backward<double>(const Model&, const void*, const double*, const int32_t*, int64_t, int64_t, int64_t,
-                                  double*, int64_t, double*, char*, hipStream_t);
+                                  double*, int64_t, double*, char*, hipStream_t, const double*, double*);
 template int run_backward<float>(const Model&, const void*, const double*, const int32_t*, int64_t, int64_t, int64_t,
-                                 float*, int64_t, double*, char*, hipStream_t);
+                                 float*, int64_t, double*, char*, hipStream_t, const double*, double*);
 
 }  // namespace nqs

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libnqs_hip.so")
 NQS_OK, NQS_ERR_ARG, NQS_ERR_HIP, NQS_ERR_WORKSPACE, NQS_ERR_NUMERIC, NQS_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 NQS_FP64, NQS_MIXED = 0, 1
 NQS_GEMM_BF16X3, NQS_GEMM_FP32, NQS_GEMM_BF16 = 0, 1, 2
-OP_LOGPSI, OP_ELOC, OP_MCMC, OP_GRAD, OP_SPIN = 0, 1, 2, 3, 4
+OP_LOGPSI, OP_ELOC, OP_MCMC, OP_GRAD, OP_SPIN, OP_VJP = 0, 1, 2, 3, 4, 5
 
 c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
@@ -62,6 +62,8 @@ SIGNATURES = [
                          c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_psiformer_grad", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp,
                                    c_i64, c_vp]),
+    ("nqs_psiformer_vjp", c_i32, [ctypes.POINTER(ModelDesc), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64,
+                                  c_vp]),
     ("nqs_minsr_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
     ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),

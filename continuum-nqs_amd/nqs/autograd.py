@@ -8,20 +8,44 @@ src/nqs/minsr.py:56-65, and `jax.grad` / `jax.hessian` in the local energy,
 src/nqs/local_energy.py:52-81).  Here the forward is nqs_psiformer_fwd and the backward is the
 library's own:
 
-  d/dtheta : nqs_psiformer_grad writes the per-walker rows O = [d Re log psi / d theta;
-             d Im log psi / d theta] (a reverse sweep that recomputes each transformer block's
-             activations from its input instead of keeping them for every walker); the vector-
-             Jacobian product is g_re^T O_re + g_im^T O_im.
+  d/dtheta : nqs_psiformer_vjp, a reverse sweep that recomputes each transformer block's
+             activations from its input instead of keeping them for every walker, with the
+             cotangent (g_re, g_im) folded in at the determinant head and the weight gradients
+             summed over walkers and tokens on the device: sum_w g_re d Re L_w + g_im d Im L_w,
+             without forming the per-walker rows O.
   d/dR     : the forward-mode tangent channels of nqs_eloc (called without Rashba term or fields,
              so no spin-flipped configuration is evaluated) give grad log psi per coordinate; the
              product is g_re Re grad + g_im Im grad.
 
 S (integer spins) is not differentiable.  theta must be the flat real_ravel vector (float64).
 """
+import ctypes
+
 import torch
 
 from . import _lib
 from .psiformer import LogPsi, as_walkers
+
+
+def vjp_theta(logpsi, R, S, cot, ws_tag="vjp"):
+    """sum_w [cot_w.re d Re log psi_w / d theta + cot_w.im d Im log psi_w / d theta] (P,) float64,
+    by nqs_psiformer_vjp; cot: (W,) complex or (W, 2) real."""
+    lib = _lib.load()
+    Rw, Sw, _ = as_walkers(R, S)
+    W = Rw.shape[0]
+    cot = torch.as_tensor(cot, device=Rw.device)
+    if cot.is_complex():
+        cot = torch.view_as_real(cot.to(torch.complex128))
+    cot = cot.reshape(W, 2).to(torch.float64).contiguous()
+    m = logpsi.model
+    d = m.desc()
+    g = torch.empty(m.n_params(), dtype=torch.float64, device=Rw.device)
+    nb = lib.nqs_workspace_bytes(ctypes.byref(d), _lib.OP_VJP, W)
+    ws = _lib.workspace(nb, ws_tag)
+    _lib.check(lib.nqs_psiformer_vjp(ctypes.byref(d), _lib.ptr(logpsi.packed), _lib.ptr(Rw), _lib.ptr(Sw), W,
+                                     _lib.ptr(cot), _lib.ptr(g), None, _lib.ptr(ws), ws.numel(), _lib.stream()),
+               "nqs_psiformer_vjp")
+    return g
 
 
 class PsiformerLogPsi(torch.autograd.Function):
@@ -38,17 +62,13 @@ class PsiformerLogPsi(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        from .minsr import grad_params
         from .local_energy import LocalEnergy
         Rw, Sw = ctx.saved_tensors
         W = Rw.shape[0]
         g = g.reshape(W, 2).to(torch.float64)
         d_theta = d_R = None
         if ctx.needs_input_grad[0]:
-            O, _ = grad_params(ctx.lp, Rw, Sw)
-            P = O.P
-            coef = torch.cat([g[:, 0], g[:, 1]]).to(O.data.dtype)
-            d_theta = (coef @ O.data)[:P].to(torch.float64)
+            d_theta = vjp_theta(ctx.lp, Rw, Sw, g)
         if ctx.needs_input_grad[1]:
             # lambda_R = 0 and no fields: the library runs only the forward-mode pass (value, 2N
             # tangents, Laplacian) and skips the spin-flipped configurations

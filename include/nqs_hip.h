@@ -48,7 +48,7 @@ enum { NQS_FP64 = 0, NQS_MIXED = 1 };
 enum { NQS_GEMM_BF16X3 = 0, NQS_GEMM_FP32 = 1, NQS_GEMM_BF16 = 2 };
 
 /* ops for nqs_workspace_bytes */
-enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3, NQS_OP_SPIN = 4 };
+enum { NQS_OP_LOGPSI = 0, NQS_OP_ELOC = 1, NQS_OP_MCMC = 2, NQS_OP_GRAD = 3, NQS_OP_SPIN = 4, NQS_OP_VJP = 5 };
 
 /* Architecture + cell of ContinuumSpinorPsiformer (src/nqs/psiformer.py:100-116). */
 typedef struct nqs_model_desc {
@@ -162,6 +162,18 @@ int nqs_eloc(const nqs_model_desc* m, const nqs_hamiltonian_desc* h, const void*
 int nqs_psiformer_grad(const nqs_model_desc* m, const void* packed_dev, const double* R_dev,
                        const int32_t* S_dev, int64_t n_walkers, void* O_dev, int64_t ldo,
                        double* logpsi_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+
+/* ---- vector-Jacobian product of log psi with respect to theta, summed over the walkers
+ * (jax.vjp of the batched log psi; the backward of the differentiable log psi that stands in for
+ * jax.grad in src/nqs/minsr.py:56-65 and src/nqs/train.py:48-58):
+ *   g_dev (P) = sum_w [cot_w.re d Re log psi_w / d theta + cot_w.im d Im log psi_w / d theta]
+ * cot_dev (W) complex interleaved.  The reverse sweep of nqs_psiformer_grad with the cotangent
+ * folded in at the determinant head; weight, bias and LayerNorm gradients are summed over all
+ * walkers and tokens in slices, the slices added in a fixed order (no O rows, no atomics).
+ * logpsi_dev (W) complex or NULL.  Workspace: nqs_workspace_bytes(m, NQS_OP_VJP, W). */
+int nqs_psiformer_vjp(const nqs_model_desc* m, const void* packed_dev, const double* R_dev,
+                      const int32_t* S_dev, int64_t n_walkers, const double* cot_dev, double* g_dev,
+                      double* logpsi_dev, void* ws_dev, int64_t ws_bytes, void* stream);
 
 /* ---- Ewald Coulomb energy per walker: energy_dev (W) float64 */
 int nqs_ewald(const nqs_ewald_desc* e, const double* R_dev, int64_t n_walkers, int32_t n_elec,
