@@ -283,6 +283,21 @@ def time_path(model, theta, R, S, coul, args, gram, steps, warmup, graph=False, 
     return vs, ms, stages, np.array(gram_ms).mean(0)
 
 
+def _tile_choices():
+    """The autotuner's choices so far: dense-layer GEMM tile per (N, K, log2 M) class, waves per
+    workgroup of the attention jet and determinant heads per shape class."""
+    from nqs import _lib
+    names = {"1": "attention", "2": "det16", "3": "det32", "4": "gemm_tile"}
+    out = []
+    for line in _lib.autotune_report().splitlines():
+        if not line.startswith("kernel "):
+            continue
+        k = line.split()[1]
+        shape = line[line.index("(") + 1:line.index(")")]
+        out.append(f"{names.get(k, k)} ({shape}) -> {line.rsplit('-> ', 1)[-1]}")
+    return out
+
+
 def _stage_dict(stages, mins):
     return {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
             "minsr": float(stages[3]), "update_repack": float(stages[4]),
@@ -636,7 +651,8 @@ def main():
                         "pmc_source": pmc["source"] if pmc else "not measured for this source revision"},
            "step_roofline": step_roofline(args.precision, W, P, ldo, ms, args.sweeps),
            "stage_ms": _stage_dict(stages, mins),
-           "E_mean": float(stats[0]), "E_err": float(stats[1]), "failed_steps": vs.failed_steps}
+           "E_mean": float(stats[0]), "E_err": float(stats[1]), "failed_steps": vs.failed_steps,
+           "autotune": _tile_choices()}
     if args.precision == "mixed" and not args.no_accuracy:
         out["accuracy"] = accuracy(vs, sysd, coul, args.gemm)
     if args.precision == "mixed" and args.fp64_steps > 0:

@@ -10,6 +10,7 @@
 // and differ in the accumulator row map: f64 row = lane/16 + 4 r, f32 row = 4 (lane/16) + r.
 #include "nqs_common.h"
 #include "kernels.h"
+#include "tune.h"
 
 
 namespace nqs {
@@ -237,64 +238,84 @@ __device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& 
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
 }
 
-constexpr int BF_BM = 128, BF_BK = 32, BF_LD = BF_BK + 8;
-// staging geometry: 8-float chunks per tile row, rows per pass of the 256 threads, passes
-constexpr int BF_CPR = BF_BK / 8, BF_RPP = 256 / BF_CPR, BF_NQ = BF_BM / BF_RPP;
+constexpr int BF_BK = 32, BF_LD = BF_BK + 8;
+// staging geometry: 8-float chunks per tile row, rows per pass of the 256 threads
+constexpr int BF_CPR = BF_BK / 8, BF_RPP = 256 / BF_CPR;
 
-template <int SPLIT>
+// Tile shapes: 4 waves in WM x WN, each wave BI x BJ blocks of 32 x 32, so the workgroup tile is
+// (32 WM BI) x (32 WN BJ).  Every shape accumulates each output element over K in the same order
+// (16-wide k chunks, lo*hi, hi*lo, hi*hi per chunk) with the same epilogue, so all shapes give
+// bit-identical results and the autotuner may pick any of them per shape class.
+template <int WM, int WN, int BI, int BJ>
+struct BfTile {
+  static constexpr int BM = 32 * WM * BI, BN = 32 * WN * BJ;
+  static constexpr int NQA = BM / BF_RPP, NQB = BN / BF_RPP;
+  static_assert(WM * WN == 4 && NQA >= 1 && NQB >= 1, "four waves, tiles of at least 64 rows");
+};
+
+template <int SPLIT, int BM, int BN>
 constexpr size_t bf_lds_bytes() {
-  return sizeof(__bf16) * (size_t)2 * (SPLIT == 1 ? 1 : 2) * BF_BM * BF_LD;
+  return sizeof(__bf16) * (size_t)(SPLIT == 1 ? 1 : 2) * (BM + BN) * BF_LD;
 }
 
-// Staging: a thread moves rows tid / 4 + 64 q (q = 0..1), k-chunk tid % 4 (8 floats, two 16-byte
-// loads) and writes its 8 hi and 8 lo bf16 as 16-byte LDS stores; 4 lanes fill one 64-byte row,
-// stored at an 80-byte row pitch.  Three workgroups share a CU (the epilogue keeps one 32-row
-// block of addends live at a time, so the kernel fits 168 registers without spills).
-// One LDS buffer per operand and plane; the registers carry stage kb + 1 across the loop back
-// edge (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
-template <int SPLIT>
-__global__ __launch_bounds__(256, 3) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
+// Staging: a thread moves rows tid / 4 + 64 q, k-chunk tid % 4 (8 floats, two 16-byte loads) and
+// writes its 8 hi and 8 lo bf16 as 16-byte LDS stores; 4 lanes fill one 64-byte row, stored at an
+// 80-byte row pitch.  The epilogue keeps one 32-row block of addends live at a time.  One LDS
+// buffer per operand and plane; the registers carry stage kb + 1 across the loop back edge
+// (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
+template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
+  typedef BfTile<WM, WN, BI, BJ> TL;
+  constexpr int BM = TL::BM, BN = TL::BN, NQA = TL::NQA, NQB = TL::NQB;
   constexpr int NP = SPLIT == 1 ? 1 : 2;
-  constexpr int PLANE = BF_BM * BF_LD;  // A planes (hi, lo), then B planes
+  constexpr int APL = BM * BF_LD, BPL = BN * BF_LD;  // A planes (hi, lo), then B planes
   extern __shared__ __bf16 bfs[];
+  __bf16* As = bfs;
+  __bf16* Bs = bfs + NP * APL;
   const int64_t per_xcd = (ntiles + 7) / 8;
   const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (t >= ntiles) return;
   const int64_t bm = t / nbn;
   const int bn = (int)(t % nbn);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int64_t m0 = bm * BF_BM, n0 = (int64_t)bn * BF_BM;
-  f16v acc[2][2];
+  const int wm = wave / WN, wn = wave % WN;
+  const int64_t m0 = bm * BM, n0 = (int64_t)bn * BN;
+  f16v acc[BI][BJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
   const int srow = tid / BF_CPR, sk = (tid % BF_CPR) * 8;
-  const float* pa[BF_NQ];
-  const float* pb[BF_NQ];
+  const float* pa[NQA];
+  const float* pb[NQB];
 #pragma unroll
-  for (int q = 0; q < BF_NQ; ++q) {
-    const int64_t gm = m0 + srow + BF_RPP * q, gn = n0 + srow + BF_RPP * q;
+  for (int q = 0; q < NQA; ++q) {
+    const int64_t gm = m0 + srow + BF_RPP * q;
     pa[q] = g.A + (gm < g.M ? gm : g.M - 1) * g.lda;
+  }
+#pragma unroll
+  for (int q = 0; q < NQB; ++q) {
+    const int64_t gn = n0 + srow + BF_RPP * q;
     pb[q] = g.B + (gn < g.N ? gn : g.N - 1) * g.ldb;
   }
-  f4v ra[BF_NQ][2], rb[BF_NQ][2];
+  f4v ra[NQA][2], rb[NQB][2];
   const f4v zero = (f4v)0.f;
   const int64_t nk = cdiv_dev(g.K, BF_BK);
   // K is a multiple of 4: each 4-wide half of a chunk is either wholly inside K or wholly out
-#define NQS_BF_LOAD(kb_)                                                        \
-  {                                                                             \
-    const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                             \
-    const int64_t c0 = g0 < g.K ? g0 : g.K - 4, c1 = g0 + 4 < g.K ? g0 + 4 : g.K - 4; \
-    _Pragma("unroll") for (int q = 0; q < BF_NQ; ++q) {                             \
-      ra[q][0] = *reinterpret_cast<const f4v*>(pa[q] + c0);                     \
-      ra[q][1] = *reinterpret_cast<const f4v*>(pa[q] + c1);                     \
-      rb[q][0] = *reinterpret_cast<const f4v*>(pb[q] + c0);                     \
-      rb[q][1] = *reinterpret_cast<const f4v*>(pb[q] + c1);                     \
-    }                                                                           \
+#define NQS_BF_LOAD(kb_)                                                                \
+  {                                                                                     \
+    const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                                     \
+    const int64_t c0 = g0 < g.K ? g0 : g.K - 4, c1 = g0 + 4 < g.K ? g0 + 4 : g.K - 4;    \
+    _Pragma("unroll") for (int q = 0; q < NQA; ++q) {                                   \
+      ra[q][0] = *reinterpret_cast<const f4v*>(pa[q] + c0);                             \
+      ra[q][1] = *reinterpret_cast<const f4v*>(pa[q] + c1);                             \
+    }                                                                                   \
+    _Pragma("unroll") for (int q = 0; q < NQB; ++q) {                                   \
+      rb[q][0] = *reinterpret_cast<const f4v*>(pb[q] + c0);                             \
+      rb[q][1] = *reinterpret_cast<const f4v*>(pb[q] + c1);                             \
+    }                                                                                   \
   }
 #define NQS_BF_SPLIT8(x0, x1, hi, lo)                                             \
   {                                                                               \
@@ -310,20 +331,23 @@ __global__ __launch_bounds__(256, 3) void k_gemm_nt_bf(GemmArgs<float> g, int nb
   {                                                                                           \
     const int64_t g0 = (int64_t)(kb_) * BF_BK + sk;                                           \
     const bool ok0 = g0 < g.K, ok1 = g0 + 4 < g.K;                                            \
-    _Pragma("unroll") for (int q = 0; q < BF_NQ; ++q) {                                           \
-      const int row = srow + BF_RPP * q;                                                          \
-      const bool am = m0 + row < g.M, bnn = n0 + row < g.N;                                   \
+    _Pragma("unroll") for (int q = 0; q < NQA; ++q) {                                         \
+      const int row = srow + BF_RPP * q;                                                      \
+      const bool am = m0 + row < g.M;                                                         \
       const f4v a0 = (ok0 && am) ? ra[q][0] : zero, a1 = (ok1 && am) ? ra[q][1] : zero;       \
-      const f4v b0 = (ok0 && bnn) ? rb[q][0] : zero, b1 = (ok1 && bnn) ? rb[q][1] : zero;     \
-      u4v ah, al, bh, bl;                                                                     \
+      u4v ah, al;                                                                             \
       NQS_BF_SPLIT8(a0, a1, ah, al)                                                           \
+      *reinterpret_cast<u4v*>(As + row * BF_LD + sk) = ah;                                    \
+      if (NP == 2) *reinterpret_cast<u4v*>(As + APL + row * BF_LD + sk) = al;                 \
+    }                                                                                         \
+    _Pragma("unroll") for (int q = 0; q < NQB; ++q) {                                         \
+      const int row = srow + BF_RPP * q;                                                      \
+      const bool bnn = n0 + row < g.N;                                                        \
+      const f4v b0 = (ok0 && bnn) ? rb[q][0] : zero, b1 = (ok1 && bnn) ? rb[q][1] : zero;     \
+      u4v bh, bl;                                                                             \
       NQS_BF_SPLIT8(b0, b1, bh, bl)                                                           \
-      *reinterpret_cast<u4v*>(bfs + row * BF_LD + sk) = ah;                                   \
-      *reinterpret_cast<u4v*>(bfs + NP * PLANE + row * BF_LD + sk) = bh;                      \
-      if (NP == 2) {                                                                          \
-        *reinterpret_cast<u4v*>(bfs + PLANE + row * BF_LD + sk) = al;                         \
-        *reinterpret_cast<u4v*>(bfs + 3 * PLANE + row * BF_LD + sk) = bl;                     \
-      }                                                                                       \
+      *reinterpret_cast<u4v*>(Bs + row * BF_LD + sk) = bh;                                    \
+      if (NP == 2) *reinterpret_cast<u4v*>(Bs + BPL + row * BF_LD + sk) = bl;                 \
     }                                                                                         \
   }
   NQS_BF_LOAD(0)
@@ -333,23 +357,24 @@ __global__ __launch_bounds__(256, 3) void k_gemm_nt_bf(GemmArgs<float> g, int nb
   for (int64_t kb = 0; kb < nk; ++kb) {
 #pragma unroll
     for (int kk = 0; kk < BF_BK; kk += 16) {
-      bf8v ah[2], al[2], bh[2], bl[2];
+      bf8v ah[BI], al[BI], bh[BJ], bl[BJ];
       const int ko = kk + 8 * (lane >> 5);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int ra_ = (wm * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
-        const int rb_ = (wn * 64 + i * 32 + (lane & 31)) * BF_LD + ko;
-        ah[i] = *reinterpret_cast<const bf8v*>(bfs + ra_);
-        bh[i] = *reinterpret_cast<const bf8v*>(bfs + NP * PLANE + rb_);
-        if (NP == 2) {
-          al[i] = *reinterpret_cast<const bf8v*>(bfs + PLANE + ra_);
-          bl[i] = *reinterpret_cast<const bf8v*>(bfs + 3 * PLANE + rb_);
-        }
+      for (int i = 0; i < BI; ++i) {
+        const int ra_ = (wm * BI * 32 + i * 32 + (lane & 31)) * BF_LD + ko;
+        ah[i] = *reinterpret_cast<const bf8v*>(As + ra_);
+        if (NP == 2) al[i] = *reinterpret_cast<const bf8v*>(As + APL + ra_);
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < BJ; ++j) {
+        const int rb_ = (wn * BJ * 32 + j * 32 + (lane & 31)) * BF_LD + ko;
+        bh[j] = *reinterpret_cast<const bf8v*>(Bs + rb_);
+        if (NP == 2) bl[j] = *reinterpret_cast<const bf8v*>(Bs + BPL + rb_);
+      }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) {
           if (NP == 2) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
@@ -367,71 +392,103 @@ __global__ __launch_bounds__(256, 3) void k_gemm_nt_bf(GemmArgs<float> g, int nb
 #undef NQS_BF_LOAD
 #undef NQS_BF_STORE
 #undef NQS_BF_SPLIT8
-  // two-phase epilogue as in k_gemm_nt_v2, one 32-row block i at a time (32 operands live)
+  // two-phase epilogue as in k_gemm_nt_v2, one 32-row block i at a time (16 BJ operands live)
   const bool has_beta = g.beta != 0.f, has_bias = g.bias != nullptr;
-  float bv[2];
+  float bv[BJ];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+  for (int j = 0; j < BJ; ++j) {
+    const int64_t col = n0 + wn * BJ * 32 + j * 32 + (lane & 31);
     bv[j] = has_bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    float addv[2][16];
+  for (int i = 0; i < BI; ++i) {
+    float addv[BJ][16];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) addv[j][r] = 0.f;
     if (has_beta) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < BJ; ++j)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+          const int64_t row = m0 + wm * BI * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const int64_t col = n0 + wn * BJ * 32 + j * 32 + (lane & 31);
           const int64_t rr = row < g.M ? row : g.M - 1, cc = col < g.N ? col : g.N - 1;
           addv[j][r] = g.beta * g.C[rr * g.ldc + cc];
         }
     }
     if (has_bias) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < BJ; ++j)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const uint32_t row = (uint32_t)(m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3));
+          const uint32_t row = (uint32_t)(m0 + wm * BI * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3));
           addv[j][r] += (row % (uint32_t)g.bias_period < (uint32_t)g.bias_len) ? bv[j] : 0.f;
         }
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int64_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+        const int64_t row = m0 + wm * BI * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t col = n0 + wn * BJ * 32 + j * 32 + (lane & 31);
         const float v = g.alpha * acc[i][j][r] + addv[j][r];
         if (row < g.M && col < g.N) g.C[row * g.ldc + col] = v;
       }
   }
 }
 
-template <int SPLIT>
-static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
-  constexpr size_t lds = bf_lds_bytes<SPLIT>();
+template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB>
+static int launch_gemm_bf_tile(const GemmArgs<float>& g, hipStream_t st) {
+  typedef BfTile<WM, WN, BI, BJ> TL;
+  constexpr size_t lds = bf_lds_bytes<SPLIT, TL::BM, TL::BN>();
+  auto kern = k_gemm_nt_bf<SPLIT, WM, WN, BI, BJ, MINB>;
   if (lds > 64 * 1024) {
     static bool attr = false;
     if (!attr) {
-      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_gemm_nt_bf<SPLIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
+      NQS_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       attr = true;
     }
   }
-  const int nbn = (int)cdiv(g.N, BF_BM);
-  const int64_t ntiles = cdiv(g.M, BF_BM) * nbn;
+  const int nbn = (int)cdiv(g.N, TL::BN);
+  const int64_t ntiles = cdiv(g.M, TL::BM) * nbn;
   NQS_REQUIRE(ntiles < (int64_t)1 << 31, NQS_ERR_UNSUPPORTED, "GEMM too large for one launch");
-  hipLaunchKernelGGL((k_gemm_nt_bf<SPLIT>), dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(256), lds, st, g, nbn,
-                     ntiles);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((ntiles + 7) / 8))), dim3(256), lds, st, g, nbn, ntiles);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
+}
+
+// Tile shapes the autotuner chooses among for the bf16 dense-layer GEMM (first = default).
+static const char* const kBfTileNames[5] = {"128x128", "64x128", "128x64", "64x128w", "256x64"};
+template <int SPLIT>
+static int launch_gemm_bf_choice(const GemmArgs<float>& g, int c, hipStream_t st) {
+  switch (c) {
+    case 1: return launch_gemm_bf_tile<SPLIT, 1, 4, 2, 1, 4>(g, st);   // waves side by side, 64 x 32 each
+    case 2: return launch_gemm_bf_tile<SPLIT, 4, 1, 1, 2, 4>(g, st);   // waves stacked, 32 x 64 each
+    case 3: return launch_gemm_bf_tile<SPLIT, 2, 2, 1, 2, 4>(g, st);   // 2 x 2 waves of 32 x 64
+    case 4: return launch_gemm_bf_tile<SPLIT, 4, 1, 2, 2, 2>(g, st);   // waves stacked, 64 x 64 each
+    default: return launch_gemm_bf_tile<SPLIT, 2, 2, 2, 2, 3>(g, st);  // 2 x 2 waves of 64 x 64
+  }
+}
+
+// The dense-layer GEMM with its tile shape chosen per shape class (split, N, K, log2 M) by the
+// autotuner.  A call with beta != 0 accumulates into C, so its trials write to g.tune_scratch
+// with beta = 0; without a scratch buffer it uses the shape already chosen for its class.
+template <int SPLIT>
+static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
+  const TuneKey key{TUNE_GEMM_BF, SPLIT * 100000 + g.N, g.K, tune_bucket(g.M)};
+  const bool acc = g.beta != 0.f;
+  return tune_run_trials(key, 5, 0, kBfTileNames, [&](int c, bool trial) -> int {
+    if (trial && acc) {
+      GemmArgs<float> t = g;
+      t.C = g.tune_scratch;
+      t.ldc = g.ldc;
+      t.beta = 0.f;
+      return launch_gemm_bf_choice<SPLIT>(t, c, st);
+    }
+    return launch_gemm_bf_choice<SPLIT>(g, c, st);
+  }, !acc || g.tune_scratch != nullptr, st);
 }
 
 template <typename T>

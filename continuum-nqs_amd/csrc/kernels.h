@@ -39,6 +39,10 @@ struct GemmArgs {
   int tri;     // 0 full, 1 lower-triangular tiles only (col <= row)
   int mirror;  // also store C[col][row]
   int split;   // float only: 0 float32 MFMA, 1 bf16 MFMA, 3 bf16x3 (hi/lo split) MFMA
+  // Optional M x ldc buffer the tile-shape autotuner may overwrite: with beta != 0 the trial
+  // launches write A B there (beta = 0) instead of accumulating into C.  NULL: a beta != 0 call
+  // uses the tile shape already chosen for its shape class (or the default) without trials.
+  T* tune_scratch = nullptr;
 };
 
 template <typename T>

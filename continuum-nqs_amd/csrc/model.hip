@@ -382,12 +382,16 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
     NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + P.qkvT, b.big, rows, 3 * d, d, d, d, 3 * d, pk + P.qkv_b, per, blen, (T)0),
                        1, st));
     NQS_TRY(launch_attention_jet<T>(jg, d, md.nh, b.big, b.att, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.att, pk + P.oT, b.x, rows, d, d, d, d, d, pk + P.o_b, per, blen, (T)1), 1, st));
+    // residual GEMMs accumulate into x: the tile-shape tuner's trials write into h (free here)
+    GemmArgs<T> go = gargs<T>(md, b.att, pk + P.oT, b.x, rows, d, d, d, d, d, pk + P.o_b, per, blen, (T)1);
+    go.tune_scratch = b.h;
+    NQS_TRY(gemm_nt<T>(go, 1, st));
     NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + P.ln1_s, pk + P.ln1_b, b.h, st));
     NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + P.w0T, b.big, rows, ffd, d, d, d, ffd, pk + P.b0, per, blen, (T)0), 1, st));
     NQS_TRY(launch_gelu_jet<T>(jg, ffd, b.big, b.att, st));
-    NQS_TRY(gemm_nt<T>(gargs<T>(md, b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1, per, blen, (T)1), 1,
-                       st));
+    GemmArgs<T> g1 = gargs<T>(md, b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1, per, blen, (T)1);
+    g1.tune_scratch = b.h;
+    NQS_TRY(gemm_nt<T>(g1, 1, st));
   }
   NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + md.lnf_s, pk + md.lnf_b, b.h, st));
   NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + md.bfT, b.big, rows, 2 * KN, d, d, d, 2 * KN, pk + md.bf_b, per, blen, (T)0),

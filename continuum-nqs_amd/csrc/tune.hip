@@ -24,6 +24,12 @@ int64_t tune_bucket(int64_t n) {
 
 int tune_run(const TuneKey& key, int n, int def, const char* const* names, const std::function<int(int)>& launch,
              hipStream_t st) {
+  return tune_run_trials(key, n, def, names, [&](int c, bool trial) { return launch(c); }, true, st);
+}
+
+int tune_run_trials(const TuneKey& key, int n, int def, const char* const* names,
+                    const std::function<int(int, bool)>& launch_t, bool trials_ok, hipStream_t st) {
+  auto launch = [&](int c) { return launch_t(c, true); };
   int choice = -1;
   bool enabled;
   {
@@ -32,10 +38,10 @@ int tune_run(const TuneKey& key, int n, int def, const char* const* names, const
     auto it = g_table.find(key);
     if (it != g_table.end()) choice = it->second.choice;
   }
-  if (choice >= 0) return launch(choice);
+  if (choice >= 0) return launch_t(choice, false);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   NQS_CHECK_HIP(hipStreamIsCapturing(st, &cs));
-  if (!enabled || cs != hipStreamCaptureStatusNone || n <= 1) return launch(def);
+  if (!enabled || !trials_ok || cs != hipStreamCaptureStatusNone || n <= 1) return launch_t(def, false);
   hipEvent_t e[2];
   NQS_CHECK_HIP(hipEventCreate(&e[0]));
   NQS_CHECK_HIP(hipEventCreate(&e[1]));
@@ -63,10 +69,8 @@ int tune_run(const TuneKey& key, int n, int def, const char* const* names, const
     std::lock_guard<std::mutex> lk(g_mu);
     g_table[key] = Entry{best_c, rep.str()};
   }
-  // the last candidate's output is already in place unless it failed; make sure the kept
-  // choice has written the outputs
-  if (best_c != n - 1) return launch(best_c);
-  return NQS_OK;
+  // the kept choice writes the outputs (for pure kernels the trials already did; once per class)
+  return launch_t(best_c, false);
 }
 
 }  // namespace nqs
