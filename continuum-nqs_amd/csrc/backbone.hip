@@ -207,18 +207,6 @@ int launch_layernorm_jet(const JetGeom& j, int d, const T* x, const T* scale, co
 }
 
 // ------------------------------------------------------------------ tanh-GELU jet
-template <typename T>
-__device__ inline void gelu_derivs(T x, T& g, T& g1, T& g2) {
-  const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
-  const T u = k0 * (x + k1 * x * x * x);
-  const T th = tanh(u);
-  const T sech2 = (T)1 - th * th;
-  const T du = k0 * ((T)1 + (T)3 * k1 * x * x);
-  const T ddu = (T)6 * k0 * k1 * x;
-  g = (T)0.5 * x * ((T)1 + th);
-  g1 = (T)0.5 * ((T)1 + th) + (T)0.5 * x * sech2 * du;
-  g2 = sech2 * du + (T)0.5 * x * (sech2 * ddu - (T)2 * sech2 * th * du * du);
-}
 
 template <typename T>
 __global__ void k_gelu_jet(JetGeom j, int width, const T* __restrict__ x, T* __restrict__ y) {

@@ -263,7 +263,10 @@ constexpr size_t bf_lds_bytes() {
 // 80-byte row pitch.  The epilogue keeps one 32-row block of addends live at a time.  One LDS
 // buffer per operand and plane; the registers carry stage kb + 1 across the loop back edge
 // (loaded at the end of iteration kb - 1), so global latency overlaps a stage of MFMA work.
-template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB>
+// PRO = 1: the first-order GELU jet prologue on A (GemmArgs::gelu_cn): each staged A row also loads
+// its token's value row (the same row for the value channel) and the pair is turned into the GELU
+// jet before the bf16 split.
+template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB, int PRO = 0>
 __global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int nbn, int64_t ntiles) {
   typedef BfTile<WM, WN, BI, BJ> TL;
   constexpr int BM = TL::BM, BN = TL::BN, NQA = TL::NQA, NQB = TL::NQB;
@@ -301,6 +304,19 @@ __global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int
     pb[q] = g.B + (gn < g.N ? gn : g.N - 1) * g.ldb;
   }
   f4v ra[NQA][2], rb[NQB][2];
+  // PRO: the value row of each staged A row, and whether the row is a value row itself
+  const float* pv[PRO ? NQA : 1];
+  bool isval[PRO ? NQA : 1];
+  f4v rv[PRO ? NQA : 1][2];
+  if constexpr (PRO) {
+#pragma unroll
+    for (int q = 0; q < NQA; ++q) {
+      const int64_t gm0 = m0 + srow + BF_RPP * q, gm = gm0 < g.M ? gm0 : g.M - 1;
+      const int64_t c = (gm % g.gelu_cn) / g.gelu_n;
+      isval[q] = c == 0;
+      pv[q] = g.A + (gm - c * g.gelu_n) * g.lda;
+    }
+  }
   const f4v zero = (f4v)0.f;
   const int64_t nk = cdiv_dev(g.K, BF_BK);
   // K is a multiple of 4: each 4-wide half of a chunk is either wholly inside K or wholly out
@@ -311,6 +327,10 @@ __global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int
     _Pragma("unroll") for (int q = 0; q < NQA; ++q) {                                   \
       ra[q][0] = *reinterpret_cast<const f4v*>(pa[q] + c0);                             \
       ra[q][1] = *reinterpret_cast<const f4v*>(pa[q] + c1);                             \
+      if constexpr (PRO) {                                                              \
+        rv[q][0] = *reinterpret_cast<const f4v*>(pv[q] + c0);                           \
+        rv[q][1] = *reinterpret_cast<const f4v*>(pv[q] + c1);                           \
+      }                                                                                 \
     }                                                                                   \
     _Pragma("unroll") for (int q = 0; q < NQB; ++q) {                                   \
       rb[q][0] = *reinterpret_cast<const f4v*>(pb[q] + c0);                             \
@@ -334,7 +354,17 @@ __global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int
     _Pragma("unroll") for (int q = 0; q < NQA; ++q) {                                         \
       const int row = srow + BF_RPP * q;                                                      \
       const bool am = m0 + row < g.M;                                                         \
-      const f4v a0 = (ok0 && am) ? ra[q][0] : zero, a1 = (ok1 && am) ? ra[q][1] : zero;       \
+      f4v a0 = (ok0 && am) ? ra[q][0] : zero, a1 = (ok1 && am) ? ra[q][1] : zero;             \
+      if constexpr (PRO) {                                                                    \
+        _Pragma("unroll") for (int e_ = 0; e_ < 8; ++e_) {                                    \
+          const float u_ = e_ < 4 ? a0[e_] : a1[e_ - 4];                                      \
+          const float v_ = e_ < 4 ? rv[q][0][e_] : rv[q][1][e_ - 4];                          \
+          float g0_, g1_, g2_;                                                                \
+          gelu_derivs<float>(isval[q] ? u_ : v_, g0_, g1_, g2_);                              \
+          const float y_ = isval[q] ? g0_ : g1_ * u_;                                         \
+          if (e_ < 4) a0[e_] = y_; else a1[e_ - 4] = y_;                                      \
+        }                                                                                     \
+      }                                                                                       \
       u4v ah, al;                                                                             \
       NQS_BF_SPLIT8(a0, a1, ah, al)                                                           \
       *reinterpret_cast<u4v*>(As + row * BF_LD + sk) = ah;                                    \
@@ -439,11 +469,11 @@ __global__ __launch_bounds__(256, MINB) void k_gemm_nt_bf(GemmArgs<float> g, int
   }
 }
 
-template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB>
+template <int SPLIT, int WM, int WN, int BI, int BJ, int MINB, int PRO>
 static int launch_gemm_bf_tile(const GemmArgs<float>& g, hipStream_t st) {
   typedef BfTile<WM, WN, BI, BJ> TL;
   constexpr size_t lds = bf_lds_bytes<SPLIT, TL::BM, TL::BN>();
-  auto kern = k_gemm_nt_bf<SPLIT, WM, WN, BI, BJ, MINB>;
+  auto kern = k_gemm_nt_bf<SPLIT, WM, WN, BI, BJ, MINB, PRO>;
   if (lds > 64 * 1024) {
     static bool attr = false;
     if (!attr) {
@@ -461,14 +491,15 @@ static int launch_gemm_bf_tile(const GemmArgs<float>& g, hipStream_t st) {
 
 // Tile shapes the autotuner chooses among for the bf16 dense-layer GEMM (first = default).
 static const char* const kBfTileNames[5] = {"128x128", "64x128", "128x64", "64x128w", "256x64"};
-template <int SPLIT>
+template <int SPLIT, int PRO>
 static int launch_gemm_bf_choice(const GemmArgs<float>& g, int c, hipStream_t st) {
+  // with the GELU prologue the staging holds twice the A registers: one workgroup fewer per CU
   switch (c) {
-    case 1: return launch_gemm_bf_tile<SPLIT, 1, 4, 2, 1, 4>(g, st);   // waves side by side, 64 x 32 each
-    case 2: return launch_gemm_bf_tile<SPLIT, 4, 1, 1, 2, 4>(g, st);   // waves stacked, 32 x 64 each
-    case 3: return launch_gemm_bf_tile<SPLIT, 2, 2, 1, 2, 4>(g, st);   // 2 x 2 waves of 32 x 64
-    case 4: return launch_gemm_bf_tile<SPLIT, 4, 1, 2, 2, 2>(g, st);   // waves stacked, 64 x 64 each
-    default: return launch_gemm_bf_tile<SPLIT, 2, 2, 2, 2, 3>(g, st);  // 2 x 2 waves of 64 x 64
+    case 1: return launch_gemm_bf_tile<SPLIT, 1, 4, 2, 1, PRO ? 3 : 4, PRO>(g, st);   // waves side by side, 64 x 32 each
+    case 2: return launch_gemm_bf_tile<SPLIT, 4, 1, 1, 2, PRO ? 3 : 4, PRO>(g, st);   // waves stacked, 32 x 64 each
+    case 3: return launch_gemm_bf_tile<SPLIT, 2, 2, 1, 2, PRO ? 3 : 4, PRO>(g, st);   // 2 x 2 waves of 32 x 64
+    case 4: return launch_gemm_bf_tile<SPLIT, 4, 1, 2, 2, 2, PRO>(g, st);             // waves stacked, 64 x 64 each
+    default: return launch_gemm_bf_tile<SPLIT, 2, 2, 2, 2, PRO ? 2 : 3, PRO>(g, st);  // 2 x 2 waves of 64 x 64
   }
 }
 
@@ -477,17 +508,21 @@ static int launch_gemm_bf_choice(const GemmArgs<float>& g, int c, hipStream_t st
 // with beta = 0; without a scratch buffer it uses the shape already chosen for its class.
 template <int SPLIT>
 static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
-  const TuneKey key{TUNE_GEMM_BF, SPLIT * 100000 + g.N, g.K, tune_bucket(g.M)};
+  const bool pro = g.gelu_cn > 0;
+  const TuneKey key{TUNE_GEMM_BF, (pro ? 1000000 : 0) + SPLIT * 100000 + g.N, g.K, tune_bucket(g.M)};
   const bool acc = g.beta != 0.f;
+  auto run = [&](const GemmArgs<float>& a, int c) {
+    return pro ? launch_gemm_bf_choice<SPLIT, 1>(a, c, st) : launch_gemm_bf_choice<SPLIT, 0>(a, c, st);
+  };
   return tune_run_trials(key, 5, 0, kBfTileNames, [&](int c, bool trial) -> int {
     if (trial && acc) {
       GemmArgs<float> t = g;
       t.C = g.tune_scratch;
       t.ldc = g.ldc;
       t.beta = 0.f;
-      return launch_gemm_bf_choice<SPLIT>(t, c, st);
+      return run(t, c);
     }
-    return launch_gemm_bf_choice<SPLIT>(g, c, st);
+    return run(g, c);
   }, !acc || g.tune_scratch != nullptr, st);
 }
 
@@ -503,6 +538,9 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
     NQS_REQUIRE(g.split == 0 || g.split == 1 || g.split == 3, NQS_ERR_ARG, "bf16 GEMM split must be 0, 1 or 3");
     if (g.split != 0 && vec_ok && batch == 1 && g.tri == 0 && !g.mirror)
       return g.split == 3 ? launch_gemm_bf<3>(g, st) : launch_gemm_bf<1>(g, st);
+    NQS_REQUIRE(g.gelu_cn == 0, NQS_ERR_UNSUPPORTED, "GELU prologue needs the bf16 GEMM path");
+  } else {
+    NQS_REQUIRE(g.gelu_cn == 0, NQS_ERR_UNSUPPORTED, "GELU prologue needs the bf16 GEMM path");
   }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);

@@ -25,6 +25,20 @@ struct Mma16<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// tanh-GELU (jax.nn.gelu, approximate=True) and its first two derivatives
+template <typename T>
+__device__ inline void gelu_derivs(T x, T& g, T& g1, T& g2) {
+  const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
+  const T u = k0 * (x + k1 * x * x * x);
+  const T th = tanh(u);
+  const T sech2 = (T)1 - th * th;
+  const T du = k0 * ((T)1 + (T)3 * k1 * x * x);
+  const T ddu = (T)6 * k0 * k1 * x;
+  g = (T)0.5 * x * ((T)1 + th);
+  g1 = (T)0.5 * ((T)1 + th) + (T)0.5 * x * sech2 * du;
+  g2 = sech2 * du + (T)0.5 * x * (sech2 * ddu - (T)2 * sech2 * th * du * du);
+}
+
 template <typename T>
 struct GemmArgs {
   const T* A;  // M x K, row stride lda
@@ -43,6 +57,12 @@ struct GemmArgs {
   // launches write A B there (beta = 0) instead of accumulating into C.  NULL: a beta != 0 call
   // uses the tile shape already chosen for its shape class (or the default) without trials.
   T* tune_scratch = nullptr;
+  // Optional first-order GELU jet applied to A as it is staged (bf16 float kernels only), for jets
+  // without a Laplacian channel laid out [cfg][channel][token][feature] with gelu_cn = channels x
+  // tokens rows per configuration and gelu_n tokens: A holds the pre-activation u; the value rows
+  // (channel 0) become gelu(u), a tangent row du becomes gelu'(u of its token's value row) du.
+  // 0 = no prologue.
+  int gelu_cn = 0, gelu_n = 0;
 };
 
 template <typename T>

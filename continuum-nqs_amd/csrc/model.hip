@@ -388,9 +388,17 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
     NQS_TRY(gemm_nt<T>(go, 1, st));
     NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + P.ln1_s, pk + P.ln1_b, b.h, st));
     NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + P.w0T, b.big, rows, ffd, d, d, d, ffd, pk + P.b0, per, blen, (T)0), 1, st));
-    NQS_TRY(launch_gelu_jet<T>(jg, ffd, b.big, b.att, st));
-    GemmArgs<T> g1 = gargs<T>(md, b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1, per, blen, (T)1);
+    // without a Laplacian channel the GELU jet is elementwise given the token's value row, so the
+    // bf16 Dense_1 applies it while staging its A operand (no pass over the hidden activations)
+    const bool fuse_gelu = sizeof(T) == 4 && md.split != 0 && !jg.has_lap;
+    if (!fuse_gelu) NQS_TRY(launch_gelu_jet<T>(jg, ffd, b.big, b.att, st));
+    GemmArgs<T> g1 = gargs<T>(md, fuse_gelu ? b.big : b.att, pk + P.w1T, b.x, rows, d, ffd, ffd, ffd, d, pk + P.b1,
+                              per, blen, (T)1);
     g1.tune_scratch = b.h;
+    if (fuse_gelu) {
+      g1.gelu_cn = jg.C * md.N;
+      g1.gelu_n = md.N;
+    }
     NQS_TRY(gemm_nt<T>(g1, 1, st));
   }
   NQS_TRY(launch_layernorm_jet<T>(jg, d, b.x, pk + md.lnf_s, pk + md.lnf_b, b.h, st));
