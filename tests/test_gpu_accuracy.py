@@ -108,3 +108,35 @@ def test_batch_size_does_not_change_the_bits():
     big = lp(sysd["R"], sysd["S"])
     small = torch.cat([lp(sysd["R"][i:i + 1024], sysd["S"][i:i + 1024]) for i in range(0, 8192, 1024)])
     assert torch.equal(big, small)
+
+
+def test_value_forward_equals_laplacian_pass_value_channel():
+    """The value-only forward (mixed precision: GELU applied inside the Dense_1 GEMM's A staging,
+    value-only determinant head) against the value channel of the local energy's Laplacian pass
+    (separate GELU jet kernel, Laplacian determinant head) on the same walkers: the same float32
+    arithmetic in a different kernel arrangement, so log psi agrees to float32 rounding."""
+    import math
+    from nqs.rashba_reference import noninteracting_gs, occupied_orbital_matrix, planewave_envelope
+    from nqs.psiformer import ContinuumSpinorPsiformer, LogPsi, ravel_params
+    from nqs.local_energy import make_local_energy
+    N, RS = 16, 3.0
+    L = RS * math.sqrt(math.pi * N)
+    g = noninteracting_gs(N, L, 1.0, 0.8, 0.05, n_cut=10, twist=(0.1, 0.17))
+    Gs, Wenv = planewave_envelope(*occupied_orbital_matrix(g))
+    m = ContinuumSpinorPsiformer(n_elec=N, Gs=Gs, W=Wenv, L=L, precision="mixed", embed_dim=128, depth=4, n_heads=8,
+                                 n_freq=8, n_dets=8)
+    params = m.init(1)
+    kr = params["params"]["backflow"]["kernel"]
+    z = torch.tensor(np.random.default_rng(3).standard_normal(tuple(kr.shape)) * 0.05)
+    params["params"]["backflow"]["kernel"] = (z + 1j * z).to(torch.complex128)
+    lp = LogPsi(m, ravel_params(params).cuda())
+    rng = np.random.default_rng(6)
+    R = torch.tensor(rng.uniform(0, L, (256, N, 2)), device="cuda")
+    S = torch.tensor(rng.integers(0, 2, (256, N)), device="cuda", dtype=torch.int32)
+    v = lp(R, S)
+    _, lv, _, _ = make_local_energy(lp, 1.0, 0.8, 0.05).parts(R, S)
+    d = (v - lv)
+    dre = float(d.real.abs().max())
+    dim = float((torch.remainder(d.imag + math.pi, 2 * math.pi) - math.pi).abs().max())
+    print(f"value forward vs Laplacian-pass value channel: max |d Re| {dre:.2e}, |d Im| {dim:.2e}", flush=True)
+    assert dre < 1e-4 and dim < 1e-4
