@@ -932,6 +932,451 @@ void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
   det_head_tail<KIND>(j, p, res, rs_, pos, spin, cfg, flip, out_lp, out_tan, out_lap);
 }
 
+// ------------------------------------------------------------------ row-per-lane head, N <= 16
+// The value-only pass (KIND 0) and the spin-flip pass (KIND 1: log psi and the two tangents of the
+// flipped electron) with four determinants per wave: lane group g = lane / 16 holds one
+// determinant, lane r of the group row r of M (orbital r; electron i in slot i), 16 complex
+// entries in registers.  Rows never move.  The pivot of step q is the lane with the largest
+// |re| + |im| in column q among the rows not yet used (first lane on ties), found by DPP
+// rotations inside the group; its row reaches the other lanes by ds_bpermute.  The pivot lanes
+// p_q are packed four bits each, and the sign of the permutation q -> p_q is the parity of its
+// inversions.
+//   KIND 0  elimination below the pivots; log det = sum of log pivot + the permutation's sign.
+//   KIND 1  Gauss-Jordan in place: after step q, slot q of lane r holds E[r][p_q], E the product
+//           of the row operations (E M = P with P[p_q][q] = 1), so M^-1[q][c] = E[p_q][c]: lane
+//           r = p_q holds row q of M^-1 with its columns in pivot order, and
+//           tr(M^-1 dM) = sum_r sum_s slot_s(r) dM[p_s][q(r)], a per-lane dot product with dM
+//           gathered along the pivot order.
+// Steps q >= N are skipped: the identity padding's pivots are exactly 1 and its rows and columns
+// take no part in the real block.  The multiplier of a step is the row entry times the pivot's
+// reciprocal conj(u) / |u|^2, as LAPACK's getf2 scales a column by 1 / u.  (The pivot search
+// compares magnitudes rounded to float: entries below 1e-38 in magnitude would compare as zero.)  A zero pivot leaves
+// the rows as they are (log det = -inf, as the LU of the N <= 16 head above), and the tangents of
+// a singular matrix are NaN, as its substitution gives them.
+struct RowDet {
+  cd m[16];
+  cd piv;             // this lane's pivot (1 while the lane has not been one)
+  int q;              // the step at which this lane was the pivot
+  unsigned pk0, pk1;  // p_q: steps 0..7 in pk0, 8..15 in pk1, four bits each
+  bool used, zero;
+};
+
+__device__ __forceinline__ double group_sum_d(double v) {  // over the 16 lanes of a row
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  return v;
+}
+__device__ __forceinline__ cd group_sum_cd(cd v) { return mk(group_sum_d(v.re), group_sum_d(v.im)); }
+__device__ __forceinline__ int group_sum_i(int v) {
+  v += dpp_i<0xB1>(v);
+  v += dpp_i<0x4E>(v);
+  v += dpp_i<0x141>(v);
+  v += dpp_i<0x140>(v);
+  return v;
+}
+__device__ __forceinline__ cd group_prod_cd(cd v) {
+  v = v * mk(dpp_d<0xB1>(v.re), dpp_d<0xB1>(v.im));
+  v = v * mk(dpp_d<0x4E>(v.re), dpp_d<0x4E>(v.im));
+  v = v * mk(dpp_d<0x141>(v.re), dpp_d<0x141>(v.im));
+  v = v * mk(dpp_d<0x140>(v.re), dpp_d<0x140>(v.im));
+  return v;
+}
+__device__ __forceinline__ int rowdet_pk(const RowDet& f, int s) {
+  return (int)(((s < 8 ? f.pk0 : f.pk1) >> (4 * (s & 7))) & 15u);
+}
+
+// m - l x, two fused multiply-adds per component
+__device__ __forceinline__ cd cfms(cd m, cd l, cd x) {
+  return mk(fma(-l.re, x.re, fma(l.im, x.im, m.re)), fma(-l.re, x.im, fma(-l.im, x.re, m.im)));
+}
+// t + a b
+__device__ __forceinline__ cd cfma(cd t, cd a, cd b) {
+  return mk(fma(a.re, b.re, fma(-a.im, b.im, t.re)), fma(a.re, b.im, fma(a.im, b.re, t.im)));
+}
+
+// Step Q.  The pivot lane writes its row (slots Q.. for KIND 0, all for KIND 1) to the group's LDS
+// row xrow and every lane of the group reads it back: the lanes of a wave issue LDS operations in
+// program order, and the next step's write follows this step's reads in every lane.
+// 1 / x for a normal x: hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// Step Q.  Pivot search on a 32-bit key: |re| + |im| rounded to float, its four low bits replaced
+// by 15 - r, so one unsigned max per DPP rotation finds the largest magnitude (to 2^-19 relative)
+// and, among equal keys, the first lane; rows already used have key 0.  The pivot lane writes its
+// row (slots Q.. for KIND 0, all for KIND 1) to the group's LDS row xrow and every lane of the
+// group reads it back: the lanes of a wave issue LDS operations in program order, and the next
+// step's write follows this step's reads in every lane.  The row update runs in chunks of four
+// slots so that only one chunk of the pivot row is held in registers at a time.
+template <int Q, int KIND>
+__device__ __forceinline__ void rowdet_step(RowDet& f, int r, cd* xrow) {
+  const float mag = (float)(fabs(f.m[Q].re) + fabs(f.m[Q].im));
+  unsigned key = f.used ? 0u : ((__float_as_uint(mag) & ~15u) | (unsigned)(15 - r));
+  key = max(key, (unsigned)dpp_i<0x121>((int)key));
+  key = max(key, (unsigned)dpp_i<0x122>((int)key));
+  key = max(key, (unsigned)dpp_i<0x124>((int)key));
+  key = max(key, (unsigned)dpp_i<0x128>((int)key));
+  const int p = 15 - (int)(key & 15u);  // the same in every lane of the group
+  const bool isp = r == p;
+  if (Q < 8) f.pk0 |= (unsigned)p << (4 * Q);
+  else f.pk1 |= (unsigned)p << (4 * (Q - 8));
+  constexpr int J0 = KIND == 0 ? Q : 0;
+  if (isp) {
+#pragma unroll
+    for (int jj = J0; jj < 16; ++jj) xrow[jj] = f.m[jj];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const cd pv = xrow[Q];
+  const bool zero = pv.re == 0.0 && pv.im == 0.0;
+  f.zero = f.zero || zero;
+  const double in = zero ? 0.0 : rcp_nr(fma(pv.re, pv.re, pv.im * pv.im));
+  const cd rp = mk(pv.re * in, -pv.im * in);
+  if (isp) {
+    f.piv = pv;
+    f.q = Q;
+  }
+  if constexpr (KIND == 0) {
+    const cd l = (f.used || isp) ? mk(0, 0) : f.m[Q] * rp;
+#pragma unroll
+    for (int jj = Q + 1; jj < 16; ++jj) {
+      f.m[jj] = cfms(f.m[jj], l, xrow[jj]);
+      if (((jj - Q) & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    const cd fr = f.m[Q] * rp;
+    const double s = isp ? 0.0 : 1.0;
+    const cd c = isp ? mk(-rp.re, -rp.im) : fr;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+      if (jj == Q) continue;
+      f.m[jj] = cfms(mk(s * f.m[jj].re, s * f.m[jj].im), c, xrow[jj]);
+      if ((jj & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    f.m[Q] = isp ? rp : mk(-fr.re, -fr.im);
+  }
+  f.used = f.used || isp;
+}
+
+template <int Q, int KIND>
+struct RowDetSteps {
+  static __device__ __forceinline__ void run(RowDet& f, int r, cd* xrow, int N) {
+    if (Q >= N) return;
+    rowdet_step<Q, KIND>(f, r, xrow);
+    RowDetSteps<Q + 1, KIND>::run(f, r, xrow, N);
+  }
+};
+template <int KIND>
+struct RowDetSteps<16, KIND> {
+  static __device__ __forceinline__ void run(RowDet&, int, cd*, int) {}
+};
+
+// log det (principal branch of the phase) from the pivots and the parity of q -> p_q
+__device__ __forceinline__ cd rowdet_logdet(const RowDet& f, int r, int N) {
+  int inv = 0;
+  if (f.used) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) inv += (s < N && s > f.q && rowdet_pk(f, s) < r) ? 1 : 0;
+  }
+  inv = group_sum_i(inv);
+  const double au = cabs_(f.piv);
+  const double la = group_sum_d(log(au));
+  cd ph = group_prod_cd(au > 0 ? mk(f.piv.re / au, f.piv.im / au) : mk(1, 0));
+  if (inv & 1) ph = mk(-ph.re, -ph.im);
+  return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
+}
+
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// det_head_tail for KIND 0 and 1 with nt <= 2, lane k taking determinant k (K <= 64): the
+// log-sum-exp weights exp(z_k - L) are evaluated once per determinant instead of once per lane.
+template <int KIND>
+__device__ __forceinline__ void det_tail_lanes(const JetGeom& j, const DetParams& p, const cd* res, int rs_,
+                                               const double* pos, const int* spin, int64_t cfg, int flip,
+                                               double* __restrict__ out_lp, double* __restrict__ out_tan) {
+  const int N = j.N, K = p.K, nt = j.nt;
+  const int lane = threadIdx.x & 63;
+  const bool hk = lane < K;
+  cd z = hk ? res[lane * rs_] : mk(0, 0);
+  cd L, pk = mk(1, 0);
+  if (K == 1) {
+    L = res[0];
+  } else {
+    if (hk) z = z + mk(p.det_logw[2 * lane], p.det_logw[2 * lane + 1]);
+    double amax = wave_max_d(hk ? z.re : -INFINITY);
+    if (!isfinite(amax)) amax = 0.0;
+    const cd s = wave_sum_cd(hk ? cexp_(z - mk(amax, 0)) : mk(0, 0));
+    L = clog_(s) + mk(amax, 0);
+    pk = hk ? cexp_(z - L) : mk(0, 0);
+  }
+  cd g1 = mk(0, 0), g2 = mk(0, 0);
+  if constexpr (KIND == 1) {
+    if (hk) {
+      g1 = pk * res[lane * rs_ + 1];
+      if (nt > 1) g2 = pk * res[lane * rs_ + 2];
+    }
+    if (K > 1) {
+      g1 = wave_sum_cd(g1);
+      g2 = wave_sum_cd(g2);
+    } else {
+      g1 = res[1];
+      g2 = nt > 1 ? res[2] : mk(0, 0);
+    }
+  }
+  if (p.use_jastrow) {
+    // lane a (< N) sums its pairs: gradient of J wrt r_a over all partners, J over partners b > a
+    const int a = lane;
+    double J = 0, gxa = 0, gya = 0;
+    if (a < N) {
+      for (int b = 0; b < N; ++b) {
+        if (b == a) continue;
+        const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+        double r, grx, gry, lapr;
+        pair_dist(p, dx, dy, r, grx, gry, lapr);
+        double u, u1, u2;
+        jastrow_u(p, spin[a] == spin[b], r, u, u1, u2);
+        gxa += grx * u1;
+        gya += gry * u1;
+        if (b > a) J += u;
+      }
+    }
+    J = wave_sum(J);
+    L = L + mk(J, 0);
+    if constexpr (KIND == 1) {
+      // tangents 1, 2: x and y of the flipped electron (KIND 1 is the spin-flip pass)
+      g1 = g1 + mk(__shfl(gxa, flip), 0);
+      g2 = g2 + mk(__shfl(gya, flip), 0);
+    }
+  }
+  if (lane == 0) {
+    out_lp[2 * cfg] = L.re;
+    out_lp[2 * cfg + 1] = L.im;
+    if constexpr (KIND == 1) {
+      out_tan[cfg * nt * 2] = g1.re;
+      out_tan[cfg * nt * 2 + 1] = g1.im;
+      if (nt > 1) {
+        out_tan[cfg * nt * 2 + 2] = g2.re;
+        out_tan[cfg * nt * 2 + 3] = g2.im;
+      }
+    }
+  }
+}
+
+// G configurations per workgroup, determinant t = 4 wave + lane / 16 of the workgroup is k = t % K
+// of configuration t / K.  Dynamic LDS (rowdet_lds): env [G][16][17] (row pitch 17 keeps the
+// per-lane row reads off one bank), the flipped column's d env / dx, dy [G][2][16] (KIND 1),
+// positions [G][32], spins [G][16], flip index [G], the pivot rows of the 16 groups [16][16], the
+// envelope weights [N][n_g][2] (when small) and the plane-wave phases [G][n_g][16].  Results per determinant go to resg [cfg][K][nt + 2]: log
+// det, then the tangents.
+// the envelope weights W[a][g][s] are copied to LDS when they take at most this many bytes
+constexpr int kRowDetWenvMax = 16384;
+__host__ __device__ inline size_t rowdet_lds(int G, int N, int n_g, int kind) {
+  const size_t head = (size_t)G * (272 + (kind == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
+                      (size_t)G * 17 * sizeof(int);
+  const size_t wb = (size_t)N * n_g * 2 * sizeof(cd);
+  const size_t a = (head + 15) / 16 * 16 + 256 * sizeof(cd) + (wb <= (size_t)kRowDetWenvMax ? wb : 0);
+  return a + (n_g <= kPhaseMax ? (size_t)G * n_g * 16 * sizeof(cd) : 0);
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 3 : 4)))
+void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
+                  int64_t w0, const T* __restrict__ bf, cd* __restrict__ resg) {
+  extern __shared__ __align__(16) char smem[];
+  const int N = j.N, K = p.K, nt = j.nt, rs_ = nt + 2, C = j.C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x, nwaves = nthr >> 6;
+  const int64_t cfg0 = (int64_t)blockIdx.x * G;
+  const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
+  cd* env = reinterpret_cast<cd*>(smem);
+  cd* dcol = env + G * 272;
+  double* pos = reinterpret_cast<double*>(dcol + (KIND == 1 ? G * 32 : 0));
+  int* spin = reinterpret_cast<int*>(pos + G * 32);
+  int* flp = spin + G * 16;
+  const size_t head = (size_t)G * (272 + (KIND == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
+                      (size_t)G * 17 * sizeof(int);
+  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
+  const int n_g = p.n_g;
+  const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
+  cd* wtab = xrows + 256;
+  cd* un = wtab + (wlds ? N * n_g * 2 : 0);
+  if (wlds)
+    for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
+  const int r = lane & 15;
+  const int t = 4 * wave + (lane >> 4);
+  const int cl = t / K, k = t % K;
+  const bool act = cl < ng;  // the same for the 16 lanes of a group
+  const T* bfc0 = bf + (cfg0 + (act ? cl : 0)) * C * chs;
+  // row r of bf; the value-only pass loads it before the envelope is built so that its latency
+  // overlaps that work (the spin-flip pass has no registers to spare there)
+  T bre[16], bim[16];
+  auto load_row = [&]() {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const T* q = bfc0 + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
+      bre[i] = act ? q[0] : (T)0;
+      bim[i] = act ? q[KN] : (T)0;
+    }
+  };
+  if constexpr (KIND == 0) load_row();
+  for (int e = tid; e < ng * 16; e += nthr) {
+    const int c = e >> 4, t = e & 15;
+    const int64_t cfg = cfg0 + c;
+    int64_t w;
+    int flip = -1;
+    if (j.mode == 2) {
+      w = w0 + cfg / N;
+      flip = (int)(cfg % N);
+    } else {
+      w = w0 + cfg;
+    }
+    if (t == 0) flp[c] = flip;
+    if (t < N) {
+      pos[c * 32 + 2 * t] = R[(w * N + t) * 2];
+      pos[c * 32 + 2 * t + 1] = R[(w * N + t) * 2 + 1];
+      const int s = S[w * N + t];
+      spin[c * 16 + t] = (t == flip) ? 1 - s : s;
+    }
+  }
+  __syncthreads();
+  const bool tab = n_g <= kPhaseMax;
+  if (tab) {
+    for (int e = tid; e < ng * n_g * 16; e += nthr) {
+      const int c = e / (n_g * 16), gg = (e >> 4) % n_g, i = e & 15;
+      double sn = 0, cs = 0;
+      if (i < N) sincos(p.gs[2 * gg] * pos[c * 32 + 2 * i] + p.gs[2 * gg + 1] * pos[c * 32 + 2 * i + 1], &sn, &cs);
+      un[e] = mk(cs, sn);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < ng * 256; e += nthr) {  // envelope, entry (a, i) of configuration c
+    const int c = e >> 8, a = (e >> 4) & 15, i = e & 15;
+    const bool fc = KIND == 1 && i == flp[c];
+    cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0);
+    if (a < N && i < N) {
+      const double px = pos[c * 32 + 2 * i], py = pos[c * 32 + 2 * i + 1];
+      const int si = spin[c * 16 + i];
+      for (int gg = 0; gg < n_g; ++gg) {
+        const double Gx = p.gs[2 * gg], Gy = p.gs[2 * gg + 1];
+        double sn, cs;
+        if (tab) {
+          const cd z = un[(c * n_g + gg) * 16 + i];
+          cs = z.re;
+          sn = z.im;
+        } else {
+          sincos(Gx * px + Gy * py, &sn, &cs);
+        }
+        const int we = (a * n_g + gg) * 2 + si;
+        const cd t = (wlds ? wtab[we] : mk(p.wenv[2 * we], p.wenv[2 * we + 1])) * mk(cs, sn);
+        v = v + t;
+        if (fc) {
+          gx = gx + t * mk(0, Gx);
+          gy = gy + t * mk(0, Gy);
+        }
+      }
+    }
+    env[c * 272 + a * 17 + i] = v;
+    if (fc) {
+      dcol[c * 32 + a] = gx;
+      dcol[c * 32 + 16 + a] = gy;
+    }
+  }
+  __syncthreads();
+  if constexpr (KIND == 1) load_row();
+  if (act) {
+    const cd* envc = env + cl * 272;
+    RowDet f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      f.m[i] = (r < N && i < N) ? envc[r * 17 + i] * mk(1.0 + (double)bre[i], (double)bim[i])
+                                : mk(r == i ? 1.0 : 0.0, 0);
+    f.piv = mk(1, 0);
+    f.q = r;
+    f.pk0 = f.pk1 = 0;
+    f.used = f.zero = false;
+    RowDetSteps<0, KIND>::run(f, r, xrows + 16 * (lane >> 4) + 64 * wave, N);
+    const cd ld = rowdet_logdet(f, r, N);
+    cd* rk = resg + ((cfg0 + cl) * K + k) * rs_;
+    if (r == 0) rk[0] = ld;
+    if constexpr (KIND == 1) {
+      // tangents c = 1, 2: d/dx, d/dy of the flipped electron
+      const int i = f.q, flip = flp[cl];
+      const cd* dc = dcol + cl * 32;
+      cd t1 = mk(0, 0), t2 = mk(0, 0);
+      if (f.used) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+          if (s >= N) continue;
+          const int a = rowdet_pk(f, s);
+          const cd e = envc[a * 17 + i];
+          cd d1 = e * bf_at(bfc0 + chs, rowlen, KN, N, k, a, i);
+          cd d2 = e * bf_at(bfc0 + 2 * chs, rowlen, KN, N, k, a, i);
+          if (i == flip) {
+            const cd ob = mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, a, i);
+            d1 = d1 + dc[a] * ob;
+            d2 = d2 + dc[16 + a] * ob;
+          }
+          t1 = cfma(t1, f.m[s], d1);
+          t2 = cfma(t2, f.m[s], d2);
+        }
+      }
+      t1 = group_sum_cd(t1);
+      t2 = group_sum_cd(t2);
+      if (f.zero) t1 = t2 = mk(NAN, NAN);
+      if (r == 0) {
+        rk[1] = t1;
+        rk[2] = t2;
+      }
+    }
+  }
+}
+
+// The log-sum-exp over the determinants and the Jastrow factor of k_det_rows16's results
+// res [cfg][K][nt + 2], one wave per configuration (kept out of the determinant kernel, whose
+// register budget it would otherwise set).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_det_tail16(JetGeom j, DetParams p, const double* __restrict__ R,
+                                                    const int32_t* __restrict__ S, int64_t w0,
+                                                    const cd* __restrict__ res, double* __restrict__ out_lp,
+                                                    double* __restrict__ out_tan) {
+  __shared__ double pos[4][32];
+  __shared__ int spin[4][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t cfg = (int64_t)blockIdx.x * 4 + wave;
+  if (cfg >= j.n_cfg) return;  // the whole wave; no workgroup barrier follows
+  const int N = j.N, rs_ = j.nt + 2;
+  int64_t w;
+  int flip = -1;
+  if (j.mode == 2) {
+    w = w0 + cfg / N;
+    flip = (int)(cfg % N);
+  } else {
+    w = w0 + cfg;
+  }
+  if (p.use_jastrow && lane < N) {
+    pos[wave][2 * lane] = R[(w * N + lane) * 2];
+    pos[wave][2 * lane + 1] = R[(w * N + lane) * 2 + 1];
+    const int s = S[w * N + lane];
+    spin[wave][lane] = lane == flip ? 1 - s : s;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  det_tail_lanes<KIND>(j, p, res + cfg * p.K * rs_, rs_, pos[wave], spin[wave], cfg, flip, out_lp, out_tan);
+}
+
 // ------------------------------------------------------------------ wave-level head for 16 < N <= 32
 // The N <= 16 scheme on 32 x 32 matrices (padded with an identity block): a matrix is 2 x 2
 // blocks of 16 x 16, each in one of the two layouts above, lane = 16 g + l:
@@ -1372,9 +1817,32 @@ void k_det_jet32(JetGeom j, DetParams p, const double* __restrict__ R,
 // Note on rint vs jnp.round: both round half to even; minimum image is exact either way.
 template <typename T>
 int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const int32_t* S, int64_t w0,
-                   const T* bf, double* out_lp, double* out_tan, double* out_lap, hipStream_t st) {
+                   const T* bf, double* out_lp, double* out_tan, double* out_lap, void* scratch, size_t scratch_bytes,
+                   hipStream_t st) {
   if (j.n_cfg == 0) return NQS_OK;
   const int N = j.N, NN = N * N;
+  // value-only and spin-flip passes: four determinants per wave, one row per lane, then the
+  // log-sum-exp per configuration; the per-determinant results are staged in `scratch`
+  if (N <= 16 && p.K <= 16 && !j.has_lap && (j.nt == 0 || (j.mode == 2 && j.nt == 2))) {
+    const int kind = j.nt == 0 ? 0 : 1;
+    const int G = std::max(1, std::min(4, 16 / p.K));
+    const int waves = std::min(4, (G * p.K + 3) / 4);
+    const size_t lds = rowdet_lds(G, N, p.n_g, kind);
+    const size_t need = (size_t)j.n_cfg * p.K * (j.nt + 2) * sizeof(cd);
+    if (lds <= 64 * 1024 && scratch != nullptr && need <= scratch_bytes) {
+      cd* res = static_cast<cd*>(scratch);
+      const dim3 grid((unsigned)cdiv(j.n_cfg, G)), block(64 * waves), tgrid((unsigned)cdiv(j.n_cfg, 4));
+      if (kind == 0) {
+        hipLaunchKernelGGL((k_det_rows16<T, 0>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
+        hipLaunchKernelGGL(k_det_tail16<0>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
+      } else {
+        hipLaunchKernelGGL((k_det_rows16<T, 1>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
+        hipLaunchKernelGGL(k_det_tail16<1>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
+      }
+      NQS_LAUNCH_CHECK();
+      return NQS_OK;
+    }
+  }
   if (N <= 16 && j.nt <= 64) {
     const size_t lds = sizeof(cd) * (size_t)p.K * (j.nt + 2);
     NQS_REQUIRE(lds <= 96 * 1024, NQS_ERR_UNSUPPORTED, "determinant head exceeds LDS");
@@ -1438,9 +1906,9 @@ int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const 
 }
 
 template int launch_det_jet<double>(const JetGeom&, const DetParams&, const double*, const int32_t*, int64_t,
-                                    const double*, double*, double*, double*, hipStream_t);
+                                    const double*, double*, double*, double*, void*, size_t, hipStream_t);
 template int launch_det_jet<float>(const JetGeom&, const DetParams&, const double*, const int32_t*, int64_t,
-                                   const float*, double*, double*, double*, hipStream_t);
+                                   const float*, double*, double*, double*, void*, size_t, hipStream_t);
 
 
 // ------------------------------------------------------------------ determinant-head backward

@@ -103,9 +103,11 @@ struct DetParams {
   const double* jas;      // jastrow params: rbf [c_same(n_knots), c_opp(n_knots)]; cusp [loga]
 };
 template <typename T>
+// scratch (may be NULL): device memory the head may use for per-determinant results, at least
+// n_cfg * K * (nt + 2) * 16 bytes for the value-only and spin-flip passes' row-per-lane kernels.
 int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const int32_t* S,
                    int64_t w0, const T* bf, double* out_logpsi, double* out_tan, double* out_lap,
-                   hipStream_t st);
+                   void* scratch, size_t scratch_bytes, hipStream_t st);
 
 int launch_ewald(const nqs_ewald_desc& e, const double* R, int64_t W, int N, double* out,
                  hipStream_t st);

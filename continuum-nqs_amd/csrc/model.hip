@@ -405,7 +405,9 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
   NQS_TRY(gemm_nt<T>(gargs<T>(md, b.h, pk + md.bfT, b.big, rows, 2 * KN, d, d, d, 2 * KN, pk + md.bf_b, per, blen, (T)0),
                      1, st));
   const DetParams dp = det_params(md, packed);
-  NQS_TRY(launch_det_jet<T>(jg, dp, R, S, w0, b.big, b.lp, b.tan, b.lap, st));
+  // the attention buffer is free here: it holds the per-determinant results
+  const int64_t watt = std::max<int64_t>(md.d, (int64_t)md.ff * md.d);
+  NQS_TRY(launch_det_jet<T>(jg, dp, R, S, w0, b.big, b.lp, b.tan, b.lap, b.att, (size_t)(rows * watt * sizeof(T)), st));
   return NQS_OK;
 }
 
