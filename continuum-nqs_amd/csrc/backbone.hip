@@ -13,6 +13,7 @@
 #include "nqs_common.h"
 #include "kernels.h"
 #include "tune.h"
+#include "attn.h"
 
 namespace nqs {
 
@@ -109,27 +110,19 @@ __global__ __launch_bounds__(64) void k_layernorm_jet(JetGeom j, int d, const T*
   const T* xb = x + cfg * j.C * chs + (int64_t)t * d;
   T* yb = y + cfg * j.C * chs + (int64_t)t * d;
   const T inv_d = (T)1 / (T)d;
-  T xv[MAXF], sc[MAXF], s1[MAXF];
-  T sum = 0, sum2 = 0;
+  T xv[MAXF], sc[MAXF], s1[MAXF], y0[MAXF];
 #pragma unroll
   for (int q = 0; q < MAXF; ++q) {
     const int f = lane + 64 * q;
     xv[q] = (f < d) ? xb[f] : (T)0;
-    sc[q] = (f < d) ? scale[f] : (T)0;
     s1[q] = 0;
-    sum += xv[q];
-    sum2 += xv[q] * xv[q];
   }
-  sum = wave_sum(sum);
-  sum2 = wave_sum(sum2);
-  const T mu = sum * inv_d;
-  T var = sum2 * inv_d - mu * mu;
-  var = var > (T)0 ? var : (T)0;
-  const T rs = (T)1 / sqrt(var + (T)1e-6);
+  T mu, rs;
+  ln_value<T, MAXF>(xv, lane, d, scale, bias, sc, y0, mu, rs);
 #pragma unroll
   for (int q = 0; q < MAXF; ++q) {
     const int f = lane + 64 * q;
-    if (f < d) yb[f] = (xv[q] - mu) * rs * sc[q] + bias[f];
+    if (f < d) yb[f] = y0[q];
   }
   T acc_dx2 = 0, acc_dmu2 = 0, acc_dvar2 = 0;
   for (int c = 1; c <= j.nt; ++c) {
@@ -408,74 +401,6 @@ __global__ __launch_bounds__(256) void k_attention_jet_wave(JetGeom j, int d, in
   }
 }
 
-// Register-resident variant on 16x16x4 MFMA for N <= 16 tokens and head width hd <= 16 NQ
-// (NQ = 1, 2).
-// One wave per (cfg, head), no LDS.  Tokens are padded to 16 and features to 16 NQ with zeros
-// (padded keys get score -inf, so their softmax weight is exactly 0).
-//
-// Scores are formed transposed, S^T = K Q^T, so a lane owns query a = lane%16 and the four keys
-// b = row(lane, r) of its accumulator slots; a softmax row is then 4 slots of 16 lanes' worth
-// reduced across the lane groups lane/16 (shuffles by 16 and 32).  The feature index of k-slot
-// g in MFMA step s is q = 16 (s/4) + 4 g + s%4, so each lane reads 4 consecutive features of
-// its token.  For P V the k index (the key) of slot g in step s is b = row(lane, s), i.e. the
-// lane's own softmax slot s, so P enters the MFMA straight from registers and V is read as
-// V[row(lane, s)][16 t + lane%16].
-template <typename T, int NQ>
-struct AttnRegs {
-  T q[4 * NQ], k[4 * NQ], v[4 * NQ];
-};
-
-template <typename T, int NQ>
-__device__ __forceinline__ void attn_load(const T* __restrict__ base, int lane, int N, int hd, int d, int h,
-                                          int64_t rs, AttnRegs<T, NQ>& x) {
-  const int li = lane & 15, g = lane >> 4;
-  const int tq = li < N ? li : N - 1;
-  if constexpr (sizeof(T) == 4) {
-    if (hd == 16 * NQ) {
-      // full-width head: a lane's 4 consecutive q (k) features are one 16-byte load
-      typedef T v4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-      for (int t = 0; t < NQ; ++t) {
-        const v4 vq = *reinterpret_cast<const v4*>(base + tq * rs + h * hd + 16 * t + 4 * g);
-        const v4 vk = *reinterpret_cast<const v4*>(base + tq * rs + d + h * hd + 16 * t + 4 * g);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          x.q[4 * t + e] = li < N ? vq[e] : (T)0;
-          x.k[4 * t + e] = li < N ? vk[e] : (T)0;
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < 4 * NQ; ++s) {
-        const int b = Mma16<T>::row(lane, s & 3), qv = 16 * (s >> 2) + li;
-        const int bc = b < N ? b : N - 1;
-        const T vv = base[bc * rs + 2 * d + h * hd + qv];
-        x.v[s] = b < N ? vv : (T)0;
-      }
-      return;
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 4 * NQ; ++s) {
-    const int q = 16 * (s >> 2) + 4 * g + (s & 3);
-    const int qc = q < hd ? q : hd - 1;
-    const bool ok = li < N && q < hd;
-    const T vq = base[tq * rs + h * hd + qc];
-    const T vk = base[tq * rs + d + h * hd + qc];
-    x.q[s] = ok ? vq : (T)0;
-    x.k[s] = ok ? vk : (T)0;
-    const int b = Mma16<T>::row(lane, s & 3), qv = 16 * (s >> 2) + li;
-    const int bc = b < N ? b : N - 1, qvc = qv < hd ? qv : hd - 1;
-    const T vv = base[bc * rs + 2 * d + h * hd + qvc];
-    x.v[s] = (b < N && qv < hd) ? vv : (T)0;
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ T xsum16(T v) {
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
-}
 
 template <typename T, int NQ>
 __device__ __forceinline__ void attn_store(T* __restrict__ o, int lane, int N, int hd, int d, int h,
@@ -513,39 +438,10 @@ __global__ __launch_bounds__(512) void k_attention_jet_mfma(JetGeom j, int d, in
   // ---- value channel
   AttnRegs<T, NQ> x0;
   attn_load<T, NQ>(in0, lane, N, hd, d, h, rs, x0);
-  acc_t s0 = zacc;
-#pragma unroll
-  for (int s = 0; s < 4 * NQ; ++s) s0 = Mma16<T>::mma(x0.k[s], x0.q[s], s0);
   T P[4];
   {
-    T sv[4], mx;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) sv[r] = Mma16<T>::row(lane, r) < N ? s0[r] * sc : -INFINITY;
-    mx = sv[0];
-#pragma unroll
-    for (int r = 1; r < 4; ++r) mx = sv[r] > mx ? sv[r] : mx;
-    T o = __shfl_xor(mx, 16);
-    mx = o > mx ? o : mx;
-    o = __shfl_xor(mx, 32);
-    mx = o > mx ? o : mx;
-    T z = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      P[r] = exp(sv[r] - mx);
-      z += P[r];
-    }
-    const T iz = (T)1 / xsum16(z);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) P[r] *= iz;
-  }
-  {
     acc_t o[NQ];
-#pragma unroll
-    for (int t = 0; t < NQ; ++t) {
-      o[t] = zacc;
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) o[t] = Mma16<T>::mma(P[kb], x0.v[4 * t + kb], o[t]);
-    }
+    attn_value<T, NQ>(x0, lane, N, sc, P, o);
     attn_store<T, NQ>(out0, lane, N, hd, d, h, o, (T)0, zq);
   }
   if (j.nt == 0 && !j.has_lap) return;

@@ -1,0 +1,322 @@
+// Fused value-only Psiformer forward for gfx950 (mixed precision, bf16x3 dense layers).
+//
+// The value-only jet (one channel: the sampler's proposals and the log psi apply) runs the whole
+// backbone in one launch: embedding, every pre-norm block (LayerNorm, q/k/v, multi-head
+// attention, output projection and residual, LayerNorm, Dense_0, tanh-GELU, Dense_1 and
+// residual), the final LayerNorm and the complex backflow projection (src/nqs/psiformer.py:67-81,
+// 128-136).  Only the Fourier features come in and only the backflow activations go out; nothing
+// in between touches HBM.  The layer-by-layer path of the same forward moves every activation
+// through HBM between its kernels (about 50 KB per token row per forward).
+//
+// Work split.  One workgroup of 4 waves takes 32 token rows (two configurations of 16 tokens).
+// Wave w owns columns 32 w .. 32 w + 31 of the residual stream x for all 32 rows, as one
+// v_mfma_f32_32x32x16_bf16 accumulator block kept in registers for the whole forward, so the
+// residual adds need no data movement.  LDS holds the bf16 hi/lo planes of the current dense
+// layer's input (LayerNorm output or attention output, K = 128) and one more region that is in
+// turn the LayerNorm input (float), q | k | v (float) and the GELU output planes (K = 256):
+// 66 KB, two workgroups per CU.  Weights come from global memory (L2) as pre-split bf16 hi/lo
+// operand fragments that the packer writes once per parameter update.
+//
+// Arithmetic.  Every dense block accumulates exactly as k_gemm_nt_bf does (16-wide k chunks,
+// lo*hi, hi*lo, hi*hi per chunk, the same number of chunks, the same epilogue: acc + bias, or
+// acc + (x + bias) for the residual layers); the LayerNorm value row is ln_value and the
+// attention value channel attn_value, the functions the layer-by-layer kernels use; GELU is
+// gelu_derivs.  The fused forward therefore reproduces the layer-by-layer forward's float32
+// values.
+#include "nqs_common.h"
+#include "kernels.h"
+#include "attn.h"
+#include "model.h"
+#include <cstdlib>
+
+namespace nqs {
+
+constexpr int FU_D = 128, FU_N = 16, FU_NH = 8, FU_HD = 16, FU_FF = 256;
+constexpr int FU_R = 32;              // token rows per workgroup
+constexpr int FU_HLD = FU_D + 8;      // bf16 row pitch of the K = 128 operand planes
+constexpr int FU_GLD = FU_FF + 8;     // bf16 row pitch of the K = 256 operand planes
+constexpr int FU_QLD = 3 * FU_D + 4;  // float row pitch of q | k | v
+constexpr int FU_MAXL = 8;            // layers
+constexpr int FU_HBYTES = 2 * FU_R * FU_HLD * 2;
+constexpr int FU_QBYTES = FU_R * FU_QLD * 4;
+constexpr int FU_LDS = FU_HBYTES + FU_QBYTES;
+static_assert(2 * FU_R * FU_GLD * 2 <= FU_QBYTES && FU_R * FU_D * 4 <= FU_QBYTES, "LDS region sizes");
+
+// A dense layer's weights (N_out x K, K-contiguous, K padded with zeros to a multiple of 16) as
+// bf16 hi/lo operand fragments of v_mfma_f32_32x32x16_bf16, written once by the packer:
+// frag[nb][kc][plane][lane][8] holds, for column block nb and 16-wide k chunk kc, lane l's eight
+// values of column 32 nb + l % 32 at k = 16 kc + 8 (l / 32) + e (plane 0 hi, plane 1 lo), so a
+// wave reads one operand as a contiguous 1 KB and does no conversion work.
+struct FusedLayer {
+  const float *ln0_s, *ln0_b, *qkv_b, *o_b, *ln1_s, *ln1_b, *b0, *b1;
+  const u4v *qkv, *o, *w0, *w1;  // fragments
+};
+struct FusedFwd {
+  const float* X0;  // rows x kpad features
+  int64_t rows;
+  int kpad;
+  const u4v* emb;  // fragments, K padded to 64
+  const float* emb_b;
+  FusedLayer L[FU_MAXL];
+  int depth;
+  const float *lnf_s, *lnf_b, *bf_b;
+  const u4v* bf;  // fragments
+  int nbf;        // backflow outputs per row (2 K N)
+  float* out;     // rows x nbf
+};
+
+// acc = A W[32 nb .. 32 nb + 32)^T over the first NCH of the nkc 16-wide k chunks: A is the 32-row
+// bf16 hi/lo operand in LDS (pitch lda), W the layer's fragments.  The same products in the same
+// order as k_gemm_nt_bf (lo*hi, hi*lo, hi*hi per chunk; zero chunks past K included).
+template <int NCH>
+__device__ __forceinline__ f16v fu_block(const __bf16* Ah, const __bf16* Al, int lda, const u4v* __restrict__ frag,
+                                         int nkc, int nb, int lane) {
+  f16v acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  const u4v* fb = frag + (int64_t)nb * nkc * 128 + lane;
+  const int ko = 8 * (lane >> 5);
+  const __bf16* ah_p = Ah + (lane & 31) * lda + ko;
+  const __bf16* al_p = Al + (lane & 31) * lda + ko;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    const bf8v bh = __builtin_bit_cast(bf8v, fb[c * 128]);
+    const bf8v bl = __builtin_bit_cast(bf8v, fb[c * 128 + 64]);
+    const bf8v ah = *reinterpret_cast<const bf8v*>(ah_p + 16 * c);
+    const bf8v al = *reinterpret_cast<const bf8v*>(al_p + 16 * c);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Packer: src (N x K floats, row stride ld) -> fragments with nkc = Kp / 16 chunks (k >= K zero).
+__global__ void k_fragsplit(const float* __restrict__ src, int N, int K, int ld, int nkc, u4v* __restrict__ dst) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one (nb, kc, lane)
+  if (id >= (int64_t)(N / 32) * nkc * 64) return;
+  const int lane = (int)(id % 64);
+  const int64_t kc = (id / 64) % nkc, nb = id / 64 / nkc;
+  const float* p = src + (nb * 32 + (lane & 31)) * (int64_t)ld;
+  const int k0 = 16 * (int)kc + 8 * (lane >> 5);
+  float v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = k0 + e < K ? p[k0 + e] : 0.f;
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) split_bf16x2(v[2 * e], v[2 * e + 1], h[e], l[e]);
+  u4v* q = dst + ((nb * nkc + kc) * 2) * 64 + lane;
+  q[0] = (u4v){h[0], h[1], h[2], h[3]};
+  q[64] = (u4v){l[0], l[1], l[2], l[3]};
+}
+
+// accumulator slot r of a lane: row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32 of the block
+__device__ __forceinline__ int fu_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// one float into the bf16 hi/lo planes at element offset o
+__device__ __forceinline__ void fu_put(__bf16* Ph, __bf16* Pl, int o, float v) {
+  uint32_t h, l;
+  split_bf16x2(v, 0.f, h, l);
+  reinterpret_cast<uint16_t*>(Ph)[o] = (uint16_t)(h & 0xffffu);
+  reinterpret_cast<uint16_t*>(Pl)[o] = (uint16_t)(l & 0xffffu);
+}
+
+// LayerNorm of the residual stream (wave w's block xr) into the K = 128 operand planes: the
+// blocks go through the float scratch Xs, then wave w normalises rows 8 w .. 8 w + 7 as
+// k_layernorm_jet does (lane holds features lane and lane + 64).
+__device__ __forceinline__ void fu_layernorm(const f16v& xr, int wave, int lane, float* Xs, const float* __restrict__ s,
+                                             const float* __restrict__ b, __bf16* Hh, __bf16* Hl) {
+  __syncthreads();  // every wave is done with the region Xs aliases
+#pragma unroll
+  for (int r = 0; r < 16; ++r) Xs[fu_row(lane, r) * FU_D + 32 * wave + (lane & 31)] = xr[r];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < FU_R / 4; ++i) {
+    const int row = (FU_R / 4) * wave + i;
+    float xv[2] = {Xs[row * FU_D + lane], Xs[row * FU_D + lane + 64]};
+    float sc[2], y[2], mu, rs;
+    ln_value<float, 2>(xv, lane, FU_D, s, b, sc, y, mu, rs);
+    fu_put(Hh, Hl, row * FU_HLD + lane, y[0]);
+    fu_put(Hh, Hl, row * FU_HLD + lane + 64, y[1]);
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
+  extern __shared__ __attribute__((aligned(16))) char fsm[];
+  __bf16* Hh = reinterpret_cast<__bf16*>(fsm);
+  __bf16* Hl = Hh + FU_R * FU_HLD;
+  char* rq = fsm + FU_HBYTES;
+  float* Q = reinterpret_cast<float*>(rq);      // q | k | v rows, and the LayerNorm scratch
+  __bf16* Gh = reinterpret_cast<__bf16*>(rq);   // GELU output planes
+  __bf16* Gl = Gh + FU_R * FU_GLD;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * FU_R;
+  const int cw = 32 * wave + (lane & 31);  // this lane's column of the wave's block
+
+  // ---- features -> operand planes (k >= kpad zero up to 64)
+  {
+    const int r = tid >> 3, k0 = 8 * (tid & 7);
+    const int64_t gr = row0 + r < a.rows ? row0 + r : a.rows - 1;
+    f4v v0 = (f4v)0.f, v1 = (f4v)0.f;
+    if (k0 < a.kpad) {
+      v0 = *reinterpret_cast<const f4v*>(a.X0 + gr * a.kpad + k0);
+      v1 = *reinterpret_cast<const f4v*>(a.X0 + gr * a.kpad + k0 + 4);
+    }
+    uint32_t h_[4], l_[4];
+    split_bf16x2(v0[0], v0[1], h_[0], l_[0]);
+    split_bf16x2(v0[2], v0[3], h_[1], l_[1]);
+    split_bf16x2(v1[0], v1[1], h_[2], l_[2]);
+    split_bf16x2(v1[2], v1[3], h_[3], l_[3]);
+    *reinterpret_cast<u4v*>(Hh + r * FU_HLD + k0) = (u4v){h_[0], h_[1], h_[2], h_[3]};
+    *reinterpret_cast<u4v*>(Hl + r * FU_HLD + k0) = (u4v){l_[0], l_[1], l_[2], l_[3]};
+  }
+  __syncthreads();
+  f16v xr;
+  {
+    const f16v acc = a.kpad <= 32 ? fu_block<2>(Hh, Hl, FU_HLD, a.emb, 4, wave, lane)
+                                  : fu_block<4>(Hh, Hl, FU_HLD, a.emb, 4, wave, lane);
+    const float bv = a.emb_b[cw];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xr[r] = acc[r] + bv;
+  }
+  const float sc_att = 1.f / sqrtf((float)FU_HD);
+  for (int l = 0; l < a.depth; ++l) {
+    const FusedLayer& L = a.L[l];
+    fu_layernorm(xr, wave, lane, Q, L.ln0_s, L.ln0_b, Hh, Hl);
+    // q, k, v: wave w takes column blocks w, w + 4, w + 8 of the 384
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int col0 = 32 * (wave + 4 * j);
+      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.qkv, 8, wave + 4 * j, lane);
+      const float bv = L.qkv_b[col0 + (lane & 31)];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Q[fu_row(lane, r) * FU_QLD + col0 + (lane & 31)] = acc[r] + bv;
+    }
+    __syncthreads();
+    // attention: wave w takes (configuration, head) pairs 4 w .. 4 w + 3 of the 2 x 8
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int p = 4 * wave + t, cfg = p >> 3, h = p & 7;
+      AttnRegs<float, 1> x0;
+      attn_load<float, 1>(Q + cfg * FU_N * FU_QLD, lane, FU_N, FU_HD, FU_D, h, FU_QLD, x0);
+      float P[4];
+      f4v o[1];
+      attn_value<float, 1>(x0, lane, FU_N, sc_att, P, o);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int tok = Mma16<float>::row(lane, r);
+        fu_put(Hh, Hl, (cfg * FU_N + tok) * FU_HLD + h * FU_HD + (lane & 15), o[0][r]);
+      }
+    }
+    __syncthreads();
+    {
+      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.o, 8, wave, lane);
+      const float bv = L.o_b[cw];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) xr[r] = acc[r] + (xr[r] + bv);
+    }
+    fu_layernorm(xr, wave, lane, Q, L.ln1_s, L.ln1_b, Hh, Hl);
+    // Dense_0 + GELU: wave w takes column blocks w and w + 4 of the 256 hidden units
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col0 = 32 * (wave + 4 * j);
+      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.w0, 8, wave + 4 * j, lane);
+      const float bv = L.b0[col0 + (lane & 31)];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float u = acc[r] + bv;
+        float g0, g1, g2;
+        gelu_derivs<float>(u, g0, g1, g2);
+        fu_put(Gh, Gl, fu_row(lane, r) * FU_GLD + col0 + (lane & 31), g0);
+      }
+    }
+    __syncthreads();
+    {
+      const f16v acc = fu_block<16>(Gh, Gl, FU_GLD, L.w1, 16, wave, lane);
+      const float bv = L.b1[cw];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) xr[r] = acc[r] + (xr[r] + bv);
+    }
+  }
+  fu_layernorm(xr, wave, lane, Q, a.lnf_s, a.lnf_b, Hh, Hl);
+  // backflow: wave w takes column blocks w, w + 4 of the nbf outputs
+  for (int j = wave; j < a.nbf / 32; j += 4) {
+    const int col0 = 32 * j;
+    const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, a.bf, 8, j, lane);
+    const float bv = a.bf_b[col0 + (lane & 31)];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t gr = row0 + fu_row(lane, r);
+      if (gr < a.rows) a.out[gr * a.nbf + col0 + (lane & 31)] = acc[r] + bv;
+    }
+  }
+}
+
+bool fused_model_ok(const Model& md) {
+  if (md.split != 3 || md.N != FU_N || md.d != FU_D || md.nh != FU_NH || md.ff * md.d != FU_FF) return false;
+  if (md.depth > FU_MAXL || md.kpad > 64 || md.kpad % 8 != 0) return false;
+  return (2 * md.KN) % 32 == 0 && 2 * md.KN <= 256;
+}
+
+bool fused_value_ok(const Model& md, const JetGeom& jg) {
+  if (!md.fused || md.tsize != 4 || jg.C != 1 || jg.nt != 0 || jg.has_lap) return false;
+  const char* e = getenv("NQS_LAYERWISE_FORWARD");
+  return !(e && e[0] == '1');
+}
+
+int pack_fused_weights(const Model& md, float* pk, hipStream_t st) {
+  auto split = [&](int64_t src, int N, int K, int ld, int Kp, int64_t dst) -> int {
+    const int nkc = Kp / 16;
+    const int64_t n = (int64_t)(N / 32) * nkc * 64;
+    hipLaunchKernelGGL(k_fragsplit, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, pk + src, N, K, ld, nkc,
+                       reinterpret_cast<u4v*>(pk + dst));
+    NQS_LAUNCH_CHECK();
+    return NQS_OK;
+  };
+  const int d = md.d, ffd = md.ff * md.d;
+  NQS_TRY(split(md.embT, d, md.kpad, md.kpad, 64, md.fz_emb));
+  for (int l = 0; l < md.depth; ++l) {
+    const LayerPack& P = md.layers[l];
+    NQS_TRY(split(P.qkvT, 3 * d, d, d, d, P.fz_qkv));
+    NQS_TRY(split(P.oT, d, d, d, d, P.fz_o));
+    NQS_TRY(split(P.w0T, ffd, d, d, d, P.fz_w0));
+    NQS_TRY(split(P.w1T, d, ffd, ffd, ffd, P.fz_w1));
+  }
+  return split(md.bfT, 2 * md.KN, d, d, d, md.fz_bf);
+}
+
+int launch_fused_value(const Model& md, const float* pk, const float* X0, int64_t rows, float* out, hipStream_t st) {
+  if (rows <= 0) return NQS_OK;
+  NQS_REQUIRE(md.fused, NQS_ERR_UNSUPPORTED, "fused forward: architecture not supported");
+  NQS_REQUIRE(rows % FU_N == 0, NQS_ERR_ARG, "fused forward: rows must be whole configurations");
+  auto fr = [&](int64_t off) { return reinterpret_cast<const u4v*>(pk + off); };
+  FusedFwd a;
+  a.X0 = X0;
+  a.rows = rows;
+  a.kpad = md.kpad;
+  a.emb = fr(md.fz_emb);
+  a.emb_b = pk + md.emb_b;
+  a.depth = md.depth;
+  for (int l = 0; l < md.depth; ++l) {
+    const LayerPack& P = md.layers[l];
+    a.L[l] = FusedLayer{pk + P.ln0_s, pk + P.ln0_b, pk + P.qkv_b, pk + P.o_b, pk + P.ln1_s, pk + P.ln1_b,
+                        pk + P.b0,    pk + P.b1,    fr(P.fz_qkv), fr(P.fz_o), fr(P.fz_w0), fr(P.fz_w1)};
+  }
+  a.lnf_s = pk + md.lnf_s;
+  a.lnf_b = pk + md.lnf_b;
+  a.bf = fr(md.fz_bf);
+  a.bf_b = pk + md.bf_b;
+  a.nbf = 2 * md.KN;
+  a.out = out;
+  static bool attr = false;
+  if (!attr) {
+    NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_fused_value, hipFuncAttributeMaxDynamicSharedMemorySize, FU_LDS));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_fused_value, dim3((unsigned)cdiv(rows, FU_R)), dim3(256), FU_LDS, st, a);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+}  // namespace nqs

@@ -227,16 +227,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_v2(GemmArgs<T> g) {
 // v_mfma_f32_32x32x16_bf16 (A[lane%32][8 (lane/32) + e], C row 8 (r/4) + 4 (lane/32) + r%4,
 // col lane%32).  Tiles are dealt XCD-aware: workgroup b runs on XCD b % 8, and the column
 // tiles of one row panel go to the same XCD back to back, so the panel is read from HBM once.
-typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-
-__device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
-  const f2v v = {x0, x1};
-  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
-  const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
-  const f2v l = {x0 - h0, x1 - h1};
-  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
-}
 
 constexpr int BF_BK = 32, BF_LD = BF_BK + 8;
 // staging geometry: 8-float chunks per tile row, rows per pass of the 256 threads

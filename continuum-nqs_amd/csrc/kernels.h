@@ -25,9 +25,25 @@ struct Mma16<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// tanh-GELU (jax.nn.gelu, approximate=True) and its first two derivatives
+// float32 pair -> bf16 hi = bf16(x) and lo = bf16(x - hi), both round-to-nearest-even, packed two
+// per 32-bit word (element 0 in the low half): the operand split of the bf16x3 products.
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+__device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  const f2v v = {x0, x1};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
+  const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
+  const f2v l = {x0 - h0, x1 - h1};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l, bf2v));
+}
+
+// tanh-GELU (jax.nn.gelu, approximate=True) and its first two derivatives.  No FMA contraction:
+// every caller gets the same roundings whichever of the outputs it uses (the fused forward and the
+// layer-by-layer kernels agree bit for bit).
 template <typename T>
 __device__ inline void gelu_derivs(T x, T& g, T& g1, T& g2) {
+#pragma clang fp contract(off)
   const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
   const T u = k0 * (x + k1 * x * x * x);
   const T th = tanh(u);
@@ -102,6 +118,36 @@ struct DetParams {
   const double* det_logw; // (K) complex or NULL
   const double* jas;      // jastrow params: rbf [c_same(n_knots), c_opp(n_knots)]; cusp [loga]
 };
+// Value row of the LayerNorm (flax: mean, fast variance E[x^2] - mu^2, eps 1e-6): lane holds
+// features f = lane + 64 q (q < MAXF) in xv, zero for f >= d.  Returns the scales sc, the outputs
+// y = (x - mu) rs scale + bias (0 for f >= d), mu and rs = (var + eps)^-1/2; no FMA contraction, so
+// every caller rounds alike.
+template <typename T, int MAXF>
+__device__ __forceinline__ void ln_value(const T (&xv)[MAXF], int lane, int d, const T* __restrict__ scale,
+                                         const T* __restrict__ bias, T (&sc)[MAXF], T (&y)[MAXF], T& mu, T& rs) {
+#pragma clang fp contract(off)
+  const T inv_d = (T)1 / (T)d;
+  T sum = 0, sum2 = 0;
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    const int f = lane + 64 * q;
+    sc[q] = (f < d) ? scale[f] : (T)0;
+    sum += xv[q];
+    sum2 += xv[q] * xv[q];
+  }
+  sum = wave_sum(sum);
+  sum2 = wave_sum(sum2);
+  mu = sum * inv_d;
+  T var = sum2 * inv_d - mu * mu;
+  var = var > (T)0 ? var : (T)0;
+  rs = (T)1 / sqrt(var + (T)1e-6);
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    const int f = lane + 64 * q;
+    y[q] = (f < d) ? (xv[q] - mu) * rs * sc[q] + bias[f] : (T)0;
+  }
+}
+
 template <typename T>
 // scratch (may be NULL): device memory the head may use for per-determinant results, at least
 // n_cfg * K * (nt + 2) * 16 bytes for the value-only and spin-flip passes' row-per-lane kernels.

@@ -20,6 +20,9 @@ struct Leaf {
 struct LayerPack {
   int64_t ln0_s, ln0_b, qkvT, qkv_b, oT, o_b, ln1_s, ln1_b, w0T, b0, w1T, b1;
   int64_t qkv, o, w0, w1;  // non-transposed (in x out) copies for the backward pass
+  // fused value-only forward: the transposed weights again as bf16 hi/lo MFMA operand fragments
+  // (fragment order, see fused.hip), when Model::fused
+  int64_t fz_qkv, fz_o, fz_w0, fz_w1;
 };
 
 struct Model {
@@ -30,6 +33,8 @@ struct Model {
   std::vector<Leaf> leaves;
   // packed layout: backbone part in T elements, then a float64 part (byte offset dbl_off)
   int64_t embT, emb_b, emb, lnf_s, lnf_b, bfT, bf_b, bfW;
+  bool fused;              // the fused value-only forward takes this architecture
+  int64_t fz_emb, fz_bf;   // its embedding (K padded to 64) and backflow operand fragments
   std::vector<LayerPack> layers;
   int64_t n_backbone;  // elements of T
   int64_t dbl_off;     // bytes
@@ -54,6 +59,16 @@ DetParams det_params(const Model& md, const void* packed);
 template <typename T>
 int run_forward(const Model& md, const void* packed, const JetGeom& jg, const double* R, const int32_t* S,
                 int64_t w0, const FwdBufs<T>& b, hipStream_t st);
+
+// The fused value-only forward (fused.hip): whether it takes this architecture (float32 bf16x3
+// backbone with N = 16, d = 128, 8 heads, d_ff = 256) and this jet (one channel;
+// NQS_LAYERWISE_FORWARD=1 in the environment turns it off), the packing of its weight fragments,
+// and the launch from the features X0 (rows x kpad) to the backflow activations out (rows x 2 K N).
+bool fused_model_ok(const Model& md);
+bool fused_value_ok(const Model& md, const JetGeom& jg);
+int pack_fused_weights(const Model& md, float* packed, hipStream_t st);
+int launch_fused_value(const Model& md, const float* packed, const float* X0, int64_t rows, float* out,
+                       hipStream_t st);
 
 // bytes of FwdBufs for n_cfg configurations with C channels and nt tangents
 int64_t fwd_bytes(const Model& md, int64_t n_cfg, int C, int nt);

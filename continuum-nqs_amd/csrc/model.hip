@@ -144,6 +144,18 @@ int build_model(const nqs_model_desc* m, Model& md) {
   md.bfT = take(2LL * md.KN * d);
   md.bf_b = take(2LL * md.KN);
   md.bfW = take((int64_t)d * 2 * md.KN);
+  md.fused = fused_model_ok(md);
+  if (md.fused) {
+    md.fz_emb = take((int64_t)d * 64);
+    for (int l = 0; l < md.depth; ++l) {
+      LayerPack& L = md.layers[l];
+      L.fz_qkv = take(3LL * d * d);
+      L.fz_o = take((int64_t)d * d);
+      L.fz_w0 = take((int64_t)ffd * d);
+      L.fz_w1 = take((int64_t)d * ffd);
+    }
+    md.fz_bf = take(2LL * md.KN * d);
+  }
   md.n_backbone = o;
   md.tsize = (m->precision == NQS_FP64) ? 8 : 4;
   md.dbl_off = round_up(o * md.tsize, 256);
@@ -255,6 +267,7 @@ int pack_params(const Model& md, const double* theta, const double* gs, const do
     NQS_TRY(pack_backbone<double>(md, theta, (double*)packed, st));
   else
     NQS_TRY(pack_backbone<float>(md, theta, (float*)packed, st));
+  if (md.fused) NQS_TRY(pack_fused_weights(md, (float*)packed, st));
   double* dp = (double*)((char*)packed + md.dbl_off);
   NQS_CHECK_HIP(hipMemcpyAsync(dp + md.gs_off, gs, sizeof(double) * 2 * md.n_g, hipMemcpyDeviceToDevice, st));
   const Leaf* dw = find_leaf(md, "params/dW");
@@ -373,6 +386,16 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
   const int64_t rows = jg.n_cfg * jg.C * md.N;
   const int64_t per = (int64_t)jg.C * md.N, blen = md.N;
   NQS_TRY(launch_features<T>(jg, R, S, md.Ainv, md.F, md.kpad, w0, b.X0, st));
+  if constexpr (sizeof(T) == 4) {
+    // value-only jets of the headline architecture: the whole backbone in one launch
+    if (fused_value_ok(md, jg)) {
+      NQS_TRY(launch_fused_value(md, pk, b.X0, rows, b.big, st));
+      const DetParams dp = det_params(md, packed);
+      const int64_t watt = std::max<int64_t>(md.d, (int64_t)md.ff * md.d);
+      return launch_det_jet<T>(jg, dp, R, S, w0, b.big, b.lp, b.tan, b.lap, b.att, (size_t)(rows * watt * sizeof(T)),
+                               st);
+    }
+  }
   NQS_TRY(gemm_nt<T>(gargs<T>(md, b.X0, pk + md.embT, b.x, rows, d, md.kpad, md.kpad, md.kpad, d, pk + md.emb_b, per,
                               blen, (T)0),
                      1, st));

