@@ -67,10 +67,14 @@ struct FusedFwd {
 
 // acc = A W[32 nb .. 32 nb + 32)^T over the first NCH of the nkc 16-wide k chunks: A is the 32-row
 // bf16 hi/lo operand in LDS (pitch lda), W the layer's fragments.  The same products in the same
-// order as k_gemm_nt_bf (lo*hi, hi*lo, hi*hi per chunk; zero chunks past K included).
+// order as k_gemm_nt_bf (lo*hi, hi*lo, hi*hi per chunk; zero chunks past K included).  The
+// fragments of up to eight chunks are loaded before the first product (the scheduling barrier keeps
+// the compiler from sinking the loads next to their uses), so a block waits for L2 once per eight
+// chunks instead of once per chunk.
 template <int NCH>
 __device__ __forceinline__ f16v fu_block(const __bf16* Ah, const __bf16* Al, int lda, const u4v* __restrict__ frag,
                                          int nkc, int nb, int lane) {
+  constexpr int H = NCH > 8 ? 8 : NCH;
   f16v acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
@@ -79,16 +83,73 @@ __device__ __forceinline__ f16v fu_block(const __bf16* Ah, const __bf16* Al, int
   const __bf16* ah_p = Ah + (lane & 31) * lda + ko;
   const __bf16* al_p = Al + (lane & 31) * lda + ko;
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    const bf8v bh = __builtin_bit_cast(bf8v, fb[c * 128]);
-    const bf8v bl = __builtin_bit_cast(bf8v, fb[c * 128 + 64]);
-    const bf8v ah = *reinterpret_cast<const bf8v*>(ah_p + 16 * c);
-    const bf8v al = *reinterpret_cast<const bf8v*>(al_p + 16 * c);
+  for (int c0 = 0; c0 < NCH; c0 += H) {
+    u4v bq[2 * H];
+#pragma unroll
+    for (int c = 0; c < H; ++c) {
+      bq[2 * c] = fb[(c0 + c) * 128];
+      bq[2 * c + 1] = fb[(c0 + c) * 128 + 64];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < H; ++c) {
+      const bf8v bh = __builtin_bit_cast(bf8v, bq[2 * c]);
+      const bf8v bl = __builtin_bit_cast(bf8v, bq[2 * c + 1]);
+      const bf8v ah = *reinterpret_cast<const bf8v*>(ah_p + 16 * (c0 + c));
+      const bf8v al = *reinterpret_cast<const bf8v*>(al_p + 16 * (c0 + c));
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+    }
+  }
+  return acc;
+}
+
+// The weight stream of the dense layers: every unit a wave consumes is eight 16-wide k chunks of
+// one 32-column block (16 fragment registers).  fu_load8 issues a unit's loads, fu_mma8 runs its
+// products; the kernel issues the next unit's loads as soon as the current unit's products are
+// issued, so the L2 latency of the weights overlaps the epilogue and whatever LayerNorm, attention
+// or barrier comes before the next dense block.
+__device__ __forceinline__ void fu_load8(u4v (&b)[16], const u4v* __restrict__ frag, int nkc, int nb, int c0,
+                                         int lane) {
+  const u4v* fb = frag + ((int64_t)nb * nkc + c0) * 128 + lane;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    b[2 * c] = fb[c * 128];
+    b[2 * c + 1] = fb[c * 128 + 64];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// (the LDS operand reads run one chunk ahead of their products; the scheduling barrier per chunk
+// keeps the compiler from hoisting all of them, which would cost 64 more registers)
+__device__ __forceinline__ void fu_mma8(f16v& acc, const u4v (&b)[16], const __bf16* Ah, const __bf16* Al, int lda,
+                                        int c0, int lane) {
+  const int ko = 8 * (lane >> 5);
+  const __bf16* ah_p = Ah + (lane & 31) * lda + ko + 16 * c0;
+  const __bf16* al_p = Al + (lane & 31) * lda + ko + 16 * c0;
+  bf8v ahn = *reinterpret_cast<const bf8v*>(ah_p), aln = *reinterpret_cast<const bf8v*>(al_p);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const bf8v ah = ahn, al = aln;
+    if (c < 7) {
+      ahn = *reinterpret_cast<const bf8v*>(ah_p + 16 * (c + 1));
+      aln = *reinterpret_cast<const bf8v*>(al_p + 16 * (c + 1));
+    }
+    const bf8v bh = __builtin_bit_cast(bf8v, b[2 * c]);
+    const bf8v bl = __builtin_bit_cast(bf8v, b[2 * c + 1]);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  return acc;
+}
+
+__device__ __forceinline__ f16v fu_zero() {
+  f16v z;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) z[r] = 0.f;
+  return z;
 }
 
 // Packer: src (N x K floats, row stride ld) -> fragments with nkc = Kp / 16 chunks (k >= K zero).
@@ -181,14 +242,24 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
     for (int r = 0; r < 16; ++r) xr[r] = acc[r] + bv;
   }
   const float sc_att = 1.f / sqrtf((float)FU_HD);
+  u4v bq[16];  // the weight unit in flight: loaded as soon as the previous unit's products are issued
+  if (a.depth > 0)
+    fu_load8(bq, a.L[0].qkv, 8, wave, 0, lane);
+  else
+    fu_load8(bq, a.bf, 8, wave, 0, lane);
   for (int l = 0; l < a.depth; ++l) {
     const FusedLayer& L = a.L[l];
     fu_layernorm(xr, wave, lane, Q, L.ln0_s, L.ln0_b, Hh, Hl);
     // q, k, v: wave w takes column blocks w, w + 4, w + 8 of the 384
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
+      f16v acc = fu_zero();
+      fu_mma8(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+      if (j < 2)
+        fu_load8(bq, L.qkv, 8, wave + 4 * (j + 1), 0, lane);
+      else
+        fu_load8(bq, L.o, 8, wave, 0, lane);
       const int col0 = 32 * (wave + 4 * j);
-      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.qkv, 8, wave + 4 * j, lane);
       const float bv = L.qkv_b[col0 + (lane & 31)];
 #pragma unroll
       for (int r = 0; r < 16; ++r) Q[fu_row(lane, r) * FU_QLD + col0 + (lane & 31)] = acc[r] + bv;
@@ -211,7 +282,9 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
     }
     __syncthreads();
     {
-      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.o, 8, wave, lane);
+      f16v acc = fu_zero();
+      fu_mma8(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+      fu_load8(bq, L.w0, 8, wave, 0, lane);
       const float bv = L.o_b[cw];
 #pragma unroll
       for (int r = 0; r < 16; ++r) xr[r] = acc[r] + (xr[r] + bv);
@@ -220,8 +293,13 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
     // Dense_0 + GELU: wave w takes column blocks w and w + 4 of the 256 hidden units
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
+      f16v acc = fu_zero();
+      fu_mma8(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+      if (j == 0)
+        fu_load8(bq, L.w0, 8, wave + 4, 0, lane);
+      else
+        fu_load8(bq, L.w1, 16, wave, 0, lane);
       const int col0 = 32 * (wave + 4 * j);
-      const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, L.w0, 8, wave + 4 * j, lane);
       const float bv = L.b0[col0 + (lane & 31)];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -233,17 +311,28 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
     }
     __syncthreads();
     {
-      const f16v acc = fu_block<16>(Gh, Gl, FU_GLD, L.w1, 16, wave, lane);
+      // Dense_1, K = 256 as two units
+      f16v acc = fu_zero();
+      fu_mma8(acc, bq, Gh, Gl, FU_GLD, 0, lane);
+      fu_load8(bq, L.w1, 16, wave, 8, lane);
+      fu_mma8(acc, bq, Gh, Gl, FU_GLD, 8, lane);
+      if (l + 1 < a.depth)
+        fu_load8(bq, a.L[l + 1].qkv, 8, wave, 0, lane);
+      else
+        fu_load8(bq, a.bf, 8, wave, 0, lane);
       const float bv = L.b1[cw];
 #pragma unroll
       for (int r = 0; r < 16; ++r) xr[r] = acc[r] + (xr[r] + bv);
     }
   }
   fu_layernorm(xr, wave, lane, Q, a.lnf_s, a.lnf_b, Hh, Hl);
-  // backflow: wave w takes column blocks w, w + 4 of the nbf outputs
-  for (int j = wave; j < a.nbf / 32; j += 4) {
-    const int col0 = 32 * j;
-    const f16v acc = fu_block<8>(Hh, Hl, FU_HLD, a.bf, 8, j, lane);
+  // backflow: wave w takes column block w, and w + 4 when there are eight
+  const int nu = a.nbf / 128;
+  for (int j = 0; j < nu; ++j) {
+    f16v acc = fu_zero();
+    fu_mma8(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+    if (j + 1 < nu) fu_load8(bq, a.bf, 8, wave + 4, 0, lane);
+    const int col0 = 32 * (wave + 4 * j);
     const float bv = a.bf_b[col0 + (lane & 31)];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -256,7 +345,7 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
 bool fused_model_ok(const Model& md) {
   if (md.split != 3 || md.N != FU_N || md.d != FU_D || md.nh != FU_NH || md.ff * md.d != FU_FF) return false;
   if (md.depth > FU_MAXL || md.kpad > 64 || md.kpad % 8 != 0) return false;
-  return (2 * md.KN) % 32 == 0 && 2 * md.KN <= 256;
+  return 2 * md.KN == 128 || 2 * md.KN == 256;  // one or two backflow units per wave
 }
 
 bool fused_value_ok(const Model& md, const JetGeom& jg) {

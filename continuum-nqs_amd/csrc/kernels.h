@@ -41,12 +41,26 @@ __device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& 
 // tanh-GELU (jax.nn.gelu, approximate=True) and its first two derivatives.  No FMA contraction:
 // every caller gets the same roundings whichever of the outputs it uses (the fused forward and the
 // layer-by-layer kernels agree bit for bit).
+// tanh for the GELU jets: float64 uses the library tanh; float32 uses 1 - 2 / (1 + e^{2u}) with the
+// hardware exp2 and reciprocal (absolute error a few 1e-7, below the bf16x3 products' rounding;
+// +-1 exactly once e^{2u} over- or underflows).
+template <typename T>
+__device__ inline T gelu_tanh(T u) {
+  return tanh(u);
+}
+template <>
+__device__ inline float gelu_tanh<float>(float u) {
+#pragma clang fp contract(off)
+  const float e = __builtin_amdgcn_exp2f(u * 2.8853900817779268f);  // 2 log2(e)
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+}
+
 template <typename T>
 __device__ inline void gelu_derivs(T x, T& g, T& g1, T& g2) {
 #pragma clang fp contract(off)
   const T k0 = (T)0.7978845608028654, k1 = (T)0.044715;
   const T u = k0 * (x + k1 * x * x * x);
-  const T th = tanh(u);
+  const T th = gelu_tanh<T>(u);
   const T sech2 = (T)1 - th * th;
   const T du = k0 * ((T)1 + (T)3 * k1 * x * x);
   const T ddu = (T)6 * k0 * k1 * x;

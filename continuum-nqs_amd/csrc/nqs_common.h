@@ -81,9 +81,34 @@ __device__ inline cd cexp_(cd a) {
 __device__ inline cd clog_(cd a) { return cd{log(hypot(a.re, a.im)), atan2(a.im, a.re)}; }
 
 // ---- wave-level reductions (64 lanes)
+// One DPP move of a 32- or 64-bit value inside each 16-lane row (dpp_ctrl CTRL: quad_perm,
+// row_mirror, row_half_mirror, ...), on the VALU without LDS traffic.
+template <int CTRL, typename T>
+__device__ __forceinline__ T row_dpp(T x) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+  } else {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
+  }
+}
+
+// Sum over the 64 lanes, the same bits in every lane: pairwise within quads (quad_perm [1,0,3,2]
+// and [2,3,0,1]), quad with quad (row_half_mirror: lane i with 7 - i), half-row with half-row
+// (row_mirror: i with 15 - i) as DPP, then row with row by two shuffles (lane i with i ^ 16, then
+// i ^ 32).  Every step adds the values of two lanes that pair with each other, so both get the
+// same sum.
 template <typename T>
 __device__ inline T wave_sum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += row_dpp<0xB1>(v);
+  v += row_dpp<0x4E>(v);
+  v += row_dpp<0x141>(v);
+  v += row_dpp<0x140>(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
   return v;
 }
 
