@@ -82,7 +82,10 @@ def grad_params(logpsi, R, S, out=None, ws_tag="default"):
     P = m.n_params()
     dt = torch.float64 if m.precision == "fp64" else torch.float32
     if out is None or out.data.shape != (2 * W, _ldo(P)) or out.data.dtype != dt:
-        out = OMatrix(torch.zeros(2 * W, _ldo(P), dtype=dt, device=R.device), W, P)
+        # The backward writes every column below P; only the pad columns need clearing.
+        buf = torch.empty(2 * W, _ldo(P), dtype=dt, device=R.device)
+        buf[:, P:].zero_()
+        out = OMatrix(buf, W, P)
     lp = torch.empty(W, 2, dtype=torch.float64, device=R.device)
     nb = lib.nqs_workspace_bytes(ctypes.byref(d), _lib.OP_GRAD, W)
     ws = _lib.workspace(nb, ws_tag)
