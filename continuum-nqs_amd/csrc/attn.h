@@ -117,4 +117,38 @@ __device__ __forceinline__ void attn_value(const AttnRegs<T, NQ>& x0, int lane, 
   }
 }
 
+// A first-order tangent channel of the same (configuration, head): with the value channel's
+// registers x0 and softmax weights P, and the tangent's q, k, v in xc,
+// dS^T = (K dQ^T + dK Q^T) sc, dP = P (dS - sum_b P dS), dO = dP V + P dV (same layout as
+// attn_value's O).  No FMA contraction.
+template <typename T, int NQ>
+__device__ __forceinline__ void attn_tangent(const AttnRegs<T, NQ>& x0, const AttnRegs<T, NQ>& xc, const T (&P)[4],
+                                             T sc, typename Mma16<T>::acc_t (&o)[NQ]) {
+#pragma clang fp contract(off)
+  typedef typename Mma16<T>::acc_t acc_t;
+  const acc_t zacc = (acc_t)(T)0;
+  acc_t sa = zacc;
+#pragma unroll
+  for (int s = 0; s < 4 * NQ; ++s) {
+    sa = Mma16<T>::mma(x0.k[s], xc.q[s], sa);
+    sa = Mma16<T>::mma(xc.k[s], x0.q[s], sa);
+  }
+  T m = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) m += P[r] * (sa[r] * sc);
+  m = xsum16(m);
+  T Pc[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Pc[r] = P[r] * (sa[r] * sc - m);
+#pragma unroll
+  for (int t = 0; t < NQ; ++t) {
+    o[t] = zacc;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      o[t] = Mma16<T>::mma(Pc[kb], x0.v[4 * t + kb], o[t]);
+      o[t] = Mma16<T>::mma(P[kb], xc.v[4 * t + kb], o[t]);
+    }
+  }
+}
+
 }  // namespace nqs

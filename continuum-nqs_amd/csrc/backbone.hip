@@ -102,6 +102,8 @@ template <typename T, int MAXF>
 __global__ __launch_bounds__(64) void k_layernorm_jet(JetGeom j, int d, const T* __restrict__ x,
                                                       const T* __restrict__ scale,
                                                       const T* __restrict__ bias, T* __restrict__ y) {
+  // no FMA contraction: a first-order tangent row rounds as ln_tangent does in the fused forward
+#pragma clang fp contract(off)
   const int64_t row = blockIdx.x;  // cfg * N + t
   const int lane = threadIdx.x;
   const int64_t cfg = row / j.N;
@@ -419,6 +421,8 @@ __device__ __forceinline__ void attn_store(T* __restrict__ o, int lane, int N, i
 template <typename T, int NQ>
 __global__ __launch_bounds__(512) void k_attention_jet_mfma(JetGeom j, int d, int nh, const T* __restrict__ qkv,
                                                             T* __restrict__ out, int64_t npairs) {
+  // no FMA contraction: a first-order tangent channel rounds as attn_tangent does in the fused forward
+#pragma clang fp contract(off)
   typedef typename Mma16<T>::acc_t acc_t;
   const int lane = threadIdx.x & 63;
   const int64_t pair = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);

@@ -342,6 +342,288 @@ __global__ __launch_bounds__(256, 2) void k_fused_value(FusedFwd a) {
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused first-order jet forward: the spin-flip pass (src/nqs/local_energy.py:63-115 through
+// psiformer.py:67-81), whose configurations carry three channels, the value and the two position
+// tangents of the flipped electron (48 token rows, channel c, token t at row 16 c + t).
+//
+// One workgroup of 4 waves per configuration.  The dense layers run on the value-only kernel's
+// v_mfma_f32_32x32x16_bf16 blocks and weight fragments, on two overlapping 32-row blocks: A = rows
+// 0..31 (value, first tangent) and B = rows 16..47 (first tangent, second tangent).  The first
+// tangent is computed twice from the same operands, so its two copies stay equal; every dense
+// product is the layer-by-layer GEMM's (k_gemm_nt_bf: 16-wide k chunks, lo*hi, hi*lo, hi*hi).  In
+// a block, accumulator slot r < 8 of a lane and slot r + 8 are the same token and column 16 rows
+// apart, so the GELU jet of a tangent row finds its value row in the same lane.  Wave w owns
+// columns 32 w .. 32 w + 31 of the residual stream (both blocks, 32 registers).  Jet rules: LayerNorm
+// ln_value / ln_tangent, attention attn_value / attn_tangent, GELU as in the first-order Dense_1
+// prologue of the layer-by-layer path (value rows gelu(u), tangent rows gelu'(u of the token's value
+// row) du); biases enter the value rows only.
+//
+// LDS (76.8 KB, two workgroups per CU): the K = 128 operand planes (LayerNorm outputs), and one
+// region that is in turn the LayerNorm scratch, [attention output planes | q, k, v of two heads],
+// and the K = 256 GELU output planes.  q, k and v are formed two heads at a time (four quarters);
+// in each quarter wave w runs head 2 p + w / 2 for tangent channel 1 + w % 2, both waves of a head
+// forming its value channel.
+constexpr int FJ_C = 3, FJ_R = FJ_C * FU_N;
+constexpr int FJ_QLD = 3 * 32 + 4;  // float pitch of q | k | v of two heads
+constexpr int FJ_HBYTES = 2 * FJ_R * FU_HLD * 2;
+constexpr int FJ_GBYTES = 2 * FJ_R * FU_GLD * 2;
+constexpr int FJ_LDS = FJ_HBYTES + FJ_GBYTES;
+static_assert(FJ_HBYTES + FJ_R * FJ_QLD * 4 <= FJ_GBYTES && FJ_R * FU_D * 4 <= FJ_GBYTES, "LDS region sizes");
+
+// the weight unit in flight: four 16-wide k chunks (from c0) of one 32-column block
+__device__ __forceinline__ void fj_load4(u4v (&b)[8], const u4v* __restrict__ frag, int nkc, int nb, int c0,
+                                         int lane) {
+  const u4v* fb = frag + ((int64_t)nb * nkc + c0) * 128 + lane;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    b[2 * c] = fb[c * 128];
+    b[2 * c + 1] = fb[c * 128 + 64];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// acc[q] += the 32-row block starting at row 16 q of the operand planes (Ah, Al) times the unit,
+// over the planes' chunks c0 .. c0 + 3
+template <int NB>
+__device__ __forceinline__ void fj_mma4(f16v* acc, const u4v (&b)[8], const __bf16* Ah, const __bf16* Al, int lda,
+                                        int c0, int lane) {
+  const int ko = 8 * (lane >> 5) + 16 * c0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const bf8v bh = __builtin_bit_cast(bf8v, b[2 * c]);
+    const bf8v bl = __builtin_bit_cast(bf8v, b[2 * c + 1]);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int o = (16 * q + (lane & 31)) * lda + ko + 16 * c;
+      const bf8v ah = *reinterpret_cast<const bf8v*>(Ah + o);
+      const bf8v al = *reinterpret_cast<const bf8v*>(Al + o);
+      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[q], 0, 0, 0);
+      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[q], 0, 0, 0);
+      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[q], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// LayerNorm jet of the residual stream (wave w's blocks xr) into the K = 128 operand planes: the
+// blocks go through the float scratch Xs; wave w then normalises tokens 4 w .. 4 w + 3, each as the
+// value row and its two tangent rows (lane holds features lane and lane + 64).
+__device__ __forceinline__ void fj_layernorm(const f16v (&xr)[2], int wave, int lane, float* Xs,
+                                             const float* __restrict__ s, const float* __restrict__ b, __bf16* Hh,
+                                             __bf16* Hl) {
+  __syncthreads();  // every wave is done with the region Xs aliases
+  const int cw = 32 * wave + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) Xs[fu_row(lane, r) * FU_D + cw] = xr[0][r];
+#pragma unroll
+  for (int r = 8; r < 16; ++r) Xs[(16 + fu_row(lane, r)) * FU_D + cw] = xr[1][r];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int t = 4 * wave + i;
+    float xv[2] = {Xs[t * FU_D + lane], Xs[t * FU_D + lane + 64]};
+    float sc[2], y[2], mu, rs;
+    ln_value<float, 2>(xv, lane, FU_D, s, b, sc, y, mu, rs);
+    fu_put(Hh, Hl, t * FU_HLD + lane, y[0]);
+    fu_put(Hh, Hl, t * FU_HLD + lane + 64, y[1]);
+#pragma unroll
+    for (int c = 1; c < FJ_C; ++c) {
+      const int row = FU_N * c + t;
+      const float dx[2] = {Xs[row * FU_D + lane], Xs[row * FU_D + lane + 64]};
+      float dy[2];
+      ln_tangent<float, 2>(xv, dx, sc, FU_D, mu, rs, dy);
+      fu_put(Hh, Hl, row * FU_HLD + lane, dy[0]);
+      fu_put(Hh, Hl, row * FU_HLD + lane + 64, dy[1]);
+    }
+  }
+  __syncthreads();
+}
+
+// residual epilogue of blocks A and B: x = acc + (x + bias) on the value rows (block A slots < 8),
+// acc + x elsewhere, as the layer-by-layer GEMM's epilogue with the bias on value rows only
+__device__ __forceinline__ void fj_residual(f16v (&xr)[2], const f16v (&acc)[2], float bv) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    xr[0][r] = acc[0][r] + (xr[0][r] + (r < 8 ? bv : 0.f));
+    xr[1][r] = acc[1][r] + xr[1][r];
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_fused_jet1(FusedFwd a) {
+  extern __shared__ __attribute__((aligned(16))) char fsm[];
+  __bf16* Hh = reinterpret_cast<__bf16*>(fsm);
+  __bf16* Hl = Hh + FJ_R * FU_HLD;
+  char* rg = fsm + FJ_HBYTES;
+  float* Xs = reinterpret_cast<float*>(rg);     // LayerNorm scratch
+  __bf16* Oh = reinterpret_cast<__bf16*>(rg);   // attention output planes
+  __bf16* Ol = Oh + FJ_R * FU_HLD;
+  float* Qq = reinterpret_cast<float*>(rg + FJ_HBYTES);  // q | k | v of two heads
+  __bf16* Gh = reinterpret_cast<__bf16*>(rg);   // GELU output planes
+  __bf16* Gl = Gh + FJ_R * FU_GLD;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * FJ_R;
+  const int cl = lane & 31;  // this lane's column within a 32-column block
+  const f16v z16 = fu_zero();
+
+  // ---- features -> operand planes (k >= kpad zero up to 64)
+  for (int it = tid; it < FJ_R * 8; it += 256) {
+    const int r = it >> 3, k0 = 8 * (it & 7);
+    f4v v0 = (f4v)0.f, v1 = (f4v)0.f;
+    if (k0 < a.kpad) {
+      v0 = *reinterpret_cast<const f4v*>(a.X0 + (row0 + r) * a.kpad + k0);
+      v1 = *reinterpret_cast<const f4v*>(a.X0 + (row0 + r) * a.kpad + k0 + 4);
+    }
+    uint32_t h_[4], l_[4];
+    split_bf16x2(v0[0], v0[1], h_[0], l_[0]);
+    split_bf16x2(v0[2], v0[3], h_[1], l_[1]);
+    split_bf16x2(v1[0], v1[1], h_[2], l_[2]);
+    split_bf16x2(v1[2], v1[3], h_[3], l_[3]);
+    *reinterpret_cast<u4v*>(Hh + r * FU_HLD + k0) = (u4v){h_[0], h_[1], h_[2], h_[3]};
+    *reinterpret_cast<u4v*>(Hl + r * FU_HLD + k0) = (u4v){l_[0], l_[1], l_[2], l_[3]};
+  }
+  __syncthreads();
+  u4v bq[8];  // the weight unit in flight
+  f16v xr[2];
+  {
+    // embedding: K = kpad <= 64, four chunks (zero past kpad)
+    fj_load4(bq, a.emb, 4, wave, 0, lane);
+    f16v acc[2] = {z16, z16};
+    fj_mma4<2>(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+    const float bv = a.emb_b[32 * wave + cl];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      xr[0][r] = acc[0][r] + (r < 8 ? bv : 0.f);
+      xr[1][r] = acc[1][r];
+    }
+  }
+  const float sc_att = 1.f / sqrtf((float)FU_HD);
+  // q, k, v of a quarter: six items (column block u = q, k, v of heads 2 p, 2 p + 1; block A or B),
+  // wave w takes items w and w + 4
+  auto qkv_nb = [](int p, int item) { return 4 * (item >> 1) + p; };
+  if (a.depth > 0)
+    fj_load4(bq, a.L[0].qkv, 8, qkv_nb(0, wave), 0, lane);
+  else
+    fj_load4(bq, a.bf, 8, wave, 0, lane);
+  for (int l = 0; l < a.depth; ++l) {
+    const FusedLayer& L = a.L[l];
+    fj_layernorm(xr, wave, lane, Xs, L.ln0_s, L.ln0_b, Hh, Hl);
+    for (int p = 0; p < 4; ++p) {
+      if (p > 0) __syncthreads();  // the previous quarter's attention is done with Qq
+      for (int it = wave; it < 6; it += 4) {
+        const int nb = qkv_nb(p, it), blk = it & 1;
+        f16v acc[1] = {z16};
+        fj_mma4<1>(acc, bq, Hh + 16 * blk * FU_HLD, Hl + 16 * blk * FU_HLD, FU_HLD, 0, lane);
+        fj_load4(bq, L.qkv, 8, nb, 4, lane);
+        fj_mma4<1>(acc, bq, Hh + 16 * blk * FU_HLD, Hl + 16 * blk * FU_HLD, FU_HLD, 4, lane);
+        // next unit: this quarter's next item, the next quarter's first, or the output projection
+        if (it + 4 < 6)
+          fj_load4(bq, L.qkv, 8, qkv_nb(p, it + 4), 0, lane);
+        else if (p < 3)
+          fj_load4(bq, L.qkv, 8, qkv_nb(p + 1, wave), 0, lane);
+        else
+          fj_load4(bq, L.o, 8, wave, 0, lane);
+        const int qc = 32 * (it >> 1) + cl;
+        const float bv = L.qkv_b[32 * nb + cl];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          if (blk == 1 && r < 8) continue;  // block B's first tangent rows: block A has them
+          const float v = acc[0][r] + ((blk == 0 && r < 8) ? bv : 0.f);
+          Qq[(16 * blk + fu_row(lane, r)) * FJ_QLD + qc] = v;
+        }
+      }
+      __syncthreads();
+      {
+        const int hl = wave >> 1, ch = 1 + (wave & 1);
+        const int oc = 16 * (2 * p + hl) + (lane & 15);
+        AttnRegs<float, 1> x0, xc;
+        attn_load<float, 1>(Qq, lane, FU_N, FU_HD, 32, hl, FJ_QLD, x0);
+        float P[4];
+        f4v o[1];
+        attn_value<float, 1>(x0, lane, FU_N, sc_att, P, o);
+        if ((wave & 1) == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) fu_put(Oh, Ol, Mma16<float>::row(lane, r) * FU_HLD + oc, o[0][r]);
+        }
+        attn_load<float, 1>(Qq + FU_N * ch * FJ_QLD, lane, FU_N, FU_HD, 32, hl, FJ_QLD, xc);
+        attn_tangent<float, 1>(x0, xc, P, sc_att, o);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) fu_put(Oh, Ol, (FU_N * ch + Mma16<float>::row(lane, r)) * FU_HLD + oc, o[0][r]);
+      }
+    }
+    __syncthreads();
+    {
+      // output projection and residual
+      f16v acc[2] = {z16, z16};
+      fj_mma4<2>(acc, bq, Oh, Ol, FU_HLD, 0, lane);
+      fj_load4(bq, L.o, 8, wave, 4, lane);
+      fj_mma4<2>(acc, bq, Oh, Ol, FU_HLD, 4, lane);
+      fj_load4(bq, L.w0, 8, wave, 0, lane);
+      fj_residual(xr, acc, L.o_b[32 * wave + cl]);
+    }
+    fj_layernorm(xr, wave, lane, Xs, L.ln1_s, L.ln1_b, Hh, Hl);
+    // Dense_0 and the GELU jet: wave w takes hidden column blocks w and w + 4
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int nb = wave + 4 * j;
+      f16v acc[2] = {z16, z16};
+      fj_mma4<2>(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+      fj_load4(bq, L.w0, 8, nb, 4, lane);
+      fj_mma4<2>(acc, bq, Hh, Hl, FU_HLD, 4, lane);
+      if (j == 0)
+        fj_load4(bq, L.w0, 8, wave + 4, 0, lane);
+      else
+        fj_load4(bq, L.w1, 16, wave, 0, lane);
+      const int gc = 32 * nb + cl;
+      const float bv = L.b0[gc];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int row = fu_row(lane, r);
+        const float u = acc[0][r] + bv;
+        float g0, g1, g2;
+        gelu_derivs<float>(u, g0, g1, g2);
+        fu_put(Gh, Gl, row * FU_GLD + gc, g0);
+        fu_put(Gh, Gl, (FU_N + row) * FU_GLD + gc, g1 * acc[0][r + 8]);
+        fu_put(Gh, Gl, (2 * FU_N + row) * FU_GLD + gc, g1 * acc[1][r + 8]);
+      }
+    }
+    __syncthreads();
+    {
+      // Dense_1 (K = 256: four units) and residual
+      f16v acc[2] = {z16, z16};
+#pragma unroll
+      for (int c0 = 0; c0 < 16; c0 += 4) {
+        fj_mma4<2>(acc, bq, Gh, Gl, FU_GLD, c0, lane);
+        if (c0 < 12)
+          fj_load4(bq, L.w1, 16, wave, c0 + 4, lane);
+        else if (l + 1 < a.depth)
+          fj_load4(bq, a.L[l + 1].qkv, 8, qkv_nb(0, wave), 0, lane);
+        else
+          fj_load4(bq, a.bf, 8, wave, 0, lane);
+      }
+      fj_residual(xr, acc, L.b1[32 * wave + cl]);
+    }
+  }
+  fj_layernorm(xr, wave, lane, Xs, a.lnf_s, a.lnf_b, Hh, Hl);
+  // backflow: column blocks wave, wave + 4, ... of the nbf outputs
+  const int ncb = a.nbf / 32;
+  for (int nb = wave; nb < ncb; nb += 4) {
+    f16v acc[2] = {z16, z16};
+    fj_mma4<2>(acc, bq, Hh, Hl, FU_HLD, 0, lane);
+    fj_load4(bq, a.bf, 8, nb, 4, lane);
+    fj_mma4<2>(acc, bq, Hh, Hl, FU_HLD, 4, lane);
+    if (nb + 4 < ncb) fj_load4(bq, a.bf, 8, nb + 4, 0, lane);
+    const float bv = a.bf_b[32 * nb + cl];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t gr = row0 + fu_row(lane, r);
+      a.out[gr * a.nbf + 32 * nb + cl] = acc[0][r] + (r < 8 ? bv : 0.f);
+    }
+#pragma unroll
+    for (int r = 8; r < 16; ++r) a.out[(row0 + 16 + fu_row(lane, r)) * a.nbf + 32 * nb + cl] = acc[1][r];
+  }
+}
+
 bool fused_model_ok(const Model& md) {
   if (md.split != 3 || md.N != FU_N || md.d != FU_D || md.nh != FU_NH || md.ff * md.d != FU_FF) return false;
   if (md.depth > FU_MAXL || md.kpad > 64 || md.kpad % 8 != 0) return false;
@@ -350,6 +632,12 @@ bool fused_model_ok(const Model& md) {
 
 bool fused_value_ok(const Model& md, const JetGeom& jg) {
   if (!md.fused || md.tsize != 4 || jg.C != 1 || jg.nt != 0 || jg.has_lap) return false;
+  const char* e = getenv("NQS_LAYERWISE_FORWARD");
+  return !(e && e[0] == '1');
+}
+
+bool fused_jet1_ok(const Model& md, const JetGeom& jg) {
+  if (!md.fused || md.tsize != 4 || jg.C != FJ_C || jg.nt != 2 || jg.has_lap) return false;
   const char* e = getenv("NQS_LAYERWISE_FORWARD");
   return !(e && e[0] == '1');
 }
@@ -375,10 +663,7 @@ int pack_fused_weights(const Model& md, float* pk, hipStream_t st) {
   return split(md.bfT, 2 * md.KN, d, d, d, md.fz_bf);
 }
 
-int launch_fused_value(const Model& md, const float* pk, const float* X0, int64_t rows, float* out, hipStream_t st) {
-  if (rows <= 0) return NQS_OK;
-  NQS_REQUIRE(md.fused, NQS_ERR_UNSUPPORTED, "fused forward: architecture not supported");
-  NQS_REQUIRE(rows % FU_N == 0, NQS_ERR_ARG, "fused forward: rows must be whole configurations");
+static FusedFwd fused_args(const Model& md, const float* pk, const float* X0, int64_t rows, float* out) {
   auto fr = [&](int64_t off) { return reinterpret_cast<const u4v*>(pk + off); };
   FusedFwd a;
   a.X0 = X0;
@@ -398,12 +683,35 @@ int launch_fused_value(const Model& md, const float* pk, const float* X0, int64_
   a.bf_b = pk + md.bf_b;
   a.nbf = 2 * md.KN;
   a.out = out;
+  return a;
+}
+
+int launch_fused_value(const Model& md, const float* pk, const float* X0, int64_t rows, float* out, hipStream_t st) {
+  if (rows <= 0) return NQS_OK;
+  NQS_REQUIRE(md.fused, NQS_ERR_UNSUPPORTED, "fused forward: architecture not supported");
+  NQS_REQUIRE(rows % FU_N == 0, NQS_ERR_ARG, "fused forward: rows must be whole configurations");
+  const FusedFwd a = fused_args(md, pk, X0, rows, out);
   static bool attr = false;
   if (!attr) {
     NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_fused_value, hipFuncAttributeMaxDynamicSharedMemorySize, FU_LDS));
     attr = true;
   }
   hipLaunchKernelGGL(k_fused_value, dim3((unsigned)cdiv(rows, FU_R)), dim3(256), FU_LDS, st, a);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
+}
+
+int launch_fused_jet1(const Model& md, const float* pk, const float* X0, int64_t n_cfg, float* out, hipStream_t st) {
+  if (n_cfg <= 0) return NQS_OK;
+  NQS_REQUIRE(md.fused, NQS_ERR_UNSUPPORTED, "fused forward: architecture not supported");
+  NQS_REQUIRE(n_cfg <= 0x7fffffffLL, NQS_ERR_ARG, "fused jet forward: too many configurations per launch");
+  const FusedFwd a = fused_args(md, pk, X0, n_cfg * FJ_R, out);
+  static bool attr = false;
+  if (!attr) {
+    NQS_CHECK_HIP(hipFuncSetAttribute((const void*)k_fused_jet1, hipFuncAttributeMaxDynamicSharedMemorySize, FJ_LDS));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_fused_jet1, dim3((unsigned)n_cfg), dim3(256), FJ_LDS, st, a);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

@@ -30,7 +30,10 @@ struct Mma16<float> {
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 
+// No FMA contraction: x - hi is never fused with a product that computed x, whichever kernel inlines
+// the split, so every caller gets the same lo.
 __device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+#pragma clang fp contract(off)
   const f2v v = {x0, x1};
   hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
   const float h0 = __builtin_bit_cast(float, hi << 16), h1 = __builtin_bit_cast(float, hi & 0xffff0000u);
@@ -160,6 +163,29 @@ __device__ __forceinline__ void ln_value(const T (&xv)[MAXF], int lane, int d, c
     const int f = lane + 64 * q;
     y[q] = (f < d) ? (xv[q] - mu) * rs * sc[q] + bias[f] : (T)0;
   }
+}
+
+// First-order tangent row of the LayerNorm jet, given the value row's xv, mu, rs and scales sc
+// (from ln_value) and the tangent dx (same lane layout): dmu = mean(dx),
+// drs = -rs^3 (mean(x dx) - mu dmu), dy = ((dx - dmu) rs + (x - mu) drs) scale.  No FMA contraction.
+template <typename T, int MAXF>
+__device__ __forceinline__ void ln_tangent(const T (&xv)[MAXF], const T (&dx)[MAXF], const T (&sc)[MAXF], int d, T mu,
+                                           T rs, T (&dy)[MAXF]) {
+#pragma clang fp contract(off)
+  const T inv_d = (T)1 / (T)d;
+  T sd = 0, sxd = 0;
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) {
+    sd += dx[q];
+    sxd += xv[q] * dx[q];
+  }
+  sd = wave_sum(sd);
+  sxd = wave_sum(sxd);
+  const T dmu = sd * inv_d;
+  const T dvar = (T)2 * (sxd * inv_d - mu * dmu);
+  const T drs = (T)(-0.5) * rs * rs * rs * dvar;
+#pragma unroll
+  for (int q = 0; q < MAXF; ++q) dy[q] = ((dx[q] - dmu) * rs + (xv[q] - mu) * drs) * sc[q];
 }
 
 template <typename T>

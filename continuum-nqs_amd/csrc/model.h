@@ -66,6 +66,10 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
 // and the launch from the features X0 (rows x kpad) to the backflow activations out (rows x 2 K N).
 bool fused_model_ok(const Model& md);
 bool fused_value_ok(const Model& md, const JetGeom& jg);
+// the spin-flip pass's jets (value and two tangents per configuration) on the fused kernel
+bool fused_jet1_ok(const Model& md, const JetGeom& jg);
+int launch_fused_jet1(const Model& md, const float* packed, const float* X0, int64_t n_cfg, float* out,
+                      hipStream_t st);
 int pack_fused_weights(const Model& md, float* packed, hipStream_t st);
 int launch_fused_value(const Model& md, const float* packed, const float* X0, int64_t rows, float* out,
                        hipStream_t st);

@@ -388,8 +388,12 @@ int run_forward(const Model& md, const void* packed, const JetGeom& jg, const do
   NQS_TRY(launch_features<T>(jg, R, S, md.Ainv, md.F, md.kpad, w0, b.X0, st));
   if constexpr (sizeof(T) == 4) {
     // value-only jets of the headline architecture: the whole backbone in one launch
-    if (fused_value_ok(md, jg)) {
-      NQS_TRY(launch_fused_value(md, pk, b.X0, rows, b.big, st));
+    const bool fv = fused_value_ok(md, jg), fj = !fv && fused_jet1_ok(md, jg);
+    if (fv || fj) {
+      if (fv)
+        NQS_TRY(launch_fused_value(md, pk, b.X0, rows, b.big, st));
+      else
+        NQS_TRY(launch_fused_jet1(md, pk, b.X0, jg.n_cfg, b.big, st));
       const DetParams dp = det_params(md, packed);
       const int64_t watt = std::max<int64_t>(md.d, (int64_t)md.ff * md.d);
       return launch_det_jet<T>(jg, dp, R, S, w0, b.big, b.lp, b.tan, b.lap, b.att, (size_t)(rows * watt * sizeof(T)),

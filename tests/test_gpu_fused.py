@@ -12,7 +12,11 @@ layer-by-layer kernels for the comparison (psiformer.py:118-189).
     written without FMA contraction, so they round alike in every kernel that inlines them);
   * log psi of 8 walkers within 5e-4 of the float64 oracle (bf16x3 rounding of the dense layers);
   * the Metropolis sweeps over the fused forward make the same accept/reject decisions as over the
-    layer-by-layer forward, given the same supplied streams.
+    layer-by-layer forward, given the same supplied streams;
+  * the spin-flip pass of E_loc (value and the two position tangents of every single-spin-flipped
+    configuration, src/nqs/local_energy.py:63-115) on the fused first-order kernel: E_loc bit for bit
+    the layer-by-layer jets' (the same dense blocks, and the LayerNorm and attention tangent rules
+    written without FMA contraction in both paths).
 """
 import math
 import os
@@ -26,6 +30,7 @@ from oracle import psiformer as opf
 from nqs.rashba_reference import noninteracting_gs, occupied_orbital_matrix, planewave_envelope
 from nqs.psiformer import ContinuumSpinorPsiformer, LogPsi
 from nqs.sampler import Chain
+from nqs.local_energy import make_local_energy
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +119,38 @@ def test_fused_forward_timing(c3m):
     tf = t()
     tl = _layerwise(t)
     print(f"value-only forward, 8192 walkers: fused {tf:.3f} ms, layer by layer {tl:.3f} ms", flush=True)
+
+
+@pytest.mark.parametrize("W", [1024, 7])
+def test_fused_spin_flip_eloc_equals_layerwise(c3m, W):
+    """E_loc with the Rashba term (spin-flip pass fused) against the layer-by-layer jets."""
+    lp = c3m["lp"]
+    R, S = c3m["R"][:W], c3m["S"][:W]
+    E = make_local_energy(lp, 1.0, LAM, HZ)
+    e = E(R, S)
+    e_lw = _layerwise(lambda: E(R, S))
+    d = (e - e_lw).abs()
+    scale = float(e_lw.abs().median())
+    print(f"spin-flip pass fused vs layer-by-layer, {W} walkers: max |dE_loc| {float(d.max()):.3e}, "
+          f"median |dE_loc| {float(d.median()):.3e}, median |E_loc| {scale:.3e}", flush=True)
+    assert bool(torch.isfinite(e).all())
+    assert torch.equal(e, e_lw)
+
+
+def test_fused_spin_flip_timing(c3m):
+    """Informational: E_loc of 4096 walkers with the spin-flip pass fused and layer by layer."""
+    lp = c3m["lp"]
+    E = make_local_energy(lp, 1.0, LAM, HZ)
+    R, S = c3m["R"], c3m["S"]
+
+    def t(n=3):
+        E(R, S)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            E(R, S)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+    tf = t()
+    tl = _layerwise(t)
+    print(f"E_loc, 4096 walkers: spin-flip pass fused {tf:.2f} ms, layer by layer {tl:.2f} ms", flush=True)
