@@ -98,91 +98,123 @@ int launch_features(const JetGeom& j, const double* R, const int32_t* S, const d
 // ------------------------------------------------------------------ LayerNorm jet
 // One wave per (cfg, token).  mu = E[x], var = E[x^2] - mu^2, rs = (var + eps)^-1/2,
 // y = (x - mu) rs scale + bias.
-template <typename T, int MAXF>
-__global__ __launch_bounds__(64) void k_layernorm_jet(JetGeom j, int d, const T* __restrict__ x,
-                                                      const T* __restrict__ scale,
-                                                      const T* __restrict__ bias, T* __restrict__ y) {
-  // no FMA contraction: a first-order tangent row rounds as ln_tangent does in the fused forward
+// F features per lane of a 16-lane row group, loaded / stored as 16-byte vectors when the row is
+// exactly 16 F wide (zero past d otherwise)
+template <typename T, int F>
+__device__ __forceinline__ void ln_load(const T* __restrict__ p, int j, int d, T (&v)[F]) {
+  constexpr int VN = 16 / sizeof(T);
+  if (F % VN == 0 && d == 16 * F) {
+    typedef T vt __attribute__((ext_vector_type(VN)));
+#pragma unroll
+    for (int q = 0; q < F / VN; ++q) {
+      const vt w = *reinterpret_cast<const vt*>(p + F * j + VN * q);
+#pragma unroll
+      for (int e = 0; e < VN; ++e) v[VN * q + e] = w[e];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < F; ++e) v[e] = (F * j + e < d) ? p[F * j + e] : (T)0;
+  }
+}
+template <typename T, int F>
+__device__ __forceinline__ void ln_store(T* __restrict__ p, int j, int d, const T (&v)[F]) {
+  constexpr int VN = 16 / sizeof(T);
+  if (F % VN == 0 && d == 16 * F) {
+    typedef T vt __attribute__((ext_vector_type(VN)));
+#pragma unroll
+    for (int q = 0; q < F / VN; ++q) {
+      vt w;
+#pragma unroll
+      for (int e = 0; e < VN; ++e) w[e] = v[VN * q + e];
+      *reinterpret_cast<vt*>(p + F * j + VN * q) = w;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < F; ++e)
+      if (F * j + e < d) p[F * j + e] = v[e];
+  }
+}
+
+// LayerNorm jet: four token rows per 64-lane block, one 16-lane row group per token (ln_value's
+// layout: lane j holds features F j .. F j + F - 1), every channel of the token in turn.  First-order
+// tangents use ln_tangent; with a Laplacian channel the same tangent rule is written out with the
+// second-order accumulators beside it.
+template <typename T, int F>
+__global__ __launch_bounds__(64) void k_layernorm_jet(JetGeom jg, int d, const T* __restrict__ x,
+                                                      const T* __restrict__ scale, const T* __restrict__ bias,
+                                                      T* __restrict__ y, int64_t rows) {
 #pragma clang fp contract(off)
-  const int64_t row = blockIdx.x;  // cfg * N + t
-  const int lane = threadIdx.x;
-  const int64_t cfg = row / j.N;
-  const int t = (int)(row % j.N);
-  const int64_t chs = (int64_t)j.N * d;
-  const T* xb = x + cfg * j.C * chs + (int64_t)t * d;
-  T* yb = y + cfg * j.C * chs + (int64_t)t * d;
+  const int lane = threadIdx.x, j = lane & 15;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (lane >> 4);  // cfg * N + t
+  if (row >= rows) return;  // whole 16-lane groups leave together
+  const int64_t cfg = row / jg.N;
+  const int t = (int)(row % jg.N);
+  const int64_t chs = (int64_t)jg.N * d;
+  const T* xb = x + cfg * jg.C * chs + (int64_t)t * d;
+  T* yb = y + cfg * jg.C * chs + (int64_t)t * d;
   const T inv_d = (T)1 / (T)d;
-  T xv[MAXF], sc[MAXF], s1[MAXF], y0[MAXF];
-#pragma unroll
-  for (int q = 0; q < MAXF; ++q) {
-    const int f = lane + 64 * q;
-    xv[q] = (f < d) ? xb[f] : (T)0;
-    s1[q] = 0;
-  }
+  T xv[F], sc[F], y0[F];
+  ln_load<T, F>(xb, j, d, xv);
   T mu, rs;
-  ln_value<T, MAXF>(xv, lane, d, scale, bias, sc, y0, mu, rs);
-#pragma unroll
-  for (int q = 0; q < MAXF; ++q) {
-    const int f = lane + 64 * q;
-    if (f < d) yb[f] = y0[q];
+  ln_value<T, F>(xv, j, d, scale, bias, sc, y0, mu, rs);
+  ln_store<T, F>(yb, j, d, y0);
+  if (!jg.has_lap) {
+    for (int c = 1; c <= jg.nt; ++c) {
+      T dx[F], dy[F];
+      ln_load<T, F>(xb + c * chs, j, d, dx);
+      ln_tangent<T, F>(xv, dx, sc, d, mu, rs, dy);
+      ln_store<T, F>(yb + c * chs, j, d, dy);
+    }
+    return;
   }
+  T s1[F];
+#pragma unroll
+  for (int e = 0; e < F; ++e) s1[e] = 0;
   T acc_dx2 = 0, acc_dmu2 = 0, acc_dvar2 = 0;
-  for (int c = 1; c <= j.nt; ++c) {
-    const T* xc = xb + c * chs;
-    T dx[MAXF];
+  for (int c = 1; c <= jg.nt; ++c) {
+    T dx[F], dy[F];
+    ln_load<T, F>(xb + c * chs, j, d, dx);
     T sd = 0, sxd = 0, sdd = 0;
 #pragma unroll
-    for (int q = 0; q < MAXF; ++q) {
-      const int f = lane + 64 * q;
-      dx[q] = (f < d) ? xc[f] : (T)0;
-      sd += dx[q];
-      sxd += xv[q] * dx[q];
-      sdd += dx[q] * dx[q];
+    for (int e = 0; e < F; ++e) {
+      sd += dx[e];
+      sxd += xv[e] * dx[e];
+      sdd += dx[e] * dx[e];
     }
-    sd = wave_sum(sd);
-    sxd = wave_sum(sxd);
+    sd = row_sum16(sd);
+    sxd = row_sum16(sxd);
     const T dmu = sd * inv_d;
     const T dvar = (T)2 * (sxd * inv_d - mu * dmu);
     const T drs = (T)(-0.5) * rs * rs * rs * dvar;
-    T* yc = yb + c * chs;
 #pragma unroll
-    for (int q = 0; q < MAXF; ++q) {
-      const int f = lane + 64 * q;
-      const T u = dx[q] - dmu;
-      if (f < d) yc[f] = (u * rs + (xv[q] - mu) * drs) * sc[q];
-      s1[q] += u * drs;
+    for (int e = 0; e < F; ++e) {
+      const T u = dx[e] - dmu;
+      dy[e] = (u * rs + (xv[e] - mu) * drs) * sc[e];
+      s1[e] += u * drs;
     }
-    if (j.has_lap) {
-      acc_dx2 += wave_sum(sdd) * inv_d;
-      acc_dmu2 += dmu * dmu;
-      acc_dvar2 += dvar * dvar;
-    }
+    ln_store<T, F>(yb + c * chs, j, d, dy);
+    acc_dx2 += row_sum16(sdd) * inv_d;
+    acc_dmu2 += dmu * dmu;
+    acc_dvar2 += dvar * dvar;
   }
-  if (j.has_lap) {
-    const int c = j.nt + 1;
-    const T* xc = xb + c * chs;
-    T lx[MAXF];
-    T sl = 0, sxl = 0;
+  const int cl = jg.nt + 1;
+  T lx[F], ly[F];
+  ln_load<T, F>(xb + cl * chs, j, d, lx);
+  T sl = 0, sxl = 0;
 #pragma unroll
-    for (int q = 0; q < MAXF; ++q) {
-      const int f = lane + 64 * q;
-      lx[q] = (f < d) ? xc[f] : (T)0;
-      sl += lx[q];
-      sxl += xv[q] * lx[q];
-    }
-    sl = wave_sum(sl);
-    sxl = wave_sum(sxl);
-    const T lmu = sl * inv_d;
-    const T lvar = (T)2 * (sxl * inv_d) + (T)2 * acc_dx2 - (T)2 * mu * lmu - (T)2 * acc_dmu2;
-    const T rs3 = rs * rs * rs;
-    const T lrs = (T)(-0.5) * rs3 * lvar + (T)0.75 * rs3 * rs * rs * acc_dvar2;
-    T* yc = yb + c * chs;
-#pragma unroll
-    for (int q = 0; q < MAXF; ++q) {
-      const int f = lane + 64 * q;
-      if (f < d) yc[f] = ((lx[q] - lmu) * rs + (xv[q] - mu) * lrs + (T)2 * s1[q]) * sc[q];
-    }
+  for (int e = 0; e < F; ++e) {
+    sl += lx[e];
+    sxl += xv[e] * lx[e];
   }
+  sl = row_sum16(sl);
+  sxl = row_sum16(sxl);
+  const T lmu = sl * inv_d;
+  const T lvar = (T)2 * (sxl * inv_d) + (T)2 * acc_dx2 - (T)2 * mu * lmu - (T)2 * acc_dmu2;
+  const T rs3 = rs * rs * rs;
+  const T lrs = (T)(-0.5) * rs3 * lvar + (T)0.75 * rs3 * rs * rs * acc_dvar2;
+#pragma unroll
+  for (int e = 0; e < F; ++e) ly[e] = ((lx[e] - lmu) * rs + (xv[e] - mu) * lrs + (T)2 * s1[e]) * sc[e];
+  ln_store<T, F>(yb + cl * chs, j, d, ly);
 }
 
 template <typename T>
@@ -191,12 +223,17 @@ int launch_layernorm_jet(const JetGeom& j, int d, const T* x, const T* scale, co
   const int64_t rows = j.n_cfg * j.N;
   if (rows == 0) return NQS_OK;
   NQS_REQUIRE(d <= 512, NQS_ERR_UNSUPPORTED, "LayerNorm width above 512 is not supported");
-  if (d <= 128)
-    hipLaunchKernelGGL((k_layernorm_jet<T, 2>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+  const dim3 grid((unsigned)cdiv(rows, 4));
+  if (d <= 32)
+    hipLaunchKernelGGL((k_layernorm_jet<T, 2>), grid, dim3(64), 0, st, j, d, x, scale, bias, y, rows);
+  else if (d <= 64)
+    hipLaunchKernelGGL((k_layernorm_jet<T, 4>), grid, dim3(64), 0, st, j, d, x, scale, bias, y, rows);
+  else if (d <= 128)
+    hipLaunchKernelGGL((k_layernorm_jet<T, 8>), grid, dim3(64), 0, st, j, d, x, scale, bias, y, rows);
   else if (d <= 256)
-    hipLaunchKernelGGL((k_layernorm_jet<T, 4>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+    hipLaunchKernelGGL((k_layernorm_jet<T, 16>), grid, dim3(64), 0, st, j, d, x, scale, bias, y, rows);
   else
-    hipLaunchKernelGGL((k_layernorm_jet<T, 8>), dim3((unsigned)rows), dim3(64), 0, st, j, d, x, scale, bias, y);
+    hipLaunchKernelGGL((k_layernorm_jet<T, 32>), grid, dim3(64), 0, st, j, d, x, scale, bias, y, rows);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

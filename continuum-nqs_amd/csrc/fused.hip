@@ -182,23 +182,43 @@ __device__ __forceinline__ void fu_put(__bf16* Ph, __bf16* Pl, int o, float v) {
   reinterpret_cast<uint16_t*>(Pl)[o] = (uint16_t)(l & 0xffffu);
 }
 
+// eight consecutive floats from LDS (16-byte aligned)
+__device__ __forceinline__ void fu_ld8(const float* p, float (&v)[8]) {
+  const f4v a = *reinterpret_cast<const f4v*>(p), b = *reinterpret_cast<const f4v*>(p + 4);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = a[e];
+    v[4 + e] = b[e];
+  }
+}
+
+// eight consecutive floats into the bf16 hi/lo planes at element offset o (16-byte aligned)
+__device__ __forceinline__ void fu_put8(__bf16* Ph, __bf16* Pl, int o, const float (&v)[8]) {
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) split_bf16x2(v[2 * e], v[2 * e + 1], h[e], l[e]);
+  *reinterpret_cast<u4v*>(Ph + o) = (u4v){h[0], h[1], h[2], h[3]};
+  *reinterpret_cast<u4v*>(Pl + o) = (u4v){l[0], l[1], l[2], l[3]};
+}
+
 // LayerNorm of the residual stream (wave w's block xr) into the K = 128 operand planes: the
-// blocks go through the float scratch Xs, then wave w normalises rows 8 w .. 8 w + 7 as
-// k_layernorm_jet does (lane holds features lane and lane + 64).
+// blocks go through the float scratch Xs, then wave w normalises rows 8 w .. 8 w + 7, four at a
+// time on 16-lane groups (ln_value's layout: lane j holds features 8 j .. 8 j + 7), as
+// k_layernorm_jet does.
 __device__ __forceinline__ void fu_layernorm(const f16v& xr, int wave, int lane, float* Xs, const float* __restrict__ s,
                                              const float* __restrict__ b, __bf16* Hh, __bf16* Hl) {
   __syncthreads();  // every wave is done with the region Xs aliases
 #pragma unroll
   for (int r = 0; r < 16; ++r) Xs[fu_row(lane, r) * FU_D + 32 * wave + (lane & 31)] = xr[r];
   __syncthreads();
+  const int j = lane & 15;
 #pragma unroll
-  for (int i = 0; i < FU_R / 4; ++i) {
-    const int row = (FU_R / 4) * wave + i;
-    float xv[2] = {Xs[row * FU_D + lane], Xs[row * FU_D + lane + 64]};
-    float sc[2], y[2], mu, rs;
-    ln_value<float, 2>(xv, lane, FU_D, s, b, sc, y, mu, rs);
-    fu_put(Hh, Hl, row * FU_HLD + lane, y[0]);
-    fu_put(Hh, Hl, row * FU_HLD + lane + 64, y[1]);
+  for (int it = 0; it < 2; ++it) {
+    const int row = (FU_R / 4) * wave + 4 * it + (lane >> 4);
+    float xv[8], sc[8], y[8], mu, rs;
+    fu_ld8(Xs + row * FU_D + 8 * j, xv);
+    ln_value<float, 8>(xv, j, FU_D, s, b, sc, y, mu, rs);
+    fu_put8(Hh, Hl, row * FU_HLD + 8 * j, y);
   }
   __syncthreads();
 }
@@ -407,8 +427,8 @@ __device__ __forceinline__ void fj_mma4(f16v* acc, const u4v (&b)[8], const __bf
 }
 
 // LayerNorm jet of the residual stream (wave w's blocks xr) into the K = 128 operand planes: the
-// blocks go through the float scratch Xs; wave w then normalises tokens 4 w .. 4 w + 3, each as the
-// value row and its two tangent rows (lane holds features lane and lane + 64).
+// blocks go through the float scratch Xs; wave w then normalises tokens 4 w .. 4 w + 3 on its four
+// 16-lane groups, each as the value row and its two tangent rows (ln_value / ln_tangent layout).
 __device__ __forceinline__ void fj_layernorm(const f16v (&xr)[2], int wave, int lane, float* Xs,
                                              const float* __restrict__ s, const float* __restrict__ b, __bf16* Hh,
                                              __bf16* Hl) {
@@ -419,23 +439,18 @@ __device__ __forceinline__ void fj_layernorm(const f16v (&xr)[2], int wave, int 
 #pragma unroll
   for (int r = 8; r < 16; ++r) Xs[(16 + fu_row(lane, r)) * FU_D + cw] = xr[1][r];
   __syncthreads();
+  const int j = lane & 15, t = 4 * wave + (lane >> 4);
+  float xv[8], sc[8], y[8], mu, rs;
+  fu_ld8(Xs + t * FU_D + 8 * j, xv);
+  ln_value<float, 8>(xv, j, FU_D, s, b, sc, y, mu, rs);
+  fu_put8(Hh, Hl, t * FU_HLD + 8 * j, y);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int t = 4 * wave + i;
-    float xv[2] = {Xs[t * FU_D + lane], Xs[t * FU_D + lane + 64]};
-    float sc[2], y[2], mu, rs;
-    ln_value<float, 2>(xv, lane, FU_D, s, b, sc, y, mu, rs);
-    fu_put(Hh, Hl, t * FU_HLD + lane, y[0]);
-    fu_put(Hh, Hl, t * FU_HLD + lane + 64, y[1]);
-#pragma unroll
-    for (int c = 1; c < FJ_C; ++c) {
-      const int row = FU_N * c + t;
-      const float dx[2] = {Xs[row * FU_D + lane], Xs[row * FU_D + lane + 64]};
-      float dy[2];
-      ln_tangent<float, 2>(xv, dx, sc, FU_D, mu, rs, dy);
-      fu_put(Hh, Hl, row * FU_HLD + lane, dy[0]);
-      fu_put(Hh, Hl, row * FU_HLD + lane + 64, dy[1]);
-    }
+  for (int c = 1; c < FJ_C; ++c) {
+    const int row = FU_N * c + t;
+    float dx[8], dy[8];
+    fu_ld8(Xs + row * FU_D + 8 * j, dx);
+    ln_tangent<float, 8>(xv, dx, sc, FU_D, mu, rs, dy);
+    fu_put8(Hh, Hl, row * FU_HLD + 8 * j, dy);
   }
   __syncthreads();
 }

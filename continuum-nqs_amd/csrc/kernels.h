@@ -135,57 +135,71 @@ struct DetParams {
   const double* det_logw; // (K) complex or NULL
   const double* jas;      // jastrow params: rbf [c_same(n_knots), c_opp(n_knots)]; cusp [loga]
 };
-// Value row of the LayerNorm (flax: mean, fast variance E[x^2] - mu^2, eps 1e-6): lane holds
-// features f = lane + 64 q (q < MAXF) in xv, zero for f >= d.  Returns the scales sc, the outputs
-// y = (x - mu) rs scale + bias (0 for f >= d), mu and rs = (var + eps)^-1/2; no FMA contraction, so
-// every caller rounds alike.
-template <typename T, int MAXF>
-__device__ __forceinline__ void ln_value(const T (&xv)[MAXF], int lane, int d, const T* __restrict__ scale,
-                                         const T* __restrict__ bias, T (&sc)[MAXF], T (&y)[MAXF], T& mu, T& rs) {
+// Sum over the 16 lanes of a DPP row (lanes 16 g .. 16 g + 15), the same bits in each of them:
+// the first four steps of wave_sum.
+template <typename T>
+__device__ __forceinline__ T row_sum16(T v) {
+  v += row_dpp<0xB1>(v);
+  v += row_dpp<0x4E>(v);
+  v += row_dpp<0x141>(v);
+  v += row_dpp<0x140>(v);
+  return v;
+}
+
+// LayerNorm rows on 16-lane groups: the row's d features are spread over the 16 lanes of one DPP
+// row, lane j (= lane % 16) holding features F j .. F j + F - 1 in xv (zero for f >= d), so a wave
+// normalises four rows at a time.  Sums run over a lane's F values in order, then over the 16
+// lanes (row_sum16); no FMA contraction, so every kernel that inlines these rounds alike.
+//
+// Value row (flax: mean, fast variance E[x^2] - mu^2, eps 1e-6): the scales sc, the outputs
+// y = (x - mu) rs scale + bias (0 for f >= d), mu and rs = (var + eps)^-1/2.
+template <typename T, int F>
+__device__ __forceinline__ void ln_value(const T (&xv)[F], int j, int d, const T* __restrict__ scale,
+                                         const T* __restrict__ bias, T (&sc)[F], T (&y)[F], T& mu, T& rs) {
 #pragma clang fp contract(off)
   const T inv_d = (T)1 / (T)d;
   T sum = 0, sum2 = 0;
 #pragma unroll
-  for (int q = 0; q < MAXF; ++q) {
-    const int f = lane + 64 * q;
-    sc[q] = (f < d) ? scale[f] : (T)0;
-    sum += xv[q];
-    sum2 += xv[q] * xv[q];
+  for (int e = 0; e < F; ++e) {
+    const int f = F * j + e;
+    sc[e] = (f < d) ? scale[f] : (T)0;
+    sum += xv[e];
+    sum2 += xv[e] * xv[e];
   }
-  sum = wave_sum(sum);
-  sum2 = wave_sum(sum2);
+  sum = row_sum16(sum);
+  sum2 = row_sum16(sum2);
   mu = sum * inv_d;
   T var = sum2 * inv_d - mu * mu;
   var = var > (T)0 ? var : (T)0;
   rs = (T)1 / sqrt(var + (T)1e-6);
 #pragma unroll
-  for (int q = 0; q < MAXF; ++q) {
-    const int f = lane + 64 * q;
-    y[q] = (f < d) ? (xv[q] - mu) * rs * sc[q] + bias[f] : (T)0;
+  for (int e = 0; e < F; ++e) {
+    const int f = F * j + e;
+    y[e] = (f < d) ? (xv[e] - mu) * rs * sc[e] + bias[f] : (T)0;
   }
 }
 
-// First-order tangent row of the LayerNorm jet, given the value row's xv, mu, rs and scales sc
-// (from ln_value) and the tangent dx (same lane layout): dmu = mean(dx),
-// drs = -rs^3 (mean(x dx) - mu dmu), dy = ((dx - dmu) rs + (x - mu) drs) scale.  No FMA contraction.
-template <typename T, int MAXF>
-__device__ __forceinline__ void ln_tangent(const T (&xv)[MAXF], const T (&dx)[MAXF], const T (&sc)[MAXF], int d, T mu,
-                                           T rs, T (&dy)[MAXF]) {
+// First-order tangent row, given the value row's xv, mu, rs and scales sc (from ln_value) and the
+// tangent dx: dmu = mean(dx), drs = -rs^3 (mean(x dx) - mu dmu),
+// dy = ((dx - dmu) rs + (x - mu) drs) scale.
+template <typename T, int F>
+__device__ __forceinline__ void ln_tangent(const T (&xv)[F], const T (&dx)[F], const T (&sc)[F], int d, T mu, T rs,
+                                           T (&dy)[F]) {
 #pragma clang fp contract(off)
   const T inv_d = (T)1 / (T)d;
   T sd = 0, sxd = 0;
 #pragma unroll
-  for (int q = 0; q < MAXF; ++q) {
-    sd += dx[q];
-    sxd += xv[q] * dx[q];
+  for (int e = 0; e < F; ++e) {
+    sd += dx[e];
+    sxd += xv[e] * dx[e];
   }
-  sd = wave_sum(sd);
-  sxd = wave_sum(sxd);
+  sd = row_sum16(sd);
+  sxd = row_sum16(sxd);
   const T dmu = sd * inv_d;
   const T dvar = (T)2 * (sxd * inv_d - mu * dmu);
   const T drs = (T)(-0.5) * rs * rs * rs * dvar;
 #pragma unroll
-  for (int q = 0; q < MAXF; ++q) dy[q] = ((dx[q] - dmu) * rs + (xv[q] - mu) * drs) * sc[q];
+  for (int e = 0; e < F; ++e) dy[e] = ((dx[e] - dmu) * rs + (xv[e] - mu) * drs) * sc[e];
 }
 
 template <typename T>
