@@ -299,9 +299,12 @@ def _tile_choices():
 
 
 def _stage_dict(stages, mins):
-    return {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
-            "minsr": float(stages[3]), "update_repack": float(stages[4]),
-            "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]), "minsr_cholesky": float(mins[2]),
+    """The step's stages in order (HIP events on the main stream): sweeps, O rows, centred Gram, the local
+    energies with Ewald and clip (the Cholesky factorisation runs beside them on a side stream), solve and
+    push-through, update; and the MinSR sub-stages timed on the streams they ran on."""
+    return {"sample": float(stages[0]), "grad": float(stages[1]), "center_gram": float(stages[2]),
+            "eloc_ewald_clip": float(stages[3]), "solve": float(stages[4]), "update_repack": float(stages[5]),
+            "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]), "minsr_cholesky_beside_eloc": float(mins[2]),
             "minsr_solve_refine": float(mins[3]), "minsr_pushthrough": float(mins[4])}
 
 
@@ -375,11 +378,7 @@ def bench_c4(args):
             "roofline": {"bound": "mfma", "kernel": "Gram (k_gemm_nt<float>, lower tiles)",
                          "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                          "frac": achieved / PEAK["fp32"], "traffic": None, "gram_ms": float(mins[1])},
-            "stage_ms": {"sample": float(stages[0]), "eloc_ewald": float(stages[1]), "clip_grad": float(stages[2]),
-                         "minsr": float(stages[3]), "update_repack": float(stages[4]),
-                         "minsr_center": float(mins[0]), "minsr_gram": float(mins[1]),
-                         "minsr_cholesky": float(mins[2]), "minsr_solve_refine": float(mins[3]),
-                         "minsr_pushthrough": float(mins[4])},
+            "stage_ms": _stage_dict(stages, mins),
             "E_mean": float(stats[0]), "E_err": float(stats[1]), "cpu_baseline": None}
 
 

@@ -269,34 +269,34 @@ static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
   return w;
 }
 
-// Optional stage timing of nqs_minsr_direction: hipEvents recorded on the call's stream
-// between centre | Gram | Cholesky | solve + refinement | push-through.
+// Optional stage timing of the MinSR calls: hipEvents recorded on each call's stream: 0 start,
+// 1 centred, 2 Gram shifted and deflated, 3 factorised, 6 solve started, 4 solved and refined,
+// 5 pushed through.
 static bool g_minsr_profile = false;
-static hipEvent_t g_minsr_ev[6];
+static hipEvent_t g_minsr_ev[7];
 static bool g_minsr_ev_ok = false;
 static int minsr_mark(int i, hipStream_t st) {
   if (!g_minsr_profile) return NQS_OK;
   if (!g_minsr_ev_ok) {
-    for (int k = 0; k < 6; ++k) NQS_CHECK_HIP(hipEventCreate(&g_minsr_ev[k]));
+    for (int k = 0; k < 7; ++k) NQS_CHECK_HIP(hipEventCreate(&g_minsr_ev[k]));
     g_minsr_ev_ok = true;
   }
   NQS_CHECK_HIP(hipEventRecord(g_minsr_ev[i], st));
   return NQS_OK;
 }
 
+// The MinSR direction in three stages on one workspace: the centred Gram shifted by lam and
+// deflated (minsr_gram), its Cholesky factor (minsr_factor, the only stage that writes info), and
+// the solve for the local energies with refinement and the push-through (minsr_solve).
 template <typename T>
-static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* dth, int32_t* info, char* ws,
-                      hipStream_t st) {
+static int minsr_gram(const nqs_minsr_desc* d, T* O, char* ws, hipStream_t st) {
   const int64_t W = d->n_walkers, n = 2 * W, P = d->n_params, ldo = d->ldo;
   MinsrWs w = minsr_ws(d, ws);
   NQS_TRY(minsr_mark(0, st));
-  NQS_TRY(fill_i32(info, 1, 0, st));
-  // the bf16 Gram reads only the centred bf16 copy, so O stays as given and the GEMVs below
+  // the bf16 Gram reads only the centred bf16 copy, so O stays as given and the GEMVs of the solve
   // apply the centring on the fly; the float64 / float32 Gram reads O, which is centred in place
   const bool keep = d->gram == NQS_GRAM_BF16;
-  const double* mc = keep ? w.mean : nullptr;
   NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, keep ? 0 : 1, w.part, st));
-  NQS_TRY(make_etil(eps, W, w.et, st));
   NQS_TRY(minsr_mark(1, st));
   if (d->gram == NQS_GRAM_BF16) {
     // A64 is written only after the Gram, so it lends its space to the Gram's partials
@@ -323,7 +323,6 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
       g.C = (T*)w.A64;
       g.ldc = n;
       NQS_TRY(gemm_nt<T>(g, 1, st));
-      
     } else {
       g.C = (T*)w.T32;
       g.ldc = n;
@@ -333,8 +332,25 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   }
   NQS_TRY(deflate_centring(w.A64, n, W, d->lam, w.ts, w.sc, st));
   NQS_TRY(minsr_mark(2, st));
+  return NQS_OK;
+}
+
+static int minsr_factor(const nqs_minsr_desc* d, int32_t* info, char* ws, hipStream_t st) {
+  const int64_t n = 2 * d->n_walkers;
+  MinsrWs w = minsr_ws(d, ws);
+  NQS_TRY(fill_i32(info, 1, 0, st));
   NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st));
   NQS_TRY(minsr_mark(3, st));
+  return NQS_OK;
+}
+
+template <typename T>
+static int minsr_solve(const nqs_minsr_desc* d, T* O, const double* eps, double* dth, char* ws, hipStream_t st) {
+  const int64_t W = d->n_walkers, n = 2 * W, P = d->n_params, ldo = d->ldo;
+  MinsrWs w = minsr_ws(d, ws);
+  const double* mc = d->gram == NQS_GRAM_BF16 ? w.mean : nullptr;
+  NQS_TRY(minsr_mark(6, st));
+  NQS_TRY(make_etil(eps, W, w.et, st));
   NQS_CHECK_HIP(hipMemcpyAsync(w.y, w.et, n * 8, hipMemcpyDeviceToDevice, st));
   NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.y, st));
   for (int it = 0; it < d->refine; ++it) {
@@ -350,6 +366,14 @@ static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* 
   NQS_TRY(gemv_cols_c<T>(O, ldo, W, P, w.y, mc, w.part, w.red, dth, P, st));
   NQS_TRY(minsr_mark(5, st));
   return NQS_OK;
+}
+
+template <typename T>
+static int minsr_impl(const nqs_minsr_desc* d, T* O, const double* eps, double* dth, int32_t* info, char* ws,
+                      hipStream_t st) {
+  NQS_TRY(minsr_gram<T>(d, O, ws, st));
+  NQS_TRY(minsr_factor(d, info, ws, st));
+  return minsr_solve<T>(d, O, eps, dth, ws, st);
 }
 
 // ------------------------------------------------------------------ MinSR of many same-size systems
@@ -722,9 +746,8 @@ int64_t nqs_minsr_workspace_bytes(const nqs_minsr_desc* d) {
   return minsr_ws(d, nullptr).bytes;
 }
 
-int nqs_minsr_direction(const nqs_minsr_desc* d, void* O, const double* eps, double* dth, int32_t* info, void* ws,
-                        int64_t ws_bytes, void* stream) {
-  NQS_REQUIRE(d && O && eps && dth && info && ws, NQS_ERR_ARG, "null pointer");
+// the checks every single-system MinSR call makes on its descriptor and workspace
+static int minsr_check(const nqs_minsr_desc* d, int64_t ws_bytes) {
   const int64_t need = nqs_minsr_workspace_bytes(d);
   NQS_REQUIRE(need >= 0, NQS_ERR_ARG, "bad MinSR descriptor");
   NQS_REQUIRE(ws_bytes >= need, NQS_ERR_WORKSPACE, "MinSR workspace too small: need " + std::to_string(need));
@@ -734,6 +757,36 @@ int nqs_minsr_direction(const nqs_minsr_desc* d, void* O, const double* eps, dou
   NQS_REQUIRE(!(d->gram == NQS_GRAM_FP64 && d->o_dtype != NQS_FP64), NQS_ERR_ARG, "a float64 Gram needs a float64 O");
   NQS_REQUIRE(!(d->gram == NQS_GRAM_FP32 && d->o_dtype != NQS_MIXED), NQS_ERR_ARG, "a float32 Gram needs a float32 O");
   NQS_REQUIRE(d->gram >= 0 && d->gram <= 2 && d->refine >= 0, NQS_ERR_ARG, "bad Gram mode");
+  return NQS_OK;
+}
+
+int nqs_minsr_gram(const nqs_minsr_desc* d, void* O, void* ws, int64_t ws_bytes, void* stream) {
+  NQS_REQUIRE(d && O && ws, NQS_ERR_ARG, "null pointer");
+  NQS_TRY(minsr_check(d, ws_bytes));
+  hipStream_t st = (hipStream_t)stream;
+  if (d->o_dtype == NQS_FP64) return minsr_gram<double>(d, (double*)O, (char*)ws, st);
+  return minsr_gram<float>(d, (float*)O, (char*)ws, st);
+}
+
+int nqs_minsr_factor(const nqs_minsr_desc* d, int32_t* info, void* ws, int64_t ws_bytes, void* stream) {
+  NQS_REQUIRE(d && info && ws, NQS_ERR_ARG, "null pointer");
+  NQS_TRY(minsr_check(d, ws_bytes));
+  return minsr_factor(d, info, (char*)ws, (hipStream_t)stream);
+}
+
+int nqs_minsr_solve(const nqs_minsr_desc* d, void* O, const double* eps, double* dth, void* ws, int64_t ws_bytes,
+                    void* stream) {
+  NQS_REQUIRE(d && O && eps && dth && ws, NQS_ERR_ARG, "null pointer");
+  NQS_TRY(minsr_check(d, ws_bytes));
+  hipStream_t st = (hipStream_t)stream;
+  if (d->o_dtype == NQS_FP64) return minsr_solve<double>(d, (double*)O, eps, dth, (char*)ws, st);
+  return minsr_solve<float>(d, (float*)O, eps, dth, (char*)ws, st);
+}
+
+int nqs_minsr_direction(const nqs_minsr_desc* d, void* O, const double* eps, double* dth, int32_t* info, void* ws,
+                        int64_t ws_bytes, void* stream) {
+  NQS_REQUIRE(d && O && eps && dth && info && ws, NQS_ERR_ARG, "null pointer");
+  NQS_TRY(minsr_check(d, ws_bytes));
   hipStream_t st = (hipStream_t)stream;
   if (d->o_dtype == NQS_FP64) return minsr_impl<double>(d, (double*)O, eps, dth, info, (char*)ws, st);
   return minsr_impl<float>(d, (float*)O, eps, dth, info, (char*)ws, st);
@@ -776,7 +829,10 @@ int nqs_minsr_profile(int32_t enable, float* ms_out) {
   if (ms_out) {
     NQS_REQUIRE(g_minsr_ev_ok, NQS_ERR_ARG, "no profiled MinSR call yet");
     NQS_CHECK_HIP(hipEventSynchronize(g_minsr_ev[5]));
-    for (int k = 0; k < 5; ++k) NQS_CHECK_HIP(hipEventElapsedTime(&ms_out[k], g_minsr_ev[k], g_minsr_ev[k + 1]));
+    // centre 0-1, Gram 1-2, Cholesky 2-3, solve + refinement 6-4, push-through 4-5
+    const int from[5] = {0, 1, 2, 6, 4}, to[5] = {1, 2, 3, 4, 5};
+    for (int k = 0; k < 5; ++k)
+      NQS_CHECK_HIP(hipEventElapsedTime(&ms_out[k], g_minsr_ev[from[k]], g_minsr_ev[to[k]]));
   }
   return NQS_OK;
 }

@@ -66,6 +66,9 @@ SIGNATURES = [
                                   c_vp]),
     ("nqs_minsr_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc)]),
     ("nqs_minsr_direction", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_minsr_gram", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_minsr_factor", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_i64, c_vp]),
+    ("nqs_minsr_solve", c_i32, [ctypes.POINTER(MinsrDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     ("nqs_clip_energies", c_i32, [c_vp, c_i64, c_dbl, c_vp, c_vp, c_vp]),
     ("nqs_minsr_batched_workspace_bytes", c_i64, [ctypes.POINTER(MinsrDesc), c_i32]),
     ("nqs_minsr_direction_batched", c_i32, [ctypes.POINTER(MinsrDesc), c_i32, ctypes.POINTER(c_vp),

@@ -138,6 +138,45 @@ def minsr_direction(O, eps, lam, gram="auto", refine=1):
     return dth
 
 
+class MinSRSplit:
+    """minsr_direction in three calls on one workspace (nqs_minsr_gram, nqs_minsr_factor,
+    nqs_minsr_solve; together exactly nqs_minsr_direction): gram() centres O and forms the shifted
+    Gram, factor() its Cholesky factor (info), solve(eps) the direction.  Each call runs on the
+    current torch stream; the caller orders them (factor may run on another stream than the local
+    energies, which the Gram does not need)."""
+
+    def __init__(self, O, lam, gram="auto", refine=1):
+        self.lib = _lib.load()
+        dev = _lib.device()
+        fp64 = O.data.dtype == torch.float64
+        if gram == "auto":
+            gram = "fp64" if fp64 else "bf16"
+        self.O = O
+        self.d = lib_minsr_desc(O.W, O.P, O.ldo, fp64, gram, refine, lam)
+        nb = self.lib.nqs_minsr_workspace_bytes(ctypes.byref(self.d))
+        if nb < 0:
+            _lib.check(-1, "nqs_minsr_workspace_bytes")
+        self.ws = _lib.workspace(nb, "minsr")
+        self.info = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def gram(self):
+        _lib.check(self.lib.nqs_minsr_gram(ctypes.byref(self.d), _lib.ptr(self.O.data), _lib.ptr(self.ws),
+                                           self.ws.numel(), _lib.stream()), "nqs_minsr_gram")
+
+    def factor(self):
+        _lib.check(self.lib.nqs_minsr_factor(ctypes.byref(self.d), _lib.ptr(self.info), _lib.ptr(self.ws),
+                                             self.ws.numel(), _lib.stream()), "nqs_minsr_factor")
+
+    def solve(self, eps):
+        dev = _lib.device()
+        eps = torch.as_tensor(eps).to(device=dev, dtype=torch.complex128).contiguous()
+        dth = torch.empty(self.O.P, dtype=torch.float64, device=dev)
+        _lib.check(self.lib.nqs_minsr_solve(ctypes.byref(self.d), _lib.ptr(self.O.data),
+                                            _lib.ptr(torch.view_as_real(eps)), _lib.ptr(dth), _lib.ptr(self.ws),
+                                            self.ws.numel(), _lib.stream()), "nqs_minsr_solve")
+        return dth
+
+
 def lib_minsr_desc(W, P, ldo, fp64, gram, refine, lam):
     d = _lib.MinsrDesc()
     d.n_walkers, d.n_params, d.ldo = W, P, ldo
@@ -181,7 +220,7 @@ class VMCStep:
 
     def __init__(self, model, theta, R, S, m_star=1.0, lambda_r=1.0, h_z=0.0, ext_field=None, coulomb=None,
                  lr=0.05, lam=1e-3, n_sweeps=10, step_size=0.4, clip_k=5.0, n_spin_flips=2, gram="auto",
-                 refine=1, graph=False, pinning=None):
+                 refine=1, graph=False, pinning=None, overlap=True):
         self.model = model
         self.theta = theta.to(_lib.device(), torch.float64).contiguous()
         self.lp = LogPsi(model, self.theta)
@@ -191,9 +230,13 @@ class VMCStep:
         self.lr, self.lam, self.n_sweeps, self.step_size, self.clip_k = lr, lam, n_sweeps, step_size, clip_k
         self.n_spin_flips, self.gram, self.refine = n_spin_flips, gram, refine
         self.O = None
+        self.last_info = None
         self.timers = None
         self.graph = graph      # replay the sweeps from a captured HIP graph (nqs.graphs.GraphSweeps)
         self._gs = None
+        # factorise the Gram on a side stream (default priority) while the local energies run
+        self.overlap = overlap
+        self._side = None
         self.failed = torch.zeros((), dtype=torch.int64, device=_lib.device())   # skipped updates
 
     @property
@@ -225,13 +268,29 @@ class VMCStep:
         else:
             ap, asp = self.chain.run(self.n_sweeps, self.step_size, self.n_spin_flips, key=key, streams=streams)
         if t: t.append(self._ev())
-        e = self.energies()
-        if t: t.append(self._ev())
-        e_clip, stats = clip_energies(e, self.clip_k)
+        # O, then the centred Gram; its Cholesky factor does not need the local energies, so it runs
+        # on a side stream beside them (overlap), and the solve waits for both
         self.O, _ = grad_params(self.lp, self.chain.R, self.chain.S, self.O)
         if t: t.append(self._ev())
-        d = minsr_direction(self.O, e_clip, self.lam, self.gram, self.refine)
-        info = minsr_direction.last_info
+        ms = MinSRSplit(self.O, self.lam, self.gram, self.refine)
+        ms.gram()
+        if t: t.append(self._ev())
+        main = torch.cuda.current_stream()
+        if self.overlap:
+            if self._side is None:
+                self._side = torch.cuda.Stream()
+            self._side.wait_stream(main)
+            with torch.cuda.stream(self._side):
+                ms.factor()
+        else:
+            ms.factor()
+        e = self.energies()
+        e_clip, stats = clip_energies(e, self.clip_k)
+        if t: t.append(self._ev())
+        if self.overlap:
+            main.wait_stream(self._side)
+        d = ms.solve(e_clip)
+        info = self.last_info = ms.info  # 0, or 1 + the first non-positive pivot (device int32)
         if t: t.append(self._ev())
         # A step whose shifted Gram was not positive definite in this precision (info != 0) or whose
         # direction is not finite leaves theta unchanged; the decision stays on the device (no host

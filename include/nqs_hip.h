@@ -214,9 +214,22 @@ int64_t nqs_minsr_workspace_bytes(const nqs_minsr_desc* d);
  * dtheta_dev (P) float64; info_dev (int32, device): 0, or 1 + the first non-positive pivot. */
 int nqs_minsr_direction(const nqs_minsr_desc* d, void* O_dev, const double* eps_dev, double* dtheta_dev,
                         int32_t* info_dev, void* ws_dev, int64_t ws_bytes, void* stream);
-/* Stage timing of nqs_minsr_direction (for measurement, no reference counterpart): enable >= 0
+/* The same direction in three calls on one workspace (no reference counterpart: they let a caller
+ * overlap the factorisation with the local energies, which the Gram does not need).
+ * nqs_minsr_gram: centres O (in place for the float64 / float32 Gram, as nqs_minsr_direction does) and
+ * forms the shifted, deflated Gram; nqs_minsr_factor: its Cholesky factor, info_dev as in
+ * nqs_minsr_direction; nqs_minsr_solve: the solve for eps with refinement and the push-through into
+ * dtheta.  The calls must come in this order with the same descriptor, O and workspace, each after
+ * the previous one has completed on the device (same stream, or ordered by events); the workspace must
+ * not be used otherwise in between.  gram + factor + solve computes exactly nqs_minsr_direction. */
+int nqs_minsr_gram(const nqs_minsr_desc* d, void* O_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+int nqs_minsr_factor(const nqs_minsr_desc* d, int32_t* info_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+int nqs_minsr_solve(const nqs_minsr_desc* d, void* O_dev, const double* eps_dev, double* dtheta_dev, void* ws_dev,
+                    int64_t ws_bytes, void* stream);
+/* Stage timing of the MinSR calls (for measurement, no reference counterpart): enable >= 0
  * switches hipEvent recording on (1) or off (0) for later calls; ms_out (5 floats, host) receives
- * the last profiled call's centre, Gram, Cholesky, solve+refinement, push-through times. */
+ * the last profiled direction's centre, Gram, Cholesky, solve+refinement, push-through times (each
+ * stage timed on the stream it ran on). */
 int nqs_minsr_profile(int32_t enable, float* ms_out);
 /* MinSR directions of n_sys independent systems with the same walker count (the batched uneven
  * ensemble; no reference counterpart: the reference solves one system per call).  descs (n_sys)

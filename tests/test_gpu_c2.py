@@ -142,5 +142,5 @@ def test_c2_vmc_step(precision):
     vs.chain.run(20, 0.3 * RS, 2, key=1)
     st = vs.step(key=2).cpu().numpy()
     assert np.all(np.isfinite(st))
-    assert int(nms.minsr_direction.last_info.item()) == 0
+    assert int(vs.last_info.item()) == 0 and vs.failed_steps == 0
     assert bool(torch.isfinite(vs.last["dtheta"]).all()) and float(vs.last["dtheta"].abs().max()) > 0

@@ -135,5 +135,5 @@ def test_c4_vmc_step_fp32_gram(c4):
     st = vs.step(key=3).cpu().numpy()
     print(f"c4 step: E = {st[0]:.6f} +- {st[1]:.2e}", flush=True)
     assert np.all(np.isfinite(st))
-    assert int(nms.minsr_direction.last_info.item()) == 0
+    assert int(vs.last_info.item()) == 0 and vs.failed_steps == 0
     assert bool(torch.isfinite(vs.last["dtheta"]).all()) and float(vs.last["dtheta"].abs().max()) > 0

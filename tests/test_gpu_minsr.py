@@ -148,6 +148,52 @@ def test_minsr_direction_is_deterministic(gram):
     assert torch.equal(d1, d2)
 
 
+@pytest.mark.parametrize("gram", ["fp64", "fp32", "bf16"])
+def test_split_minsr_calls_equal_direction(gram):
+    """nqs_minsr_gram + nqs_minsr_factor + nqs_minsr_solve (the factorisation on another stream, ordered by
+    stream waits, as VMCStep runs it beside the local energies) give nqs_minsr_direction's bits."""
+    g, arch, theta, m = _pair("base", precision="fp64" if gram == "fp64" else "mixed", seed=9)
+    R, S = _configs(512, 5, 7.0, 9)
+    eps = torch.tensor(np.random.default_rng(9).normal(size=512) + 1j * np.random.default_rng(10).normal(size=512))
+    O, _ = nms.grad_params(make_logpsi(m, theta.cuda()), R, S)
+    base = O.data.clone()
+    d1 = nms.minsr_direction(O, eps, 1e-3, gram=gram)
+    O.data.copy_(base)
+    ms = nms.MinSRSplit(O, 1e-3, gram=gram)
+    ms.gram()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ms.factor()
+    torch.cuda.current_stream().wait_stream(side)
+    d2 = ms.solve(eps)
+    assert int(ms.info.item()) == 0
+    assert torch.equal(d1, d2)
+
+
+def test_overlapped_step_equals_serial_step():
+    """A VMC step with the Gram factorised on a side stream beside the local energies equals the same
+    step with every stage on one stream, bit for bit (theta, energies, statistics)."""
+    N, L, hz = 5, 7.0, 0.5
+    g = bands.rashba_gs(L, N, 1.0, 1.0, hz, n_cut=10, twist=(0.1, 0.17))
+    model = ContinuumSpinorPsiformer(n_elec=N, Gs=g["Gs"], W=g["W"], L=L, embed_dim=16, depth=2, n_heads=4,
+                                     precision="mixed")
+    theta = opf.init_theta(opf.Arch(N, g["Gs"], g["W"], L, embed_dim=16, depth=2, n_heads=4), 5,
+                           backflow_scale=0.05, scale_all=0.02)
+    R, S = _configs(512, N, L, 11)
+    out = []
+    for overlap in (True, False):
+        vs = nms.VMCStep(model, theta.cuda().clone(), torch.tensor(R).cuda(), torch.tensor(S, dtype=torch.int32).cuda(),
+                         m_star=1.0, lambda_r=1.0, h_z=hz, lr=0.05, lam=1e-3, n_sweeps=2, step_size=0.5,
+                         overlap=overlap)
+        stats = [vs.step(key=k) for k in range(2)]
+        torch.cuda.synchronize()
+        out.append((vs.theta.clone(), vs.last["e"].clone(), torch.stack(stats)))
+    assert vs.failed_steps == 0
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
+
+
 def test_bf16_gram_tail_split_matches_unsplit():
     """W = 4096: the 256-tile bf16 Gram has 528 tiles, so on a 256-CU device its last round holds
     16 tiles, which the single-system call splits 16 ways along K (partials summed in a fixed
