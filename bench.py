@@ -256,7 +256,8 @@ def time_path(model, theta, R, S, coul, args, gram, steps, warmup, graph=False, 
     from nqs import _lib
     from nqs.minsr import VMCStep
     vs = VMCStep(model, theta, R, S, m_star=1.0, lambda_r=LAM, h_z=HZ, coulomb=coul, lr=0.05, lam=1e-4,
-                 n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1, graph=graph)
+                 n_sweeps=args.sweeps, step_size=0.3 * RS, clip_k=5.0, gram=gram, refine=1, graph=graph,
+                 overlap=args.overlap == "on")
     if burn:
         vs.chain.run(burn, 0.3 * RS, 2, key=7)
     lib = _lib.load()
@@ -340,7 +341,8 @@ def bench_c4(args):
     pins = make_gaussian_pins([(0.25 * L, 0.3 * L), (0.7 * L, 0.65 * L)], 0.05, rs, kind="spin")
     _, _, coul = make_ewald(L, 1.0, n_real=2, n_recip=8)
     vs = VMCStep(model, theta, R, S, m_star=1.0, lambda_r=C4["lam"], h_z=C4["hz"], coulomb=coul, lr=0.05, lam=1e-4,
-                 n_sweeps=args.sweeps, step_size=0.3 * rs, clip_k=5.0, gram="fp32", refine=1, pinning=pins)
+                 n_sweeps=args.sweeps, step_size=0.3 * rs, clip_k=5.0, gram="fp32", refine=1, pinning=pins,
+                 overlap=args.overlap == "on")
     if args.burn:
         vs.chain.run(args.burn, 0.3 * rs, 2, key=7)
     lib = _lib.load()
@@ -581,6 +583,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="timed steps of the plain float64 path after the mixed run (0 = skip; mixed only)")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps from a captured HIP graph")
+    ap.add_argument("--overlap", default="on", choices=["on", "off"],
+                    help="factorise the Gram on a side stream beside the local energies (VMCStep(overlap=...))")
     ap.add_argument("--config", default="c3", choices=["c3", "c4", "c5", "next"],
                     help="c3: the headline N=16 workload (default); c4: the N=32 large configuration; c5: the "
                          "24-system uneven ensemble; next: the Adam step, spin estimators, texture E_loc and "
