@@ -99,7 +99,10 @@ __global__ __launch_bounds__(256) void k_wgrad_cols(WgradArgs<T> a, int ib) {
 // workgroup sized to the layer width (a 128-wide layer takes one wave, not half of four).  Taken
 // when every segment offset, the segment width and the strides are even, so a column pair never
 // straddles two leaves and the stores stay 8-byte aligned.  Each column's sum runs in the same
-// order as in k_wgrad_cols.
+// order as in k_wgrad_cols.  NPT bounds the padded token count (16 or 32), so the cotangents take
+// NPT registers per column; four rows i are swept per iteration (eight independent sums in
+// flight), and O goes out with non-temporal stores (nothing reads it before the MinSR passes).
+template <int NPT>
 __global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib) {
   typedef float f2 __attribute__((ext_vector_type(2)));
   extern __shared__ char smem_raw[];
@@ -118,9 +121,9 @@ __global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib)
   const int j = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (j >= a.out) return;
   const float* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg + j;
-  float g0[32], g1[32];
+  float g0[NPT], g1[NPT];
 #pragma unroll
-  for (int t = 0; t < 32; ++t) {
+  for (int t = 0; t < NPT; ++t) {
     const f2 v = t < N ? *reinterpret_cast<const f2*>(G + (int64_t)t * a.ldg) : (f2)0.f;
     g0[t] = v[0];
     g1[t] = v[1];
@@ -128,17 +131,32 @@ __global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib)
   const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
   const int sg = j / a.seg_w, jj = j % a.seg_w;
   float* O = a.O + row * a.ldo + a.seg_base[sg] + jj;
-  for (int ii = 0; ii < nrow; ++ii) {
+  auto rowsum = [&](int ii, float& s0, float& s1) {
     const float* xr = Xt + ii * NP;
-    float s0 = 0.f, s1 = 0.f;
+    s0 = 0.f;
+    s1 = 0.f;
 #pragma unroll
-    for (int t = 0; t < 32; t += 4) {
+    for (int t = 0; t < NPT; t += 4) {
       if (t < NP) {
         s0 += xr[t] * g0[t] + xr[t + 1] * g0[t + 1] + xr[t + 2] * g0[t + 2] + xr[t + 3] * g0[t + 3];
         s1 += xr[t] * g1[t] + xr[t + 1] * g1[t + 1] + xr[t + 2] * g1[t + 2] + xr[t + 3] * g1[t + 3];
       }
     }
-    *reinterpret_cast<f2*>(O + (int64_t)(i0 + ii) * a.seg_w) = (f2){s0, s1};
+  };
+  int ii = 0;
+  for (; ii + 4 <= nrow; ii += 4) {
+    float s[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rowsum(ii + q, s[q][0], s[q][1]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      __builtin_nontemporal_store((f2){s[q][0], s[q][1]},
+                                  reinterpret_cast<f2*>(O + (int64_t)(i0 + ii + q) * a.seg_w));
+  }
+  for (; ii < nrow; ++ii) {
+    float s0, s1;
+    rowsum(ii, s0, s1);
+    __builtin_nontemporal_store((f2){s0, s1}, reinterpret_cast<f2*>(O + (int64_t)(i0 + ii) * a.seg_w));
   }
 }
 
@@ -164,9 +182,11 @@ int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
     if constexpr (sizeof(T) == 4) {
       if (wgrad_pairs_ok(a)) {
         const int threads = (int)std::min<int64_t>(256, round_up(cdiv(a.out, 2), 64));
-        hipLaunchKernelGGL(k_wgrad_cols2, dim3((unsigned)cdiv(a.out, 2 * threads), (unsigned)(a.n_cfg * 2),
-                                               (unsigned)cdiv(a.in, ib)),
-                           dim3(threads), lds, st, a, ib);
+        const dim3 grid((unsigned)cdiv(a.out, 2 * threads), (unsigned)(a.n_cfg * 2), (unsigned)cdiv(a.in, ib));
+        if (a.N <= 16)
+          hipLaunchKernelGGL(k_wgrad_cols2<16>, grid, dim3(threads), lds, st, a, ib);
+        else
+          hipLaunchKernelGGL(k_wgrad_cols2<32>, grid, dim3(threads), lds, st, a, ib);
         NQS_LAUNCH_CHECK();
         return NQS_OK;
       }
