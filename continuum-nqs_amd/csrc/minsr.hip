@@ -483,25 +483,41 @@ __global__ __launch_bounds__(512) void k_syrk_bf16_256(GramBatch gb, int64_t ldc
     // stage kb + 1 into the other buffer (its last readers passed the barrier that ended kb - 1)
     if (kb + 1 < nk) NQS_G2_DL(cur ^ 1, kb + 1)
     const char* base = lds + cur * 65536;
-#pragma unroll
-    for (int kk = 0; kk < G2_BK; kk += 32) {
-      const int slot = ((kk / 8 + (lane >> 4)) ^ (lane & 7)) * 16;
-      bf8v a[4], b[8];
+    // the fragments of both 32-deep halves in their own registers; the second half's 12 reads go out
+    // one after each of the first half's first 12 MFMAs, so their LDS latency hides behind them
+    bf8v a[2][4], b[2][8];
+    auto rd = [&](int h) {
+      const int slot = ((4 * h + (lane >> 4)) ^ (lane & 7)) * 16;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        a[i] = *reinterpret_cast<const bf8v*>(base + (wm * 64 + i * 16 + (lane & 15)) * 128 + slot);
+        a[h][i] = *reinterpret_cast<const bf8v*>(base + (wm * 64 + i * 16 + (lane & 15)) * 128 + slot);
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        b[j] = *reinterpret_cast<const bf8v*>(base + 32768 + (wn * 128 + j * 16 + (lane & 15)) * 128 + slot);
-      // raised wave priority while this wave issues its MFMA burst: the waves that are between
-      // bursts (LDS reads, staging) yield issue slots to it
-      __builtin_amdgcn_s_setprio(1);
+        b[h][j] = *reinterpret_cast<const bf8v*>(base + 32768 + (wn * 128 + j * 16 + (lane & 15)) * 128 + slot);
+    };
+    auto mm = [&](int h) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-      __builtin_amdgcn_s_setprio(0);
+        for (int j = 0; j < 8; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][i], b[h][j], acc[i][j], 0, 0, 0);
+    };
+    rd(0);
+    __builtin_amdgcn_sched_barrier(0);
+    // raised wave priority while this wave issues its MFMA bursts: the waves that are between
+    // bursts (LDS reads, staging) yield issue slots to it
+    __builtin_amdgcn_s_setprio(1);
+    rd(1);
+    mm(0);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
     }
+    __builtin_amdgcn_sched_group_barrier(0x008, 20, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(1);
+    __builtin_amdgcn_s_setprio(0);
     // this wave's loads of stage kb + 1 have landed; the barrier makes every wave's visible and
     // ends the reads of stage kb
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
