@@ -1190,47 +1190,19 @@ __host__ __device__ inline size_t rowdet_lds(int G, int N, int n_g, int kind) {
   return a + (n_g <= kPhaseMax ? (size_t)G * n_g * 16 * sizeof(cd) : 0);
 }
 
-template <typename T, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 3 : 4)))
-void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
-                  int64_t w0, const T* __restrict__ bf, cd* __restrict__ resg) {
-  extern __shared__ __align__(16) char smem[];
-  const int N = j.N, K = p.K, nt = j.nt, rs_ = nt + 2, C = j.C;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x, nwaves = nthr >> 6;
-  const int64_t cfg0 = (int64_t)blockIdx.x * G;
-  const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
-  cd* env = reinterpret_cast<cd*>(smem);
-  cd* dcol = env + G * 272;
-  double* pos = reinterpret_cast<double*>(dcol + (KIND == 1 ? G * 32 : 0));
-  int* spin = reinterpret_cast<int*>(pos + G * 32);
-  int* flp = spin + G * 16;
-  const size_t head = (size_t)G * (272 + (KIND == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
-                      (size_t)G * 17 * sizeof(int);
-  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
-  const int n_g = p.n_g;
-  const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
-  cd* wtab = xrows + 256;
-  cd* un = wtab + (wlds ? N * n_g * 2 : 0);
-  if (wlds)
-    for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
-  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
-  const int r = lane & 15;
-  const int t = 4 * wave + (lane >> 4);
-  const int cl = t / K, k = t % K;
-  const bool act = cl < ng;  // the same for the 16 lanes of a group
-  const T* bfc0 = bf + (cfg0 + (act ? cl : 0)) * C * chs;
-  // row r of bf; the value-only pass loads it before the envelope is built so that its latency
-  // overlaps that work (the spin-flip pass has no registers to spare there)
-  T bre[16], bim[16];
-  auto load_row = [&]() {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const T* q = bfc0 + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
-      bre[i] = act ? q[0] : (T)0;
-      bim[i] = act ? q[KN] : (T)0;
-    }
-  };
-  if constexpr (KIND == 0) load_row();
+// Shared prologue of the row-per-lane heads: positions and spins of the workgroup's ng
+// configurations (the flipped electron's spin inverted in the spin-flip pass, mode 2), the
+// plane-wave phases (when n_g <= kPhaseMax) and the envelope env[c][a][i] in LDS, and for KIND 1
+// the flipped column's d env / dx, dy.  wtab: the envelope weights in LDS, or null to read them
+// from p.wenv.  Ends with a workgroup barrier.
+template <int KIND>
+__device__ __forceinline__ void rowdet_envelope(const JetGeom& j, const DetParams& p, int64_t cfg0, int ng,
+                                                const double* __restrict__ R, const int32_t* __restrict__ S,
+                                                int64_t w0, const cd* wtab, cd* un, cd* env, cd* dcol, double* pos,
+                                                int* spin, int* flp) {
+  const int N = j.N, n_g = p.n_g;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const bool wlds = wtab != nullptr;
   for (int e = tid; e < ng * 16; e += nthr) {
     const int c = e >> 4, t = e & 15;
     const int64_t cfg = cfg0 + c;
@@ -1294,6 +1266,50 @@ void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, c
     }
   }
   __syncthreads();
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 3 : 4)))
+void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
+                  int64_t w0, const T* __restrict__ bf, cd* __restrict__ resg) {
+  extern __shared__ __align__(16) char smem[];
+  const int N = j.N, K = p.K, nt = j.nt, rs_ = nt + 2, C = j.C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x, nwaves = nthr >> 6;
+  const int64_t cfg0 = (int64_t)blockIdx.x * G;
+  const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
+  cd* env = reinterpret_cast<cd*>(smem);
+  cd* dcol = env + G * 272;
+  double* pos = reinterpret_cast<double*>(dcol + (KIND == 1 ? G * 32 : 0));
+  int* spin = reinterpret_cast<int*>(pos + G * 32);
+  int* flp = spin + G * 16;
+  const size_t head = (size_t)G * (272 + (KIND == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
+                      (size_t)G * 17 * sizeof(int);
+  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
+  const int n_g = p.n_g;
+  const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
+  cd* wtab = xrows + 256;
+  cd* un = wtab + (wlds ? N * n_g * 2 : 0);
+  if (wlds)
+    for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
+  const int r = lane & 15;
+  const int t = 4 * wave + (lane >> 4);
+  const int cl = t / K, k = t % K;
+  const bool act = cl < ng;  // the same for the 16 lanes of a group
+  const T* bfc0 = bf + (cfg0 + (act ? cl : 0)) * C * chs;
+  // row r of bf; the value-only pass loads it before the envelope is built so that its latency
+  // overlaps that work (the spin-flip pass has no registers to spare there)
+  T bre[16], bim[16];
+  auto load_row = [&]() {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const T* q = bfc0 + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
+      bre[i] = act ? q[0] : (T)0;
+      bim[i] = act ? q[KN] : (T)0;
+    }
+  };
+  if constexpr (KIND == 0) load_row();
+  rowdet_envelope<KIND>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, dcol, pos, spin, flp);
   if constexpr (KIND == 1) load_row();
   if (act) {
     const cd* envc = env + cl * 272;
@@ -1917,6 +1933,41 @@ template int launch_det_jet<float>(const JetGeom&, const DetParams&, const doubl
 // The backflow Dense has 2KN real outputs y = [Re bf | Im bf]; its complex cotangent is
 // [dL/dbf, i dL/dbf], stored as two real channels (Re, Im).  Complex leaves (det_logw, dW)
 // give O(re part) = dL/dtheta and O(im part) = i dL/dtheta.
+// O entry q (< 2 nk, nk = n_knots for the RBF Jastrow, 1 for the cusp) of the Jastrow leaves
+// for one configuration (pos [N][2], spin [N]): q < nk the same-spin knot q (the cusp's loga),
+// else the opposite-spin knot q - nk.  Returns the entry's theta offset, -1 when it has no leaf
+// (the cusp has no opposite-spin entry).
+__device__ __forceinline__ int64_t jastrow_o_entry(const DetParams& p, const DetBwdOffsets& off, int N,
+                                                   const double* pos, const int* spin, int q, double& v) {
+  const int nk = p.jastrow_kind == 0 ? p.n_knots : 1;
+  const bool same_leaf = q < nk;
+  const int kk = q % nk;
+  v = 0;
+  if (p.jastrow_kind != 0 && !same_leaf) return -1;
+  for (int a = 0; a < N; ++a)
+    for (int b = a + 1; b < N; ++b) {
+      const bool same = spin[a] == spin[b];
+      const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
+      double r, grx, gry, lapr;
+      pair_dist(p, dx, dy, r, grx, gry, lapr);
+      if (p.jastrow_kind == 0) {
+        if (same != same_leaf) continue;
+        const double mu = (p.n_knots > 1) ? p.rbf_rcut * (double)kk / (double)(p.n_knots - 1) : 0.0;
+        const double z = (r - mu) / p.rbf_w;
+        v += exp(-z * z);
+      } else {
+        const double la = p.jas[0];
+        const double aa = (la > 0 ? la + log1p(exp(-la)) : log1p(exp(la))) + 0.05;
+        const double sig = 1.0 / (1.0 + exp(-la));
+        const double al = same ? p.cusp / 3.0 : p.cusp;
+        const double e = exp(-r / aa);
+        v += (al * (1.0 - e) - al * (r / aa) * e) * sig;
+      }
+    }
+  if (p.jastrow_kind == 0) return same_leaf ? off.jas_same + kk : off.jas_opp + kk;
+  return off.jas_loga;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __restrict__ R,
                                                  const int32_t* __restrict__ S, int64_t w0, const T* __restrict__ bf,
@@ -2057,40 +2108,161 @@ __global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __re
   if (p.use_jastrow) {
     const int nk = p.jastrow_kind == 0 ? p.n_knots : 1;
     for (int q = tid; q < 2 * nk; q += nthr) {
-      // q < nk: same-spin knot (or cusp loga), else opposite-spin knot
-      const bool same_leaf = q < nk;
-      const int kk = q % nk;
-      double v = 0;
-      for (int a = 0; a < N; ++a)
-        for (int b = a + 1; b < N; ++b) {
-          const bool same = spin[a] == spin[b];
-          const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-          double r, grx, gry, lapr;
-          pair_dist(p, dx, dy, r, grx, gry, lapr);
-          if (p.jastrow_kind == 0) {
-            if (same != same_leaf) continue;
-            const double mu = (p.n_knots > 1) ? p.rbf_rcut * (double)kk / (double)(p.n_knots - 1) : 0.0;
-            const double z = (r - mu) / p.rbf_w;
-            v += exp(-z * z);
-          } else {
-            if (!same_leaf) continue;
-            const double la = p.jas[0];
-            const double aa = (la > 0 ? la + log1p(exp(-la)) : log1p(exp(la))) + 0.05;
-            const double sig = 1.0 / (1.0 + exp(-la));
-            const double al = same ? p.cusp / 3.0 : p.cusp;
-            const double e = exp(-r / aa);
-            v += (al * (1.0 - e) - al * (r / aa) * e) * sig;
-          }
-        }
-      int64_t o;
-      if (p.jastrow_kind == 0)
-        o = same_leaf ? off.jas_same + kk : off.jas_opp + kk;
-      else {
-        if (!same_leaf) continue;
-        o = off.jas_loga;
-      }
+      double v;
+      const int64_t o = jastrow_o_entry(p, off, N, pos, spin, q, v);
+      if (o < 0) continue;
       Ore[o] = (T)v;
       Oim[o] = (T)0;
+    }
+  }
+}
+
+// The same backward, row per lane, for N <= 16 without a learnable envelope (no dW leaf): four
+// determinants per wave as in k_det_rows16 (determinant t = 4 wave + lane / 16 of the workgroup is
+// k = t % K of configuration t / K), each factored by Gauss-Jordan (KIND 1), after which lane
+// r = p_q holds row q of M^-1 with its columns in pivot order, so dL/dbf[k,a,q] =
+// p_k env[a,q] M^-1[q,a] is lane-local.  The log dets reach the configuration's other
+// determinants through LDS for the log-sum-exp weights p_k.  LDS: rowdet_lds(G, N, n_g, 0), then
+// the log dets [G][16].
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_det_bwd_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R,
+                                                        const int32_t* __restrict__ S, int64_t w0,
+                                                        const T* __restrict__ bf, T* __restrict__ ybar,
+                                                        double* __restrict__ out_lp, T* __restrict__ O, int64_t ldo,
+                                                        int64_t W_total, DetBwdOffsets off) {
+  extern __shared__ __align__(16) char smem[];
+  const int N = j.N, K = p.K, n_g = p.n_g;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x;
+  const int64_t cfg0 = (int64_t)blockIdx.x * G;
+  const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
+  cd* env = reinterpret_cast<cd*>(smem);
+  double* pos = reinterpret_cast<double*>(env + G * 272);
+  int* spin = reinterpret_cast<int*>(pos + G * 32);
+  int* flp = spin + G * 16;
+  const size_t head = (size_t)G * 272 * sizeof(cd) + (size_t)G * 32 * sizeof(double) + (size_t)G * 17 * sizeof(int);
+  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
+  const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
+  cd* wtab = xrows + 256;
+  cd* un = wtab + (wlds ? N * n_g * 2 : 0);
+  cd* lds_ld = reinterpret_cast<cd*>(smem + rowdet_lds(G, N, n_g, 0));
+  if (wlds)
+    for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
+  const int64_t KN = (int64_t)K * N, rowlen = 2 * KN;
+  const int r = lane & 15;
+  const int t = 4 * wave + (lane >> 4);
+  const int cl = t / K, k = t % K;
+  const bool act = cl < ng;  // the same for the 16 lanes of a group
+  const int64_t cfg = cfg0 + (act ? cl : 0);
+  const T* bfc = bf + cfg * (int64_t)N * rowlen;
+  // row r of M's backflow factor, loaded before the envelope is built so that its latency overlaps
+  // that work
+  T bre[16], bim[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const T* q = bfc + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
+    bre[i] = act ? q[0] : (T)0;
+    bim[i] = act ? q[KN] : (T)0;
+  }
+  rowdet_envelope<0>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, nullptr, pos, spin, flp);
+  const cd* envc = env + cl * 272;
+  RowDet f;
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      f.m[i] = (r < N && i < N) ? envc[r * 17 + i] * mk(1.0 + (double)bre[i], (double)bim[i])
+                                : mk(r == i ? 1.0 : 0.0, 0);
+    f.piv = mk(1, 0);
+    f.q = r;
+    f.pk0 = f.pk1 = 0;
+    f.used = f.zero = false;
+    RowDetSteps<0, 1>::run(f, r, xrows + 16 * (lane >> 4) + 64 * wave, N);
+    const cd ld = rowdet_logdet(f, r, N);
+    if (r == 0) lds_ld[cl * 16 + k] = ld;
+  }
+  __syncthreads();
+  if (act) {
+    // log-sum-exp over the configuration's determinants, in determinant order
+    const cd* lz = lds_ld + cl * 16;
+    cd L, pk;
+    if (K == 1) {
+      L = lz[0];
+      pk = mk(1, 0);
+    } else {
+      double amax = -INFINITY;
+      for (int q = 0; q < K; ++q) {
+        const cd z = lz[q] + mk(p.det_logw[2 * q], p.det_logw[2 * q + 1]);
+        amax = z.re > amax ? z.re : amax;
+      }
+      if (!isfinite(amax)) amax = 0.0;
+      cd sum = mk(0, 0);
+      for (int q = 0; q < K; ++q) sum = sum + cexp_(lz[q] + mk(p.det_logw[2 * q], p.det_logw[2 * q + 1]) - mk(amax, 0));
+      L = clog_(sum) + mk(amax, 0);
+      pk = cexp_(lz[k] + mk(p.det_logw[2 * k], p.det_logw[2 * k + 1]) - L);
+    }
+    const int64_t w = w0 + cfg;
+    if (k == 0) {
+      double J = 0;
+      if (p.use_jastrow) {
+        const double* pc = pos + cl * 32;
+        const int* sc = spin + cl * 16;
+        double Ja = 0;
+        if (r < N)
+          for (int b = r + 1; b < N; ++b) {
+            double d, grx, gry, lapr, u, u1, u2;
+            pair_dist(p, pc[2 * r] - pc[2 * b], pc[2 * r + 1] - pc[2 * b + 1], d, grx, gry, lapr);
+            jastrow_u(p, sc[r] == sc[b], d, u, u1, u2);
+            Ja += u;
+          }
+        J = group_sum_d(Ja);
+      }
+      if (out_lp && r == 0) {
+        out_lp[2 * cfg] = L.re + J;
+        out_lp[2 * cfg + 1] = L.im;
+      }
+    }
+    if (off.dlw >= 0 && r == 0) {
+      T* Ore = O + w * ldo;
+      T* Oim = O + (W_total + w) * ldo;
+      Ore[off.dlw + k] = (T)pk.re;
+      Oim[off.dlw + k] = (T)pk.im;
+      Ore[off.dlw + K + k] = (T)(-pk.im);
+      Oim[off.dlw + K + k] = (T)pk.re;
+    }
+    // backflow cotangent of row i = q of M^-1: [Re, Im] channels of dL/dbf[k,a,i]
+    T* y0 = ybar + (cfg * 2 + 0) * (int64_t)N * rowlen;
+    T* y1 = ybar + (cfg * 2 + 1) * (int64_t)N * rowlen;
+    if (f.zero) {
+      // singular M: no inverse; the cotangent of this determinant is NaN, lane r < N taking row r
+      if (r < N)
+        for (int a = 0; a < N; ++a) {
+          const int64_t col = (int64_t)r * rowlen + k * N + a;
+          y0[col] = y0[col + KN] = y1[col] = y1[col + KN] = (T)NAN;
+        }
+    } else if (f.used) {
+      const int i = f.q;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int a = rowdet_pk(f, s);
+        if (s >= N || a >= N) continue;
+        const cd g = envc[a * 17 + i] * (pk * f.m[s]);
+        const int64_t col = (int64_t)i * rowlen + k * N + a;
+        y0[col] = (T)g.re;
+        y0[col + KN] = (T)(-g.im);
+        y1[col] = (T)g.im;
+        y1[col + KN] = (T)g.re;
+      }
+    }
+  }
+  if (p.use_jastrow) {
+    const int nk = p.jastrow_kind == 0 ? p.n_knots : 1;
+    for (int e = tid; e < ng * 2 * nk; e += nthr) {
+      const int c = e / (2 * nk), q = e % (2 * nk);
+      double v;
+      const int64_t o = jastrow_o_entry(p, off, N, pos + c * 32, spin + c * 16, q, v);
+      if (o < 0) continue;
+      const int64_t w = w0 + cfg0 + c;
+      O[w * ldo + o] = (T)v;
+      O[(W_total + w) * ldo + o] = (T)0;
     }
   }
 }
@@ -2101,6 +2273,18 @@ int launch_det_bwd(int64_t n_cfg, const DetParams& p, const double* R, const int
                    hipStream_t st) {
   if (n_cfg == 0) return NQS_OK;
   const int N = p.N, NN = N * N;
+  if (N <= 16 && p.K <= 16 && off.dW < 0) {
+    const int G = std::max(1, std::min(4, 16 / p.K));
+    const int waves = std::min(4, (G * p.K + 3) / 4);
+    const size_t lds = rowdet_lds(G, N, p.n_g, 0) + (size_t)G * 16 * sizeof(cd);
+    if (lds <= 64 * 1024) {
+      const JetGeom j{n_cfg, N, 1, 0, 0, 0};
+      hipLaunchKernelGGL(k_det_bwd_rows16<T>, dim3((unsigned)cdiv(n_cfg, G)), dim3(64 * waves), lds, st, j, p, G, R,
+                         S, w0, bf, ybar, out_lp, O, ldo, W_total, off);
+      NQS_LAUNCH_CHECK();
+      return NQS_OK;
+    }
+  }
   size_t lds = sizeof(cd) * (size_t)(4 * NN + p.K) + sizeof(double) * 2 * N + sizeof(int) * (2 * N + 4);
   lds = round_up((int64_t)lds, 16);
   NQS_REQUIRE(lds <= 160 * 1024, NQS_ERR_UNSUPPORTED, "determinant backward exceeds LDS");
