@@ -255,7 +255,7 @@ static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
   w.T32 = d->gram != NQS_GRAM_FP64 ? (float*)grab(n * n * 4) : nullptr;
   w.A64 = (double*)grab(n * n * 8);
   w.Linv = (double*)grab(n * 64 * 8);
-  w.panel = (double*)grab(n * 64 * 8);
+  w.panel = (double*)grab(cholesky_panel_elems(n) * 8);
   w.et = (double*)grab(n * 8);
   w.y = (double*)grab(n * 8);
   w.r = (double*)grab(n * 8);
@@ -402,7 +402,7 @@ static MinsrBatchWs minsr_batch_ws(const nqs_minsr_desc* d, int nsys, char* base
   };
   w.A64 = (double*)grab(nsys * n * n * 8);
   w.Linv = (double*)grab(nsys * n * 64 * 8);
-  w.panel = (double*)grab(nsys * n * 64 * 8);
+  w.panel = (double*)grab(nsys * cholesky_panel_elems(n) * 8);
   w.Y = (double*)grab(nsys * n * 8);
   w.Rr = (double*)grab(nsys * n * 8);
   w.T32 = d[0].gram != NQS_GRAM_FP64 ? (float*)grab(nsys * n * n * 4) : nullptr;
@@ -478,7 +478,7 @@ static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov
     }
     NQS_TRY(deflate_centring(A, n, W, d[s].lam, w.sys[s].ts, w.sys[s].sc, st));
   }
-  NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st, nsys, n * n, n * 64, n * 64));
+  NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st, nsys, n * n, n * 64, cholesky_panel_elems(n)));
   for (int s = 0; s < nsys; ++s)
     NQS_CHECK_HIP(hipMemcpyAsync(w.Y + s * n, w.sys[s].et, n * 8, hipMemcpyDeviceToDevice, st));
   NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Y, st, nsys, n * n, n * 64, n));

@@ -40,7 +40,8 @@ int to_f64_shift(const T* Tin, int64_t ldt, int64_t n, double lam, double* A, in
 int64_t defl_ts_elems(int64_t n);
 int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, double* sc, hipStream_t st);
 // nb same-size systems share each launch: system z uses A + z sA, Linv + z sL, panel + z sP,
-// info + z (x + z sx in the solve)
+// info + z (x + z sx in the solve).  panel: cholesky_panel_elems(n) doubles per system.
+int64_t cholesky_panel_elems(int64_t n);
 int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st, int nb = 1,
              int64_t sA = 0, int64_t sL = 0, int64_t sP = 0);
 int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb = 1,

@@ -866,15 +866,23 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64
   for (int r = 0; r < CH_NB; ++r) Linv[r * CH_NB + c] = x[r];
 }
 
-__global__ void k_copy_panel(const double* __restrict__ P, int64_t m, int nb, double* __restrict__ A, int64_t lda,
-                             int64_t r0, int64_t c0, int64_t sP, int64_t sA) {
+// The factor's columns J0 .. J0 + nbo - 1 below their diagonal blocks, from the outer panel buffer
+// P ([n - J0][CH_NBO], row r - J0, column c - J0) into A: element (r, c) with r >= (start of c's
+// 64-column block) + 64.  One thread per element of rows J0 + 64 .. n - 1.
+__global__ void k_copy_outer_panel(const double* __restrict__ P, int64_t n, int64_t J0, int nbo,
+                                   double* __restrict__ A, int64_t lda, int64_t sP, int64_t sA) {
   P += blockIdx.z * sP;
   A += blockIdx.z * sA;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= m * nb) return;
-  const int64_t r = q / nb, c = q % nb;
-  A[(r0 + r) * lda + c0 + c] = P[q];
+  const int64_t rows = n - J0 - CH_NB;
+  if (q >= rows * nbo) return;
+  const int64_t rr = CH_NB + q / nbo;  // row - J0
+  const int cc = (int)(q % nbo);       // column - J0
+  if (rr < (cc & ~(CH_NB - 1)) + CH_NB) return;
+  A[(J0 + rr) * lda + J0 + cc] = P[rr * CH_NBO + cc];
 }
+
+int64_t cholesky_panel_elems(int64_t n) { return n * CH_NBO; }
 
 static GemmArgs<double> g64(const double* A, const double* B, double* C, int64_t M, int64_t N, int64_t K, int64_t lda,
                             int64_t ldb, int64_t ldc, double alpha, double beta, int tri) {
@@ -968,20 +976,25 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
       NQS_LAUNCH_CHECK();
       const int64_t m = n - j0 - CH_NB;
       if (m == 0) break;
-      // panel = A21 Linv^T  (NT GEMM with B = Linv, K = 64)
+      // the block's factor column A21 Linv^T (NT GEMM with B = Linv, K = 64) into the outer panel
+      // buffer, rows j0 + 64 .. n - 1 and columns j0 .. j0 + 63 relative to J0
+      double* pb = panel + (j0 + CH_NB - J0) * CH_NBO + (j0 - J0);
       NQS_TRY(gemm_nt<double>(
-          strided(g64(A + (j0 + CH_NB) * lda + j0, Li, panel, m, CH_NB, CH_NB, lda, CH_NB, CH_NB, 1.0, 0.0, 0), sA,
-                  sL, sP),
+          strided(g64(A + (j0 + CH_NB) * lda + j0, Li, pb, m, CH_NB, CH_NB, lda, CH_NB, CH_NBO, 1.0, 0.0, 0), sA, sL,
+                  sP),
           nb, st));
-      // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= panel panel[0:w]^T (lower part)
+      // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= pb pb[0:w]^T (lower part)
       const int64_t w = J0 + nbo - (j0 + CH_NB);
       if (w > 0)
-        NQS_TRY(gemm_nt<double>(strided(g64(panel, panel, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NB,
-                                            CH_NB, lda, -1.0, 1.0, 1),
+        NQS_TRY(gemm_nt<double>(strided(g64(pb, pb, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NBO,
+                                            CH_NBO, lda, -1.0, 1.0, 1),
                                         sP, sP, sA),
                                 nb, st));
-      hipLaunchKernelGGL(k_copy_panel, dim3((unsigned)cdiv(m * CH_NB, 256), 1, (unsigned)nb), dim3(256), 0, st, panel,
-                         m, CH_NB, A, lda, j0 + CH_NB, j0, sP, sA);
+    }
+    // the outer panel's factor columns into A, one launch per outer panel
+    if (n - J0 > CH_NB) {
+      hipLaunchKernelGGL(k_copy_outer_panel, dim3((unsigned)cdiv((n - J0 - CH_NB) * nbo, 256), 1, (unsigned)nb),
+                         dim3(256), 0, st, panel, n, J0, (int)nbo, A, lda, sP, sA);
       NQS_LAUNCH_CHECK();
     }
     // trailing update with the whole outer panel, A22 -= Lp Lp^T (lower tiles), K = nbo: first the
