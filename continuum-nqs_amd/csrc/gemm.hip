@@ -101,7 +101,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_v2(GemmArgs<T> g) {
   typedef typename Vec16<T>::t vt;
   __shared__ T As[2][BM][LD];
   __shared__ T Bs[2][BM][LD];
-  const int bm = blockIdx.y, bn = blockIdx.x, bz = blockIdx.z;
+  int bm = blockIdx.y, bn = blockIdx.x;
+  const int bz = blockIdx.z;
+  if (g.tile_order == 1) {
+    const int64_t nt = cdiv_dev(g.M, (int64_t)BM), ntiles = supertile_count(nt), per_xcd = (ntiles + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (t >= ntiles) return;
+    supertile_of(t, bm, bn);
+    if (bm >= nt) return;
+  }
   if (g.tri == 1 && bn > bm) return;
   const T* A = g.A + (int64_t)bz * g.strideA;
   const T* B = g.B + (int64_t)bz * g.strideB;
@@ -534,10 +542,17 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   }
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
+    GemmArgs<T> gl = g;
+    gl.tile_order = 0;
+    if (g.tri == 1 && g.M == g.N && cdiv(g.M, 128) >= 16) {
+      // the lower triangle in super-tile order, 1-D grid
+      gl.tile_order = 1;
+      grid = dim3((unsigned)(8 * cdiv(supertile_count(cdiv(g.M, 128)), 8)), 1, (unsigned)batch);
+    }
     if constexpr (sizeof(T) == 8)
-      hipLaunchKernelGGL((k_gemm_nt_v2<T, 16>), grid, dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_gemm_nt_v2<T, 16>), grid, dim3(256), 0, st, gl);
     else
-      hipLaunchKernelGGL((k_gemm_nt_v2<T, 32>), grid, dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_gemm_nt_v2<T, 32>), grid, dim3(256), 0, st, gl);
     NQS_LAUNCH_CHECK();
     return NQS_OK;
   }

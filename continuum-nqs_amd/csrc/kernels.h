@@ -84,6 +84,10 @@ struct GemmArgs {
   int64_t bias_period;  // bias added to rows with (row % bias_period) < bias_len
   int64_t bias_len;
   int tri;     // 0 full, 1 lower-triangular tiles only (col <= row)
+  // set by the launcher: 1 = a square lower triangle (tri, M == N) on a 1-D grid in super-tile
+  // order, each XCD taking a contiguous stretch of it, so that an XCD's concurrent tiles share
+  // row and column panels in its L2
+  int tile_order = 0;
   int mirror;  // also store C[col][row]
   int split;   // float only: 0 float32 MFMA, 1 bf16 MFMA, 3 bf16x3 (hi/lo split) MFMA
   // Optional M x ldc buffer the tile-shape autotuner may overwrite: with beta != 0 the trial
@@ -200,6 +204,35 @@ __device__ __forceinline__ void ln_tangent(const T (&xv)[F], const T (&dx)[F], c
   const T drs = (T)(-0.5) * rs * rs * rs * dvar;
 #pragma unroll
   for (int e = 0; e < F; ++e) dy[e] = ((dx[e] - dmu) * rs + (xv[e] - mu) * drs) * sc[e];
+}
+
+// Lower-triangular tile grids in super-tile order: tiles grouped in SUPER x SUPER super-tiles (lower
+// triangle of super-tiles, row-major; a diagonal super-tile keeps its lower triangle), row-major
+// inside a super-tile.  Position t of the sequence -> tile (bm, bn) of the padded grid; positions
+// whose tile lies outside the lower triangle of the nt x nt grid are skipped by the caller.
+constexpr int SUPER = 8;
+__device__ inline void supertile_of(int64_t t, int& bm, int& bn) {
+  // super row I holds I full super-tiles (SUPER^2 tiles each) and one diagonal one
+  // (SUPER (SUPER + 1) / 2 tiles): prefix(I) = SUPER^2 I (I - 1) / 2 + SUPER (SUPER + 1) / 2 I
+  constexpr int64_t F = SUPER * SUPER, D = SUPER * (SUPER + 1) / 2;
+  int I = 0;
+  while (F * (int64_t)(I + 1) * I / 2 + D * (I + 1) <= t) ++I;
+  const int64_t o = t - (F * (int64_t)I * (I - 1) / 2 + D * I);
+  if (o < F * I) {
+    const int J = (int)(o / F), u = (int)(o % F);
+    bm = SUPER * I + u / SUPER;
+    bn = SUPER * J + u % SUPER;
+  } else {
+    const int u = (int)(o - F * I);  // lower triangle of the diagonal super-tile, row-major
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= u) ++r;
+    bm = SUPER * I + r;
+    bn = SUPER * I + (u - r * (r + 1) / 2);
+  }
+}
+__host__ __device__ inline int64_t supertile_count(int64_t nt) {
+  const int64_t ns = (nt + SUPER - 1) / SUPER;
+  return (int64_t)SUPER * SUPER * ns * (ns - 1) / 2 + (int64_t)SUPER * (SUPER + 1) / 2 * ns;
 }
 
 template <typename T>

@@ -381,32 +381,6 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __bf16* __restrict__ A,
 // block reading 12 panels, where a row-major walk of the triangle reads 33.
 #define G2_BM 256
 #define G2_BK 64
-// Position t of the super-tile sequence -> tile (bm, bn) of the padded grid; positions whose tile
-// lies outside the lower triangle of the nt x nt grid are skipped by the caller.
-constexpr int SUPER = 8;
-__device__ inline void supertile_of(int64_t t, int& bm, int& bn) {
-  // super row I holds I full super-tiles (SUPER^2 tiles each) and one diagonal one
-  // (SUPER (SUPER + 1) / 2 tiles): prefix(I) = SUPER^2 I (I - 1) / 2 + SUPER (SUPER + 1) / 2 I
-  constexpr int64_t F = SUPER * SUPER, D = SUPER * (SUPER + 1) / 2;
-  int I = 0;
-  while (F * (int64_t)(I + 1) * I / 2 + D * (I + 1) <= t) ++I;
-  const int64_t o = t - (F * (int64_t)I * (I - 1) / 2 + D * I);
-  if (o < F * I) {
-    const int J = (int)(o / F), u = (int)(o % F);
-    bm = SUPER * I + u / SUPER;
-    bn = SUPER * J + u % SUPER;
-  } else {
-    const int u = (int)(o - F * I);  // lower triangle of the diagonal super-tile, row-major
-    int r = 0;
-    while ((r + 1) * (r + 2) / 2 <= u) ++r;
-    bm = SUPER * I + r;
-    bn = SUPER * I + (u - r * (r + 1) / 2);
-  }
-}
-static int64_t supertile_count(int64_t nt) {
-  const int64_t ns = (nt + SUPER - 1) / SUPER;
-  return (int64_t)SUPER * SUPER * ns * (ns - 1) / 2 + (int64_t)SUPER * (SUPER + 1) / 2 * ns;
-}
 
 // Split of the last, partly filled round of tiles: tiles [t0, t0 + ntail) (super-tile order) are
 // cut into nsplit chunks of kchunk k-steps, one workgroup per (tile, chunk), each writing a
