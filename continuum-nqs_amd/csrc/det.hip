@@ -932,59 +932,82 @@ void k_det_jet16(JetGeom j, DetParams p, const double* __restrict__ R,
   det_head_tail<KIND>(j, p, res, rs_, pos, spin, cfg, flip, out_lp, out_tan, out_lap);
 }
 
-// ------------------------------------------------------------------ row-per-lane head, N <= 16
+// ------------------------------------------------------------------ row-per-lane heads, N <= 32
 // The value-only pass (KIND 0) and the spin-flip pass (KIND 1: log psi and the two tangents of the
-// flipped electron) with four determinants per wave: lane group g = lane / 16 holds one
-// determinant, lane r of the group row r of M (orbital r; electron i in slot i), 16 complex
-// entries in registers.  Rows never move.  The pivot of step q is the lane with the largest
-// |re| + |im| in column q among the rows not yet used (first lane on ties), found by DPP
-// rotations inside the group; its row reaches the other lanes by ds_bpermute.  The pivot lanes
-// p_q are packed four bits each, and the sign of the permutation q -> p_q is the parity of its
-// inversions.
+// flipped electron) with one determinant per group of NR lanes: NR = 16 for N <= 16 (four
+// determinants per wave), NR = 32 for 16 < N <= 32 (two per wave); lane r of the group holds row r
+// of M (orbital r; electron i in slot i), NR complex entries in registers.  Rows never move.  The
+// pivot of step q is the lane with the largest |re| + |im| in column q among the rows not yet used
+// (first lane on ties), found by DPP rotations inside each 16-lane row (and one exchange between
+// the two rows of a 32-lane group); its row reaches the other lanes through LDS.  The pivot lanes
+// p_q are packed four bits each (NR = 16) or five (NR = 32), and the sign of the permutation
+// q -> p_q is the parity of its inversions.
 //   KIND 0  elimination below the pivots; log det = sum of log pivot + the permutation's sign.
 //   KIND 1  Gauss-Jordan in place: after step q, slot q of lane r holds E[r][p_q], E the product
 //           of the row operations (E M = P with P[p_q][q] = 1), so M^-1[q][c] = E[p_q][c]: lane
 //           r = p_q holds row q of M^-1 with its columns in pivot order, and
 //           tr(M^-1 dM) = sum_r sum_s slot_s(r) dM[p_s][q(r)], a per-lane dot product with dM
 //           gathered along the pivot order.
-// Steps q >= N are skipped: the identity padding's pivots are exactly 1 and its rows and columns
-// take no part in the real block.  The multiplier of a step is the row entry times the pivot's
+// Steps q >= N are skipped: the identity padding (to NR) has pivots exactly 1 and its rows and
+// columns take no part in the real block.  The multiplier of a step is the row entry times the pivot's
 // reciprocal conj(u) / |u|^2, as LAPACK's getf2 scales a column by 1 / u.  (The pivot search
 // compares magnitudes rounded to float: entries below 1e-38 in magnitude would compare as zero.)  A zero pivot leaves
 // the rows as they are (log det = -inf, as the LU of the N <= 16 head above), and the tangents of
 // a singular matrix are NaN, as its substitution gives them.
+template <int NR>
+struct RowPk {  // p_q packed BITS bits each, PER per 32-bit word, in NW words
+  static constexpr int BITS = NR == 16 ? 4 : 5, PER = NR == 16 ? 8 : 6, NW = (NR + PER - 1) / PER;
+};
+template <int NR>
 struct RowDet {
-  cd m[16];
-  cd piv;             // this lane's pivot (1 while the lane has not been one)
-  int q;              // the step at which this lane was the pivot
-  unsigned pk0, pk1;  // p_q: steps 0..7 in pk0, 8..15 in pk1, four bits each
+  cd m[NR];
+  cd piv;                       // this lane's pivot (1 while the lane has not been one)
+  int q;                        // the step at which this lane was the pivot
+  unsigned pk[RowPk<NR>::NW];   // p_q of steps PER w .. PER w + PER - 1 in pk[w]
   bool used, zero;
 };
 
-__device__ __forceinline__ double group_sum_d(double v) {  // over the 16 lanes of a row
+// Reductions over the NR lanes of a group, the same bits in every lane of it: DPP inside each
+// 16-lane row, then (NR = 32) the two rows combined in the same operand order in both.
+template <int NR = 16>
+__device__ __forceinline__ double group_sum_d(double v) {
   v += dpp_d<0xB1>(v);
   v += dpp_d<0x4E>(v);
   v += dpp_d<0x141>(v);
   v += dpp_d<0x140>(v);
+  if constexpr (NR == 32) {
+    const double o = __shfl_xor(v, 16);
+    v = (threadIdx.x & 16) ? o + v : v + o;
+  }
   return v;
 }
-__device__ __forceinline__ cd group_sum_cd(cd v) { return mk(group_sum_d(v.re), group_sum_d(v.im)); }
+template <int NR = 16>
+__device__ __forceinline__ cd group_sum_cd(cd v) { return mk(group_sum_d<NR>(v.re), group_sum_d<NR>(v.im)); }
+template <int NR = 16>
 __device__ __forceinline__ int group_sum_i(int v) {
   v += dpp_i<0xB1>(v);
   v += dpp_i<0x4E>(v);
   v += dpp_i<0x141>(v);
   v += dpp_i<0x140>(v);
+  if constexpr (NR == 32) v += __shfl_xor(v, 16);
   return v;
 }
+template <int NR = 16>
 __device__ __forceinline__ cd group_prod_cd(cd v) {
   v = v * mk(dpp_d<0xB1>(v.re), dpp_d<0xB1>(v.im));
   v = v * mk(dpp_d<0x4E>(v.re), dpp_d<0x4E>(v.im));
   v = v * mk(dpp_d<0x141>(v.re), dpp_d<0x141>(v.im));
   v = v * mk(dpp_d<0x140>(v.re), dpp_d<0x140>(v.im));
+  if constexpr (NR == 32) {
+    const cd o = mk(__shfl_xor(v.re, 16), __shfl_xor(v.im, 16));
+    v = (threadIdx.x & 16) ? o * v : v * o;
+  }
   return v;
 }
-__device__ __forceinline__ int rowdet_pk(const RowDet& f, int s) {
-  return (int)(((s < 8 ? f.pk0 : f.pk1) >> (4 * (s & 7))) & 15u);
+template <int NR>
+__device__ __forceinline__ int rowdet_pk(const RowDet<NR>& f, int s) {
+  typedef RowPk<NR> K_;
+  return (int)((f.pk[s / K_::PER] >> (K_::BITS * (s % K_::PER))) & (unsigned)(NR - 1));
 }
 
 // m - l x, two fused multiply-adds per component
@@ -1008,29 +1031,31 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(y, e, y);
 }
 
-// Step Q.  Pivot search on a 32-bit key: |re| + |im| rounded to float, its four low bits replaced
-// by 15 - r, so one unsigned max per DPP rotation finds the largest magnitude (to 2^-19 relative)
-// and, among equal keys, the first lane; rows already used have key 0.  The pivot lane writes its
-// row (slots Q.. for KIND 0, all for KIND 1) to the group's LDS row xrow and every lane of the
-// group reads it back: the lanes of a wave issue LDS operations in program order, and the next
-// step's write follows this step's reads in every lane.  The row update runs in chunks of four
-// slots so that only one chunk of the pivot row is held in registers at a time.
-template <int Q, int KIND>
-__device__ __forceinline__ void rowdet_step(RowDet& f, int r, cd* xrow) {
+// Step Q.  Pivot search on a 32-bit key: |re| + |im| rounded to float, its log2(NR) low bits
+// replaced by NR - 1 - r, so one unsigned max per DPP rotation finds the largest magnitude (to
+// 2^-19 relative for NR = 16, 2^-18 for NR = 32) and, among equal keys, the first lane; rows
+// already used have key 0.  The pivot lane writes its row (slots Q.. for KIND 0, all for KIND 1)
+// to the group's LDS row xrow and every lane of the group reads it back: the lanes of a wave issue
+// LDS operations in program order, and the next step's write follows this step's reads in every
+// lane.  The row update runs in chunks of four slots so that only one chunk of the pivot row is
+// held in registers at a time.
+template <int Q, int KIND, int NR>
+__device__ __forceinline__ void rowdet_step(RowDet<NR>& f, int r, cd* xrow) {
+  constexpr unsigned LOW = NR - 1;
   const float mag = (float)(fabs(f.m[Q].re) + fabs(f.m[Q].im));
-  unsigned key = f.used ? 0u : ((__float_as_uint(mag) & ~15u) | (unsigned)(15 - r));
+  unsigned key = f.used ? 0u : ((__float_as_uint(mag) & ~LOW) | (LOW - (unsigned)r));
   key = max(key, (unsigned)dpp_i<0x121>((int)key));
   key = max(key, (unsigned)dpp_i<0x122>((int)key));
   key = max(key, (unsigned)dpp_i<0x124>((int)key));
   key = max(key, (unsigned)dpp_i<0x128>((int)key));
-  const int p = 15 - (int)(key & 15u);  // the same in every lane of the group
+  if constexpr (NR == 32) key = max(key, (unsigned)__shfl_xor((int)key, 16));
+  const int p = (int)(LOW - (key & LOW));  // the same in every lane of the group
   const bool isp = r == p;
-  if (Q < 8) f.pk0 |= (unsigned)p << (4 * Q);
-  else f.pk1 |= (unsigned)p << (4 * (Q - 8));
+  f.pk[Q / RowPk<NR>::PER] |= (unsigned)p << (RowPk<NR>::BITS * (Q % RowPk<NR>::PER));
   constexpr int J0 = KIND == 0 ? Q : 0;
   if (isp) {
 #pragma unroll
-    for (int jj = J0; jj < 16; ++jj) xrow[jj] = f.m[jj];
+    for (int jj = J0; jj < NR; ++jj) xrow[jj] = f.m[jj];
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1047,7 +1072,7 @@ __device__ __forceinline__ void rowdet_step(RowDet& f, int r, cd* xrow) {
   if constexpr (KIND == 0) {
     const cd l = (f.used || isp) ? mk(0, 0) : f.m[Q] * rp;
 #pragma unroll
-    for (int jj = Q + 1; jj < 16; ++jj) {
+    for (int jj = Q + 1; jj < NR; ++jj) {
       f.m[jj] = cfms(f.m[jj], l, xrow[jj]);
       if (((jj - Q) & 3) == 0) __builtin_amdgcn_sched_barrier(0);
     }
@@ -1056,7 +1081,7 @@ __device__ __forceinline__ void rowdet_step(RowDet& f, int r, cd* xrow) {
     const double s = isp ? 0.0 : 1.0;
     const cd c = isp ? mk(-rp.re, -rp.im) : fr;
 #pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
+    for (int jj = 0; jj < NR; ++jj) {
       if (jj == Q) continue;
       f.m[jj] = cfms(mk(s * f.m[jj].re, s * f.m[jj].im), c, xrow[jj]);
       if ((jj & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -1066,30 +1091,31 @@ __device__ __forceinline__ void rowdet_step(RowDet& f, int r, cd* xrow) {
   f.used = f.used || isp;
 }
 
-template <int Q, int KIND>
+template <int Q, int KIND, int NR>
 struct RowDetSteps {
-  static __device__ __forceinline__ void run(RowDet& f, int r, cd* xrow, int N) {
+  static __device__ __forceinline__ void run(RowDet<NR>& f, int r, cd* xrow, int N) {
     if (Q >= N) return;
-    rowdet_step<Q, KIND>(f, r, xrow);
-    RowDetSteps<Q + 1, KIND>::run(f, r, xrow, N);
+    rowdet_step<Q, KIND, NR>(f, r, xrow);
+    RowDetSteps<Q + 1, KIND, NR>::run(f, r, xrow, N);
   }
 };
-template <int KIND>
-struct RowDetSteps<16, KIND> {
-  static __device__ __forceinline__ void run(RowDet&, int, cd*, int) {}
+template <int KIND, int NR>
+struct RowDetSteps<NR, KIND, NR> {
+  static __device__ __forceinline__ void run(RowDet<NR>&, int, cd*, int) {}
 };
 
 // log det (principal branch of the phase) from the pivots and the parity of q -> p_q
-__device__ __forceinline__ cd rowdet_logdet(const RowDet& f, int r, int N) {
+template <int NR>
+__device__ __forceinline__ cd rowdet_logdet(const RowDet<NR>& f, int r, int N) {
   int inv = 0;
   if (f.used) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) inv += (s < N && s > f.q && rowdet_pk(f, s) < r) ? 1 : 0;
+    for (int s = 0; s < NR; ++s) inv += (s < N && s > f.q && rowdet_pk(f, s) < r) ? 1 : 0;
   }
-  inv = group_sum_i(inv);
+  inv = group_sum_i<NR>(inv);
   const double au = cabs_(f.piv);
-  const double la = group_sum_d(log(au));
-  cd ph = group_prod_cd(au > 0 ? mk(f.piv.re / au, f.piv.im / au) : mk(1, 0));
+  const double la = group_sum_d<NR>(log(au));
+  cd ph = group_prod_cd<NR>(au > 0 ? mk(f.piv.re / au, f.piv.im / au) : mk(1, 0));
   if (inv & 1) ph = mk(-ph.re, -ph.im);
   return mk(la + log(hypot(ph.re, ph.im)), atan2(ph.im, ph.re));
 }
@@ -1174,20 +1200,23 @@ __device__ __forceinline__ void det_tail_lanes(const JetGeom& j, const DetParams
   }
 }
 
-// G configurations per workgroup, determinant t = 4 wave + lane / 16 of the workgroup is k = t % K
-// of configuration t / K.  Dynamic LDS (rowdet_lds): env [G][16][17] (row pitch 17 keeps the
-// per-lane row reads off one bank), the flipped column's d env / dx, dy [G][2][16] (KIND 1),
-// positions [G][32], spins [G][16], flip index [G], the pivot rows of the 16 groups [16][16], the
-// envelope weights [N][n_g][2] (when small) and the plane-wave phases [G][n_g][16].  Results per determinant go to resg [cfg][K][nt + 2]: log
-// det, then the tangents.
+// G configurations per workgroup, determinant t = (64 / NR) wave + lane / NR of the workgroup is
+// k = t % K of configuration t / K.  Dynamic LDS (rowdet_lds): env [G][NR][NR + 1] (row pitch
+// NR + 1 keeps the per-lane row reads off one bank), the flipped column's d env / dx, dy
+// [G][2][NR] (KIND 1), positions [G][2 NR], spins [G][NR], flip index [G], the pivot rows of the
+// 16 groups [16][NR], the envelope weights [N][n_g][2] (when small) and the plane-wave phases
+// [G][n_g][NR].  Results per determinant go to resg [cfg][K][nt + 2]: log det, then the tangents.
 // the envelope weights W[a][g][s] are copied to LDS when they take at most this many bytes
 constexpr int kRowDetWenvMax = 16384;
-__host__ __device__ inline size_t rowdet_lds(int G, int N, int n_g, int kind) {
-  const size_t head = (size_t)G * (272 + (kind == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
-                      (size_t)G * 17 * sizeof(int);
+__host__ __device__ inline size_t rowdet_head(int NR, int G, int kind) {
+  return (size_t)G * (NR * (NR + 1) + (kind == 1 ? 2 * NR : 0)) * sizeof(cd) + (size_t)G * 2 * NR * sizeof(double) +
+         (size_t)G * (NR + 1) * sizeof(int);
+}
+__host__ __device__ inline size_t rowdet_lds(int NR, int G, int N, int n_g, int kind) {
   const size_t wb = (size_t)N * n_g * 2 * sizeof(cd);
-  const size_t a = (head + 15) / 16 * 16 + 256 * sizeof(cd) + (wb <= (size_t)kRowDetWenvMax ? wb : 0);
-  return a + (n_g <= kPhaseMax ? (size_t)G * n_g * 16 * sizeof(cd) : 0);
+  const size_t a = (rowdet_head(NR, G, kind) + 15) / 16 * 16 + (size_t)16 * NR * sizeof(cd) +
+                   (wb <= (size_t)kRowDetWenvMax ? wb : 0);
+  return a + (n_g <= kPhaseMax ? (size_t)G * n_g * NR * sizeof(cd) : 0);
 }
 
 // Shared prologue of the row-per-lane heads: positions and spins of the workgroup's ng
@@ -1195,7 +1224,7 @@ __host__ __device__ inline size_t rowdet_lds(int G, int N, int n_g, int kind) {
 // plane-wave phases (when n_g <= kPhaseMax) and the envelope env[c][a][i] in LDS, and for KIND 1
 // the flipped column's d env / dx, dy.  wtab: the envelope weights in LDS, or null to read them
 // from p.wenv.  Ends with a workgroup barrier.
-template <int KIND>
+template <int KIND, int NR>
 __device__ __forceinline__ void rowdet_envelope(const JetGeom& j, const DetParams& p, int64_t cfg0, int ng,
                                                 const double* __restrict__ R, const int32_t* __restrict__ S,
                                                 int64_t w0, const cd* wtab, cd* un, cd* env, cd* dcol, double* pos,
@@ -1203,8 +1232,9 @@ __device__ __forceinline__ void rowdet_envelope(const JetGeom& j, const DetParam
   const int N = j.N, n_g = p.n_g;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const bool wlds = wtab != nullptr;
-  for (int e = tid; e < ng * 16; e += nthr) {
-    const int c = e >> 4, t = e & 15;
+  constexpr int EP = NR + 1;  // env row pitch
+  for (int e = tid; e < ng * NR; e += nthr) {
+    const int c = e / NR, t = e % NR;
     const int64_t cfg = cfg0 + c;
     int64_t w;
     int flip = -1;
@@ -1216,35 +1246,36 @@ __device__ __forceinline__ void rowdet_envelope(const JetGeom& j, const DetParam
     }
     if (t == 0) flp[c] = flip;
     if (t < N) {
-      pos[c * 32 + 2 * t] = R[(w * N + t) * 2];
-      pos[c * 32 + 2 * t + 1] = R[(w * N + t) * 2 + 1];
+      pos[c * 2 * NR + 2 * t] = R[(w * N + t) * 2];
+      pos[c * 2 * NR + 2 * t + 1] = R[(w * N + t) * 2 + 1];
       const int s = S[w * N + t];
-      spin[c * 16 + t] = (t == flip) ? 1 - s : s;
+      spin[c * NR + t] = (t == flip) ? 1 - s : s;
     }
   }
   __syncthreads();
   const bool tab = n_g <= kPhaseMax;
   if (tab) {
-    for (int e = tid; e < ng * n_g * 16; e += nthr) {
-      const int c = e / (n_g * 16), gg = (e >> 4) % n_g, i = e & 15;
+    for (int e = tid; e < ng * n_g * NR; e += nthr) {
+      const int c = e / (n_g * NR), gg = (e / NR) % n_g, i = e % NR;
       double sn = 0, cs = 0;
-      if (i < N) sincos(p.gs[2 * gg] * pos[c * 32 + 2 * i] + p.gs[2 * gg + 1] * pos[c * 32 + 2 * i + 1], &sn, &cs);
+      if (i < N)
+        sincos(p.gs[2 * gg] * pos[c * 2 * NR + 2 * i] + p.gs[2 * gg + 1] * pos[c * 2 * NR + 2 * i + 1], &sn, &cs);
       un[e] = mk(cs, sn);
     }
     __syncthreads();
   }
-  for (int e = tid; e < ng * 256; e += nthr) {  // envelope, entry (a, i) of configuration c
-    const int c = e >> 8, a = (e >> 4) & 15, i = e & 15;
+  for (int e = tid; e < ng * NR * NR; e += nthr) {  // envelope, entry (a, i) of configuration c
+    const int c = e / (NR * NR), a = (e / NR) % NR, i = e % NR;
     const bool fc = KIND == 1 && i == flp[c];
     cd v = mk(0, 0), gx = mk(0, 0), gy = mk(0, 0);
     if (a < N && i < N) {
-      const double px = pos[c * 32 + 2 * i], py = pos[c * 32 + 2 * i + 1];
-      const int si = spin[c * 16 + i];
+      const double px = pos[c * 2 * NR + 2 * i], py = pos[c * 2 * NR + 2 * i + 1];
+      const int si = spin[c * NR + i];
       for (int gg = 0; gg < n_g; ++gg) {
         const double Gx = p.gs[2 * gg], Gy = p.gs[2 * gg + 1];
         double sn, cs;
         if (tab) {
-          const cd z = un[(c * n_g + gg) * 16 + i];
+          const cd z = un[(c * n_g + gg) * NR + i];
           cs = z.re;
           sn = z.im;
         } else {
@@ -1259,97 +1290,99 @@ __device__ __forceinline__ void rowdet_envelope(const JetGeom& j, const DetParam
         }
       }
     }
-    env[c * 272 + a * 17 + i] = v;
+    env[c * NR * EP + a * EP + i] = v;
     if (fc) {
-      dcol[c * 32 + a] = gx;
-      dcol[c * 32 + 16 + a] = gy;
+      dcol[c * 2 * NR + a] = gx;
+      dcol[c * 2 * NR + NR + a] = gy;
     }
   }
   __syncthreads();
 }
 
-template <typename T, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 3 : 4)))
-void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
-                  int64_t w0, const T* __restrict__ bf, cd* __restrict__ resg) {
+// NR = 16: up to four waves (256 threads); NR = 32: up to eight (512 threads), two determinants each
+template <typename T, int KIND, int NR>
+__global__ __launch_bounds__(NR == 16 ? 256 : 512)
+__attribute__((amdgpu_waves_per_eu(NR == 16 ? (KIND == 0 ? 3 : 4) : 2)))
+void k_det_rows(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
+                int64_t w0, const T* __restrict__ bf, cd* __restrict__ resg) {
   extern __shared__ __align__(16) char smem[];
+  constexpr int EP = NR + 1, GW = 64 / NR;
   const int N = j.N, K = p.K, nt = j.nt, rs_ = nt + 2, C = j.C;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x, nwaves = nthr >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x;
   const int64_t cfg0 = (int64_t)blockIdx.x * G;
   const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
   cd* env = reinterpret_cast<cd*>(smem);
-  cd* dcol = env + G * 272;
-  double* pos = reinterpret_cast<double*>(dcol + (KIND == 1 ? G * 32 : 0));
-  int* spin = reinterpret_cast<int*>(pos + G * 32);
-  int* flp = spin + G * 16;
-  const size_t head = (size_t)G * (272 + (KIND == 1 ? 32 : 0)) * sizeof(cd) + (size_t)G * 32 * sizeof(double) +
-                      (size_t)G * 17 * sizeof(int);
-  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
+  cd* dcol = env + G * NR * EP;
+  double* pos = reinterpret_cast<double*>(dcol + (KIND == 1 ? G * 2 * NR : 0));
+  int* spin = reinterpret_cast<int*>(pos + G * 2 * NR);
+  int* flp = spin + G * NR;
+  cd* xrows = reinterpret_cast<cd*>(smem + (rowdet_head(NR, G, KIND) + 15) / 16 * 16);
   const int n_g = p.n_g;
   const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
-  cd* wtab = xrows + 256;
+  cd* wtab = xrows + 16 * NR;
   cd* un = wtab + (wlds ? N * n_g * 2 : 0);
   if (wlds)
     for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
   const int64_t KN = (int64_t)K * N, rowlen = 2 * KN, chs = (int64_t)N * rowlen;
-  const int r = lane & 15;
-  const int t = 4 * wave + (lane >> 4);
+  const int r = lane % NR;
+  const int t = GW * wave + lane / NR;
   const int cl = t / K, k = t % K;
-  const bool act = cl < ng;  // the same for the 16 lanes of a group
+  const bool act = cl < ng;  // the same for the NR lanes of a group
   const T* bfc0 = bf + (cfg0 + (act ? cl : 0)) * C * chs;
-  // row r of bf; the value-only pass loads it before the envelope is built so that its latency
-  // overlaps that work (the spin-flip pass has no registers to spare there)
-  T bre[16], bim[16];
+  // row r of bf; the value-only pass with NR = 16 loads it before the envelope is built so that
+  // its latency overlaps that work (the others have no registers to spare there)
+  T bre[NR], bim[NR];
   auto load_row = [&]() {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < NR; ++i) {
       const T* q = bfc0 + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
       bre[i] = act ? q[0] : (T)0;
       bim[i] = act ? q[KN] : (T)0;
     }
   };
-  if constexpr (KIND == 0) load_row();
-  rowdet_envelope<KIND>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, dcol, pos, spin, flp);
-  if constexpr (KIND == 1) load_row();
+  if constexpr (KIND == 0 && NR == 16) load_row();
+  rowdet_envelope<KIND, NR>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, dcol, pos, spin, flp);
+  if constexpr (KIND == 1 || NR != 16) load_row();
   if (act) {
-    const cd* envc = env + cl * 272;
-    RowDet f;
+    const cd* envc = env + cl * NR * EP;
+    RowDet<NR> f;
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      f.m[i] = (r < N && i < N) ? envc[r * 17 + i] * mk(1.0 + (double)bre[i], (double)bim[i])
+    for (int i = 0; i < NR; ++i)
+      f.m[i] = (r < N && i < N) ? envc[r * EP + i] * mk(1.0 + (double)bre[i], (double)bim[i])
                                 : mk(r == i ? 1.0 : 0.0, 0);
     f.piv = mk(1, 0);
     f.q = r;
-    f.pk0 = f.pk1 = 0;
+#pragma unroll
+    for (int w = 0; w < RowPk<NR>::NW; ++w) f.pk[w] = 0;
     f.used = f.zero = false;
-    RowDetSteps<0, KIND>::run(f, r, xrows + 16 * (lane >> 4) + 64 * wave, N);
+    RowDetSteps<0, KIND, NR>::run(f, r, xrows + NR * (lane / NR) + 64 * wave, N);
     const cd ld = rowdet_logdet(f, r, N);
     cd* rk = resg + ((cfg0 + cl) * K + k) * rs_;
     if (r == 0) rk[0] = ld;
     if constexpr (KIND == 1) {
       // tangents c = 1, 2: d/dx, d/dy of the flipped electron
       const int i = f.q, flip = flp[cl];
-      const cd* dc = dcol + cl * 32;
+      const cd* dc = dcol + cl * 2 * NR;
       cd t1 = mk(0, 0), t2 = mk(0, 0);
       if (f.used) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
+        for (int s = 0; s < NR; ++s) {
           if (s >= N) continue;
           const int a = rowdet_pk(f, s);
-          const cd e = envc[a * 17 + i];
+          const cd e = envc[a * EP + i];
           cd d1 = e * bf_at(bfc0 + chs, rowlen, KN, N, k, a, i);
           cd d2 = e * bf_at(bfc0 + 2 * chs, rowlen, KN, N, k, a, i);
           if (i == flip) {
             const cd ob = mk(1, 0) + bf_at(bfc0, rowlen, KN, N, k, a, i);
             d1 = d1 + dc[a] * ob;
-            d2 = d2 + dc[16 + a] * ob;
+            d2 = d2 + dc[NR + a] * ob;
           }
           t1 = cfma(t1, f.m[s], d1);
           t2 = cfma(t2, f.m[s], d2);
         }
       }
-      t1 = group_sum_cd(t1);
-      t2 = group_sum_cd(t2);
+      t1 = group_sum_cd<NR>(t1);
+      t2 = group_sum_cd<NR>(t2);
       if (f.zero) t1 = t2 = mk(NAN, NAN);
       if (r == 0) {
         rk[1] = t1;
@@ -1359,16 +1392,16 @@ void k_det_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R, c
   }
 }
 
-// The log-sum-exp over the determinants and the Jastrow factor of k_det_rows16's results
+// The log-sum-exp over the determinants and the Jastrow factor of k_det_rows's results
 // res [cfg][K][nt + 2], one wave per configuration (kept out of the determinant kernel, whose
 // register budget it would otherwise set).
 template <int KIND>
-__global__ __launch_bounds__(256) void k_det_tail16(JetGeom j, DetParams p, const double* __restrict__ R,
-                                                    const int32_t* __restrict__ S, int64_t w0,
-                                                    const cd* __restrict__ res, double* __restrict__ out_lp,
-                                                    double* __restrict__ out_tan) {
-  __shared__ double pos[4][32];
-  __shared__ int spin[4][16];
+__global__ __launch_bounds__(256) void k_det_tail(JetGeom j, DetParams p, const double* __restrict__ R,
+                                                  const int32_t* __restrict__ S, int64_t w0,
+                                                  const cd* __restrict__ res, double* __restrict__ out_lp,
+                                                  double* __restrict__ out_tan) {
+  __shared__ double pos[4][64];
+  __shared__ int spin[4][32];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t cfg = (int64_t)blockIdx.x * 4 + wave;
   if (cfg >= j.n_cfg) return;  // the whole wave; no workgroup barrier follows
@@ -1837,24 +1870,36 @@ int launch_det_jet(const JetGeom& j, const DetParams& p, const double* R, const 
                    hipStream_t st) {
   if (j.n_cfg == 0) return NQS_OK;
   const int N = j.N, NN = N * N;
-  // value-only and spin-flip passes: four determinants per wave, one row per lane, then the
-  // log-sum-exp per configuration; the per-determinant results are staged in `scratch`
-  if (N <= 16 && p.K <= 16 && !j.has_lap && (j.nt == 0 || (j.mode == 2 && j.nt == 2))) {
+  // value-only and spin-flip passes: one determinant per group of NR lanes (NR = 16 for N <= 16,
+  // 32 up to N = 32), one row per lane, then the log-sum-exp per configuration; the
+  // per-determinant results are staged in `scratch`
+  if (N <= 32 && p.K <= 16 && !j.has_lap && (j.nt == 0 || (j.mode == 2 && j.nt == 2))) {
     const int kind = j.nt == 0 ? 0 : 1;
-    const int G = std::max(1, std::min(4, 16 / p.K));
-    const int waves = std::min(4, (G * p.K + 3) / 4);
-    const size_t lds = rowdet_lds(G, N, p.n_g, kind);
+    const int NR = N <= 16 ? 16 : 32, GW = 64 / NR;
+    int G = std::max(1, std::min(4, 16 / p.K));
+    while (G > 1 && rowdet_lds(NR, G, N, p.n_g, kind) > 64 * 1024) --G;
+    const int waves = std::min(16 / GW, (G * p.K + GW - 1) / GW);
+    const size_t lds = rowdet_lds(NR, G, N, p.n_g, kind);
     const size_t need = (size_t)j.n_cfg * p.K * (j.nt + 2) * sizeof(cd);
     if (lds <= 64 * 1024 && scratch != nullptr && need <= scratch_bytes) {
       cd* res = static_cast<cd*>(scratch);
       const dim3 grid((unsigned)cdiv(j.n_cfg, G)), block(64 * waves), tgrid((unsigned)cdiv(j.n_cfg, 4));
-      if (kind == 0) {
-        hipLaunchKernelGGL((k_det_rows16<T, 0>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
-        hipLaunchKernelGGL(k_det_tail16<0>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
+      if (NR == 16) {
+        if (kind == 0)
+          hipLaunchKernelGGL((k_det_rows<T, 0, 16>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
+        else
+          hipLaunchKernelGGL((k_det_rows<T, 1, 16>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
       } else {
-        hipLaunchKernelGGL((k_det_rows16<T, 1>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
-        hipLaunchKernelGGL(k_det_tail16<1>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
+        if (kind == 0)
+          hipLaunchKernelGGL((k_det_rows<T, 0, 32>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
+        else
+          hipLaunchKernelGGL((k_det_rows<T, 1, 32>), grid, block, lds, st, j, p, G, R, S, w0, bf, res);
       }
+      NQS_LAUNCH_CHECK();
+      if (kind == 0)
+        hipLaunchKernelGGL(k_det_tail<0>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
+      else
+        hipLaunchKernelGGL(k_det_tail<1>, tgrid, dim3(256), 0, st, j, p, R, S, w0, res, out_lp, out_tan);
       NQS_LAUNCH_CHECK();
       return NQS_OK;
     }
@@ -2117,65 +2162,69 @@ __global__ __launch_bounds__(256) void k_det_bwd(DetParams p, const double* __re
   }
 }
 
-// The same backward, row per lane, for N <= 16 without a learnable envelope (no dW leaf): four
-// determinants per wave as in k_det_rows16 (determinant t = 4 wave + lane / 16 of the workgroup is
-// k = t % K of configuration t / K), each factored by Gauss-Jordan (KIND 1), after which lane
-// r = p_q holds row q of M^-1 with its columns in pivot order, so dL/dbf[k,a,q] =
-// p_k env[a,q] M^-1[q,a] is lane-local.  The log dets reach the configuration's other
-// determinants through LDS for the log-sum-exp weights p_k.  LDS: rowdet_lds(G, N, n_g, 0), then
-// the log dets [G][16].
-template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_det_bwd_rows16(JetGeom j, DetParams p, int G, const double* __restrict__ R,
-                                                        const int32_t* __restrict__ S, int64_t w0,
-                                                        const T* __restrict__ bf, T* __restrict__ ybar,
-                                                        double* __restrict__ out_lp, T* __restrict__ O, int64_t ldo,
-                                                        int64_t W_total, DetBwdOffsets off) {
+// The same backward, row per lane, for N <= 32 without a learnable envelope (no dW leaf): one
+// determinant per group of NR lanes as in k_det_rows (determinant t = (64 / NR) wave + lane / NR of
+// the workgroup is k = t % K of configuration t / K), each factored by Gauss-Jordan (KIND 1),
+// after which lane r = p_q holds row q of M^-1 with its columns in pivot order, so
+// dL/dbf[k,a,q] = p_k env[a,q] M^-1[q,a] is lane-local.  The log dets reach the configuration's
+// other determinants through LDS for the log-sum-exp weights p_k.  LDS: rowdet_lds(NR, G, N, n_g,
+// 0), then the log dets [G][16].
+template <typename T, int NR>
+__global__ __launch_bounds__(NR == 16 ? 256 : 512) __attribute__((amdgpu_waves_per_eu(NR == 16 ? 3 : 2)))
+void k_det_bwd_rows(JetGeom j, DetParams p, int G, const double* __restrict__ R, const int32_t* __restrict__ S,
+                    int64_t w0, const T* __restrict__ bf, T* __restrict__ ybar, double* __restrict__ out_lp,
+                    T* __restrict__ O, int64_t ldo, int64_t W_total, DetBwdOffsets off) {
   extern __shared__ __align__(16) char smem[];
+  constexpr int EP = NR + 1, GW = 64 / NR;
   const int N = j.N, K = p.K, n_g = p.n_g;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x;
   const int64_t cfg0 = (int64_t)blockIdx.x * G;
   const int ng = (int)min((int64_t)G, j.n_cfg - cfg0);
   cd* env = reinterpret_cast<cd*>(smem);
-  double* pos = reinterpret_cast<double*>(env + G * 272);
-  int* spin = reinterpret_cast<int*>(pos + G * 32);
-  int* flp = spin + G * 16;
-  const size_t head = (size_t)G * 272 * sizeof(cd) + (size_t)G * 32 * sizeof(double) + (size_t)G * 17 * sizeof(int);
-  cd* xrows = reinterpret_cast<cd*>(smem + (head + 15) / 16 * 16);
+  double* pos = reinterpret_cast<double*>(env + G * NR * EP);
+  int* spin = reinterpret_cast<int*>(pos + G * 2 * NR);
+  int* flp = spin + G * NR;
+  cd* xrows = reinterpret_cast<cd*>(smem + (rowdet_head(NR, G, 0) + 15) / 16 * 16);
   const bool wlds = N * n_g * 2 * (int)sizeof(cd) <= kRowDetWenvMax;
-  cd* wtab = xrows + 256;
+  cd* wtab = xrows + 16 * NR;
   cd* un = wtab + (wlds ? N * n_g * 2 : 0);
-  cd* lds_ld = reinterpret_cast<cd*>(smem + rowdet_lds(G, N, n_g, 0));
+  cd* lds_ld = reinterpret_cast<cd*>(smem + rowdet_lds(NR, G, N, n_g, 0));
   if (wlds)
     for (int e = tid; e < N * n_g * 2; e += nthr) wtab[e] = mk(p.wenv[2 * e], p.wenv[2 * e + 1]);
   const int64_t KN = (int64_t)K * N, rowlen = 2 * KN;
-  const int r = lane & 15;
-  const int t = 4 * wave + (lane >> 4);
+  const int r = lane % NR;
+  const int t = GW * wave + lane / NR;
   const int cl = t / K, k = t % K;
-  const bool act = cl < ng;  // the same for the 16 lanes of a group
+  const bool act = cl < ng;  // the same for the NR lanes of a group
   const int64_t cfg = cfg0 + (act ? cl : 0);
   const T* bfc = bf + cfg * (int64_t)N * rowlen;
-  // row r of M's backflow factor, loaded before the envelope is built so that its latency overlaps
-  // that work
-  T bre[16], bim[16];
+  // row r of M's backflow factor (NR = 16: loaded before the envelope is built so that its latency
+  // overlaps that work)
+  T bre[NR], bim[NR];
+  auto load_row = [&]() {
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const T* q = bfc + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
-    bre[i] = act ? q[0] : (T)0;
-    bim[i] = act ? q[KN] : (T)0;
-  }
-  rowdet_envelope<0>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, nullptr, pos, spin, flp);
-  const cd* envc = env + cl * 272;
-  RowDet f;
+    for (int i = 0; i < NR; ++i) {
+      const T* q = bfc + (int64_t)(i < N ? i : N - 1) * rowlen + (int64_t)k * N + (r < N ? r : N - 1);
+      bre[i] = act ? q[0] : (T)0;
+      bim[i] = act ? q[KN] : (T)0;
+    }
+  };
+  if constexpr (NR == 16) load_row();
+  rowdet_envelope<0, NR>(j, p, cfg0, ng, R, S, w0, wlds ? wtab : nullptr, un, env, nullptr, pos, spin, flp);
+  if constexpr (NR != 16) load_row();
+  const cd* envc = env + cl * NR * EP;
+  RowDet<NR> f;
   if (act) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      f.m[i] = (r < N && i < N) ? envc[r * 17 + i] * mk(1.0 + (double)bre[i], (double)bim[i])
+    for (int i = 0; i < NR; ++i)
+      f.m[i] = (r < N && i < N) ? envc[r * EP + i] * mk(1.0 + (double)bre[i], (double)bim[i])
                                 : mk(r == i ? 1.0 : 0.0, 0);
     f.piv = mk(1, 0);
     f.q = r;
-    f.pk0 = f.pk1 = 0;
+#pragma unroll
+    for (int w = 0; w < RowPk<NR>::NW; ++w) f.pk[w] = 0;
     f.used = f.zero = false;
-    RowDetSteps<0, 1>::run(f, r, xrows + 16 * (lane >> 4) + 64 * wave, N);
+    RowDetSteps<0, 1, NR>::run(f, r, xrows + NR * (lane / NR) + 64 * wave, N);
     const cd ld = rowdet_logdet(f, r, N);
     if (r == 0) lds_ld[cl * 16 + k] = ld;
   }
@@ -2203,8 +2252,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (k == 0) {
       double J = 0;
       if (p.use_jastrow) {
-        const double* pc = pos + cl * 32;
-        const int* sc = spin + cl * 16;
+        const double* pc = pos + cl * 2 * NR;
+        const int* sc = spin + cl * NR;
         double Ja = 0;
         if (r < N)
           for (int b = r + 1; b < N; ++b) {
@@ -2213,7 +2262,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             jastrow_u(p, sc[r] == sc[b], d, u, u1, u2);
             Ja += u;
           }
-        J = group_sum_d(Ja);
+        J = group_sum_d<NR>(Ja);
       }
       if (out_lp && r == 0) {
         out_lp[2 * cfg] = L.re + J;
@@ -2241,10 +2290,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     } else if (f.used) {
       const int i = f.q;
 #pragma unroll
-      for (int s = 0; s < 16; ++s) {
+      for (int s = 0; s < NR; ++s) {
         const int a = rowdet_pk(f, s);
         if (s >= N || a >= N) continue;
-        const cd g = envc[a * 17 + i] * (pk * f.m[s]);
+        const cd g = envc[a * EP + i] * (pk * f.m[s]);
         const int64_t col = (int64_t)i * rowlen + k * N + a;
         y0[col] = (T)g.re;
         y0[col + KN] = (T)(-g.im);
@@ -2258,7 +2307,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int e = tid; e < ng * 2 * nk; e += nthr) {
       const int c = e / (2 * nk), q = e % (2 * nk);
       double v;
-      const int64_t o = jastrow_o_entry(p, off, N, pos + c * 32, spin + c * 16, q, v);
+      const int64_t o = jastrow_o_entry(p, off, N, pos + c * 2 * NR, spin + c * NR, q, v);
       if (o < 0) continue;
       const int64_t w = w0 + cfg0 + c;
       O[w * ldo + o] = (T)v;
@@ -2273,14 +2322,22 @@ int launch_det_bwd(int64_t n_cfg, const DetParams& p, const double* R, const int
                    hipStream_t st) {
   if (n_cfg == 0) return NQS_OK;
   const int N = p.N, NN = N * N;
-  if (N <= 16 && p.K <= 16 && off.dW < 0) {
-    const int G = std::max(1, std::min(4, 16 / p.K));
-    const int waves = std::min(4, (G * p.K + 3) / 4);
-    const size_t lds = rowdet_lds(G, N, p.n_g, 0) + (size_t)G * 16 * sizeof(cd);
+  if (N <= 32 && p.K <= 16 && off.dW < 0) {
+    const int NR = N <= 16 ? 16 : 32, GW = 64 / NR;
+    auto bytes = [&](int G) { return rowdet_lds(NR, G, N, p.n_g, 0) + (size_t)G * 16 * sizeof(cd); };
+    int G = std::max(1, std::min(4, 16 / p.K));
+    while (G > 1 && bytes(G) > 64 * 1024) --G;
+    const int waves = std::min(16 / GW, (G * p.K + GW - 1) / GW);
+    const size_t lds = bytes(G);
     if (lds <= 64 * 1024) {
       const JetGeom j{n_cfg, N, 1, 0, 0, 0};
-      hipLaunchKernelGGL(k_det_bwd_rows16<T>, dim3((unsigned)cdiv(n_cfg, G)), dim3(64 * waves), lds, st, j, p, G, R,
-                         S, w0, bf, ybar, out_lp, O, ldo, W_total, off);
+      const dim3 grid((unsigned)cdiv(n_cfg, G)), block(64 * waves);
+      if (NR == 16)
+        hipLaunchKernelGGL((k_det_bwd_rows<T, 16>), grid, block, lds, st, j, p, G, R, S, w0, bf, ybar, out_lp, O, ldo,
+                           W_total, off);
+      else
+        hipLaunchKernelGGL((k_det_bwd_rows<T, 32>), grid, block, lds, st, j, p, G, R, S, w0, bf, ybar, out_lp, O, ldo,
+                           W_total, off);
       NQS_LAUNCH_CHECK();
       return NQS_OK;
     }

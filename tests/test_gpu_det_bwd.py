@@ -1,13 +1,14 @@
-"""The determinant-head backward of the O rows (nqs_psiformer_grad) on its row-per-lane path (N <= 16,
-no learnable envelope: four determinants per wave, Gauss-Jordan inverses, log-sum-exp weights
-exchanged through LDS) against the oracle's reverse-mode autodiff of log psi (the restatement of
+"""The determinant-head backward of the O rows (nqs_psiformer_grad) on its row-per-lane path (N <= 32,
+no learnable envelope: one determinant per group of 16 or 32 lanes, Gauss-Jordan inverses,
+log-sum-exp weights exchanged through LDS) against the oracle's reverse-mode autodiff of log psi (the restatement of
 per_sample_O, src/nqs/minsr.py:56-65), float64, 1e-9 relative to max |O|, and the log psi the
 backward returns against the forward's (1e-12).
 
 Determinant counts that fill the workgroup differently: K = 1 (four configurations per wave),
 K = 3 (four configurations over three waves, the last group idle), K = 2 with the cusp Jastrow,
 K = 3 with the RBF Jastrow (its O entries and its share of log psi), and K = 16 (one configuration
-per workgroup); N = 5 electrons, so the identity padding of the 16 x 16 determinants is exercised.
+per workgroup), at N = 5 electrons (16-lane groups with identity padding); and the 32-lane groups
+of 16 < N <= 32 at N = 20 (padded, K = 3, RBF Jastrow) and N = 32 (K = 4).
 """
 import numpy as np
 import pytest
@@ -19,19 +20,21 @@ from nqs import minsr as nms
 
 pytestmark = pytest.mark.gpu
 
-ARCHS = {
-    "k1": dict(embed_dim=16, depth=2, n_heads=4, n_freq=4),
-    "k3": dict(embed_dim=16, depth=1, n_heads=2, n_freq=3, n_dets=3),
-    "k2_cusp": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=2, use_jastrow=True, jastrow_kind="cusp"),
-    "k3_rbf": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=3, use_jastrow=True, n_knots=6),
-    "k16": dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=16),
+ARCHS = {  # name: (N, architecture)
+    "k1": (5, dict(embed_dim=16, depth=2, n_heads=4, n_freq=4)),
+    "k3": (5, dict(embed_dim=16, depth=1, n_heads=2, n_freq=3, n_dets=3)),
+    "k2_cusp": (5, dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=2, use_jastrow=True, jastrow_kind="cusp")),
+    "k3_rbf": (5, dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=3, use_jastrow=True, n_knots=6)),
+    "k16": (5, dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=16)),
+    "n20_k3_rbf": (20, dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=3, use_jastrow=True, n_knots=6)),
+    "n32_k4": (32, dict(embed_dim=8, depth=1, n_heads=2, n_freq=2, n_dets=4)),
 }
 
 
 @pytest.mark.parametrize("name", list(ARCHS))
 def test_det_backward_rows_match_oracle(name):
-    kw = ARCHS[name]
-    N, L, W, seed = 5, 7.0, 5, 11
+    N, kw = ARCHS[name]
+    L, W, seed = (7.0 if N <= 16 else 2.0 * np.sqrt(np.pi * N)), (5 if N <= 16 else 2), 11
     g = bands.rashba_gs(L, N, 1.0, 1.0, 0.5, n_cut=10, twist=(0.1, 0.17))
     arch = opf.Arch(N, g["Gs"], g["W"], L, **kw)
     theta = opf.init_theta(arch, seed, backflow_scale=0.05, scale_all=0.02)
