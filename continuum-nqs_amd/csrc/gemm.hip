@@ -224,6 +224,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt_v2(GemmArgs<T> g) {
         }
       }
   }
+  if constexpr (sizeof(T) == 8) {
+    if (g.potrf_linv != nullptr && bm == 0 && bn == 0) {
+      // the block at C's origin is this tile's and final: every wave's stores complete and are
+      // visible to wave 0 (the fence on both sides of the barrier), which factors it with the
+      // staging LDS as scratch
+      __threadfence();
+      __syncthreads();
+      if (wave == 0) {
+        __threadfence();
+        potrf_diag64_call(reinterpret_cast<double*>(C), g.ldc, g.potrf_j0, g.potrf_linv + (int64_t)bz * g.potrf_sl,
+                     g.potrf_info + bz, reinterpret_cast<double (*)[65]>(&As[0][0][0]),
+                     reinterpret_cast<double*>(&Bs[0][0][0]));
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -524,6 +539,12 @@ static int launch_gemm_bf(const GemmArgs<float>& g, hipStream_t st) {
   }, !acc || g.tune_scratch != nullptr, st);
 }
 
+__device__ __attribute__((noinline)) void potrf_diag64_call(double* __restrict__ A, int64_t lda, int64_t jg,
+                                                            double* __restrict__ Linv, int* __restrict__ info,
+                                                            double (*Ls)[65], double* rdiag) {
+  potrf_diag64(A, lda, jg, Linv, info, Ls, rdiag);
+}
+
 template <typename T>
 int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return NQS_OK;
@@ -540,6 +561,8 @@ int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st) {
   } else {
     NQS_REQUIRE(g.gelu_cn == 0, NQS_ERR_UNSUPPORTED, "GELU prologue needs the bf16 GEMM path");
   }
+  NQS_REQUIRE(g.potrf_linv == nullptr || (vec_ok && g.M >= 64 && g.N >= 64), NQS_ERR_ARG,
+              "a fused diagonal factor needs the vector float64 kernel and a 64 x 64 block at C's origin");
   if (vec_ok) {
     dim3 grid((unsigned)cdiv(g.N, 128), (unsigned)cdiv(g.M, 128), (unsigned)batch);
     GemmArgs<T> gl = g;

@@ -100,10 +100,89 @@ struct GemmArgs {
   // (channel 0) become gelu(u), a tangent row du becomes gelu'(u of its token's value row) du.
   // 0 = no prologue.
   int gelu_cn = 0, gelu_n = 0;
+  // float64 only (the Cholesky factorisation): when potrf_linv is set, the workgroup of tile (0, 0)
+  // factors the 64 x 64 block at C's origin after storing its tile (potrf_diag64: the block is final
+  // once this product is added), writing the block's inverse to potrf_linv + z potrf_sl and its first
+  // non-positive pivot to potrf_info[z] (potrf_j0 = the block's column in the factored matrix)
+  double* potrf_linv = nullptr;
+  int* potrf_info = nullptr;
+  int64_t potrf_j0 = 0, potrf_sl = 0;
 };
 
 template <typename T>
 int gemm_nt(const GemmArgs<T>& g, int batch, hipStream_t st);
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// One wave (lane i = threadIdx.x % 64) factors the 64 x 64 diagonal block at A (row stride lda) in
+// place (lower; the upper triangle is set to zero) and writes its inverse (lower, row-major) to Linv.
+// Lane i keeps row i in registers; column k's multipliers reach the other lanes by v_readlane, so no
+// step needs a barrier.  The registers of the upper triangle (k > i) are updated along with the rest
+// but never read by another lane nor stored, so the steps need no masks.  The inverse is formed by
+// forward substitution, column c in lane c, reading L and the reciprocal pivots from the LDS scratch
+// Ls (64 x 65 doubles) and rdiag (64 doubles), the same address on every lane.  info[0] = jg + first
+// non-positive pivot + 1 (jg: the block's column in the factored matrix; the pivot is then taken as
+// 1 and the factor goes on).
+__device__ __forceinline__ void potrf_diag64(double* __restrict__ A, int64_t lda, int64_t jg, double* __restrict__ Linv,
+                                             int* __restrict__ info, double (*Ls)[65], double* rdiag) {
+  constexpr int NB = 64;
+  const int i = threadIdx.x & 63;
+  double a[NB];
+  double* row = A + (int64_t)i * lda;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) a[k] = row[k];
+  int bad = 0;
+  double myinv = 0.0;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    double d = readlane_f64(a[k], k);
+    if (!(d > 0.0)) {
+      if (!bad) bad = k + 1;
+      d = 1.0;
+    }
+    const double lkk = sqrt(d), inv = 1.0 / lkk;
+    a[k] = i == k ? lkk : a[k] * inv;
+    myinv = i == k ? inv : myinv;
+#pragma unroll
+    for (int j = k + 1; j < NB; ++j) a[j] -= a[k] * readlane_f64(a[k], j);
+  }
+  if (bad && i == 0 && info[0] == 0) info[0] = (int)(jg + bad);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const double v = k <= i ? a[k] : 0.0;
+    row[k] = v;
+    Ls[i][k] = v;
+  }
+  rdiag[i] = myinv;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int c = i;
+  double x[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    double s0 = r == c ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int k = 0; k + 3 < r; k += 4) {
+      s0 -= Ls[r][k] * x[k];
+      s1 -= Ls[r][k + 1] * x[k + 1];
+      s2 -= Ls[r][k + 2] * x[k + 2];
+      s3 -= Ls[r][k + 3] * x[k + 3];
+    }
+#pragma unroll
+    for (int k = r & ~3; k < r; ++k) s0 -= Ls[r][k] * x[k];
+    x[r] = ((s0 + s1) + (s2 + s3)) * rdiag[r];
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) Linv[r * NB + c] = x[r];
+}
+// The same as a called function, for kernels whose own register budget it would otherwise set.
+__device__ __attribute__((noinline)) void potrf_diag64_call(double* __restrict__ A, int64_t lda, int64_t jg,
+                                                            double* __restrict__ Linv, int* __restrict__ info,
+                                                            double (*Ls)[65], double* rdiag);
 
 // Jet geometry: activations are laid out [cfg][channel][token][feature]; channel 0 is the
 // value, channels 1..nt the tangents, channel nt+1 the Laplacian when has_lap.

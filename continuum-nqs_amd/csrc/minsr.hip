@@ -769,75 +769,16 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
 #define CH_NB 64
 #define CH_NBO 512
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
-}
-
-// Factor the 64 x 64 diagonal block at (j0, j0) in place (lower) and store its inverse (lower,
-// row-major) in Linv, with one wave: lane i keeps row i in registers; column k's multipliers
-// reach the other lanes by v_readlane, so no step needs a barrier.  The registers of the upper
-// triangle (k > i) are updated along with the rest but never read by another lane nor stored,
-// so the steps need no masks.  The inverse is formed by forward substitution, column c in lane c,
-// reading L and the reciprocal pivots from LDS (the same address on every lane).
-// info[0] = first non-positive pivot + 1 (the pivot is then taken as 1 and the factor goes on).
+// Factor the 64 x 64 diagonal block at (j0, j0) in place and store its inverse in Linv
+// (potrf_diag64, one wave).  The factorisation launches this for its first block only; every later
+// block is factored by the workgroup that finishes it inside the preceding update (GemmArgs::potrf_*).
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t j0,
                                                    double* __restrict__ Linv, int* __restrict__ info, int64_t sA,
                                                    int64_t sL) {
-  A += blockIdx.z * sA;
-  Linv += blockIdx.z * sL;
-  info += blockIdx.z;
   __shared__ double Ls[CH_NB][CH_NB + 1];
   __shared__ double rdiag[CH_NB];
-  const int i = threadIdx.x;
-  double a[CH_NB];
-  double* row = A + (j0 + i) * lda + j0;
-#pragma unroll
-  for (int k = 0; k < CH_NB; ++k) a[k] = row[k];
-  int bad = 0;
-  double myinv = 0.0;
-#pragma unroll
-  for (int k = 0; k < CH_NB; ++k) {
-    double d = readlane_d(a[k], k);
-    if (!(d > 0.0)) {
-      if (!bad) bad = k + 1;
-      d = 1.0;
-    }
-    const double lkk = sqrt(d), inv = 1.0 / lkk;
-    a[k] = i == k ? lkk : a[k] * inv;
-    myinv = i == k ? inv : myinv;
-#pragma unroll
-    for (int j = k + 1; j < CH_NB; ++j) a[j] -= a[k] * readlane_d(a[k], j);
-  }
-  if (bad && i == 0 && info[0] == 0) info[0] = (int)(j0 + bad);
-#pragma unroll
-  for (int k = 0; k < CH_NB; ++k) {
-    const double v = k <= i ? a[k] : 0.0;
-    row[k] = v;
-    Ls[i][k] = v;
-  }
-  rdiag[i] = myinv;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  const int c = i;
-  double x[CH_NB];
-#pragma unroll
-  for (int r = 0; r < CH_NB; ++r) {
-    double s0 = r == c ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-    for (int k = 0; k + 3 < r; k += 4) {
-      s0 -= Ls[r][k] * x[k];
-      s1 -= Ls[r][k + 1] * x[k + 1];
-      s2 -= Ls[r][k + 2] * x[k + 2];
-      s3 -= Ls[r][k + 3] * x[k + 3];
-    }
-#pragma unroll
-    for (int k = r & ~3; k < r; ++k) s0 -= Ls[r][k] * x[k];
-    x[r] = ((s0 + s1) + (s2 + s3)) * rdiag[r];
-  }
-#pragma unroll
-  for (int r = 0; r < CH_NB; ++r) Linv[r * CH_NB + c] = x[r];
+  A += blockIdx.z * sA;
+  potrf_diag64(A + j0 * lda + j0, lda, j0, Linv + blockIdx.z * sL, info + blockIdx.z, Ls, rdiag);
 }
 
 // The factor's columns J0 .. J0 + nbo - 1 below their diagonal blocks, from the outer panel buffer
@@ -942,12 +883,23 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
       if (*pending) (void)hipStreamWaitEvent(st, side->done, 0);
     }
   } join{st, side, &pending};
+  // the diagonal factor of block j (with its inverse) into the update that finishes the block
+  auto with_potrf = [&](GemmArgs<double> g, int64_t j) {
+    g.potrf_linv = Linv + (j / CH_NB) * CH_NB * CH_NB;
+    g.potrf_info = info;
+    g.potrf_j0 = j;
+    g.potrf_sl = sL;
+    return g;
+  };
+  // block 0 on its own; every later diagonal block is factored inside the product that finalises it
+  // (the update of the rest of its outer panel, or the trailing update of the panel before), so the
+  // dependency chain has no launch of its own for it
+  hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, (unsigned)nb), dim3(64), 0, st, A, lda, (int64_t)0, Linv, info, sA, sL);
+  NQS_LAUNCH_CHECK();
   for (int64_t J0 = 0; J0 < n; J0 += CH_NBO) {
     const int64_t nbo = n - J0 < CH_NBO ? n - J0 : CH_NBO;
     for (int64_t j0 = J0; j0 < J0 + nbo; j0 += CH_NB) {
       double* Li = Linv + (j0 / CH_NB) * CH_NB * CH_NB;
-      hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, (unsigned)nb), dim3(64), 0, st, A, lda, j0, Li, info, sA, sL);
-      NQS_LAUNCH_CHECK();
       const int64_t m = n - j0 - CH_NB;
       if (m == 0) break;
       // the block's factor column A21 Linv^T (NT GEMM with B = Linv, K = 64) into the outer panel
@@ -957,12 +909,14 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
           strided(g64(A + (j0 + CH_NB) * lda + j0, Li, pb, m, CH_NB, CH_NB, lda, CH_NB, CH_NBO, 1.0, 0.0, 0), sA, sL,
                   sP),
           nb, st));
-      // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= pb pb[0:w]^T (lower part)
+      // rest of the outer panel: A[j0+64:, j0+64 : J0+nbo] -= pb pb[0:w]^T (lower part), then the
+      // factor of block j0 + 64
       const int64_t w = J0 + nbo - (j0 + CH_NB);
       if (w > 0)
-        NQS_TRY(gemm_nt<double>(strided(g64(pb, pb, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB, CH_NBO,
-                                            CH_NBO, lda, -1.0, 1.0, 1),
-                                        sP, sP, sA),
+        NQS_TRY(gemm_nt<double>(with_potrf(strided(g64(pb, pb, A + (j0 + CH_NB) * lda + j0 + CH_NB, m, w, CH_NB,
+                                                        CH_NBO, CH_NBO, lda, -1.0, 1.0, 1),
+                                                    sP, sP, sA),
+                                            j0 + CH_NB),
                                 nb, st));
     }
     // the outer panel's factor columns into A, one launch per outer panel
@@ -981,8 +935,9 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
       NQS_CHECK_HIP(hipStreamWaitEvent(st, side->done, 0));
       pending = false;
     }
-    NQS_TRY(gemm_nt<double>(strided(g64(Lp, Lp, A + J1 * lda + J1, M, nb1, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA),
-                            nb, st));
+    NQS_TRY(gemm_nt<double>(
+        with_potrf(strided(g64(Lp, Lp, A + J1 * lda + J1, M, nb1, nbo, lda, lda, lda, -1.0, 1.0, 1), sA, sA, sA), J1),
+        nb, st));
     if (M2 > 0) {
       const double* Lq = Lp + nb1 * lda;
       NQS_CHECK_HIP(hipEventRecord(side->ready, st));
