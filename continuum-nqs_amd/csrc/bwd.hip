@@ -95,31 +95,38 @@ __global__ __launch_bounds__(256) void k_wgrad_cols(WgradArgs<T> a, int ib) {
   }
 }
 
-// Float variant of the column form with two adjacent columns per thread and 8-byte stores, the
-// workgroup sized to the layer width (a 128-wide layer takes one wave, not half of four).  Taken
-// when every segment offset, the segment width and the strides are even, so a column pair never
-// straddles two leaves and the stores stay 8-byte aligned.  Each column's sum runs in the same
-// order as in k_wgrad_cols.  NPT bounds the padded token count (16 or 32), so the cotangents take
-// NPT registers per column; four rows i are swept per iteration (eight independent sums in
-// flight), and O goes out with non-temporal stores (nothing reads it before the MinSR passes).
+// Float variant of the column form with two adjacent columns per thread and 8-byte stores.  A
+// workgroup takes R rows of O (R = 1, 2 or 4: R x the pairs of one row <= 256 threads), rows
+// 2 cfg + c for the cotangent channels c of its configurations, which share the staged inputs, so a
+// 128-wide layer runs four waves per workgroup instead of one.  Taken when every segment offset,
+// the segment width and the strides are even, so a column pair never straddles two leaves and the
+// stores stay 8-byte aligned.  Each column's sum runs in the same order as in k_wgrad_cols.  NPT
+// bounds the padded token count (16 or 32), so the cotangents take NPT registers per column; four
+// rows i are swept per iteration (eight independent sums in flight), and O goes out with
+// non-temporal stores (nothing reads it before the MinSR passes).
 template <int NPT>
-__global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib) {
+__global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib, int pairs, int R) {
   typedef float f2 __attribute__((ext_vector_type(2)));
   extern __shared__ char smem_raw[];
-  float* Xt = reinterpret_cast<float*>(smem_raw);  // [ib][NP]
+  float* Xt = reinterpret_cast<float*>(smem_raw);  // [configurations of the workgroup][ib][NP]
   const int N = a.N, NP = (N + 3) & ~3;
-  const int64_t cfg = blockIdx.y >> 1;
-  const int c = blockIdx.y & 1;
+  const int64_t row0 = (int64_t)blockIdx.y * R, cfg0 = row0 >> 1;
+  const int ncfg = (int)min<int64_t>((R + 1) / 2, a.n_cfg - cfg0);
   const int i0 = blockIdx.z * ib;
   const int nrow = min(ib, a.in - i0);
-  const float* X = a.X + cfg * (int64_t)N * a.ldx;
-  for (int e = threadIdx.x; e < ib * NP; e += blockDim.x) {
-    const int ii = e / NP, t = e % NP;
+  for (int e = threadIdx.x; e < ncfg * ib * NP; e += blockDim.x) {
+    const int cl = e / (ib * NP), ii = (e / NP) % ib, t = e % NP;
+    const float* X = a.X + (cfg0 + cl) * (int64_t)N * a.ldx;
     Xt[e] = (t < N && ii < nrow) ? X[(int64_t)t * a.ldx + i0 + ii] : 0.f;
   }
   __syncthreads();
-  const int j = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= a.out) return;
+  const int rl = threadIdx.x / pairs;
+  const int64_t row = row0 + rl;
+  const int j = 2 * (blockIdx.x * pairs + threadIdx.x % pairs);
+  if (rl >= R || row >= 2 * a.n_cfg || j >= a.out) return;
+  const int64_t cfg = row >> 1;
+  const int c = (int)(row & 1);
+  const float* Xc = Xt + (cfg - cfg0) * ib * NP;
   const float* G = a.G + ((cfg * 2 + c) * (int64_t)N) * a.ldg + j;
   float g0[NPT], g1[NPT];
 #pragma unroll
@@ -128,11 +135,11 @@ __global__ __launch_bounds__(256) void k_wgrad_cols2(WgradArgs<float> a, int ib)
     g0[t] = v[0];
     g1[t] = v[1];
   }
-  const int64_t row = (int64_t)c * a.W_total + a.w0 + cfg;
+  const int64_t orow = (int64_t)c * a.W_total + a.w0 + cfg;
   const int sg = j / a.seg_w, jj = j % a.seg_w;
-  float* O = a.O + row * a.ldo + a.seg_base[sg] + jj;
+  float* O = a.O + orow * a.ldo + a.seg_base[sg] + jj;
   auto rowsum = [&](int ii, float& s0, float& s1) {
-    const float* xr = Xt + ii * NP;
+    const float* xr = Xc + ii * NP;
     s0 = 0.f;
     s1 = 0.f;
 #pragma unroll
@@ -181,12 +188,16 @@ int launch_wgrad(const WgradArgs<T>& a, hipStream_t st) {
     const size_t lds = sizeof(T) * (size_t)ib * NP;
     if constexpr (sizeof(T) == 4) {
       if (wgrad_pairs_ok(a)) {
-        const int threads = (int)std::min<int64_t>(256, round_up(cdiv(a.out, 2), 64));
-        const dim3 grid((unsigned)cdiv(a.out, 2 * threads), (unsigned)(a.n_cfg * 2), (unsigned)cdiv(a.in, ib));
+        // pairs of one row per workgroup (a multiple of 64, at most 256), and rows per workgroup
+        const int pairs = (int)std::min<int64_t>(256, round_up(cdiv(a.out, 2), 64));
+        int R = pairs <= 64 ? 4 : (pairs <= 128 ? 2 : 1);
+        if (R == 4 && 2 * lds > 64 * 1024) R = 2;  // two configurations' inputs would not fit
+        const size_t lds2 = lds * (size_t)((R + 1) / 2);
+        const dim3 grid((unsigned)cdiv(a.out, 2 * pairs), (unsigned)cdiv(a.n_cfg * 2, R), (unsigned)cdiv(a.in, ib));
         if (a.N <= 16)
-          hipLaunchKernelGGL(k_wgrad_cols2<16>, grid, dim3(threads), lds, st, a, ib);
+          hipLaunchKernelGGL(k_wgrad_cols2<16>, grid, dim3(pairs * R), lds2, st, a, ib, pairs, R);
         else
-          hipLaunchKernelGGL(k_wgrad_cols2<32>, grid, dim3(threads), lds, st, a, ib);
+          hipLaunchKernelGGL(k_wgrad_cols2<32>, grid, dim3(pairs * R), lds2, st, a, ib, pairs, R);
         NQS_LAUNCH_CHECK();
         return NQS_OK;
       }
