@@ -299,10 +299,13 @@ static int minsr_gram(const nqs_minsr_desc* d, T* O, char* ws, hipStream_t st) {
   const int64_t W = d->n_walkers, n = 2 * W, P = d->n_params, ldo = d->ldo;
   MinsrWs w = minsr_ws(d, ws);
   NQS_TRY(minsr_mark(0, st));
-  // the bf16 Gram reads only the centred bf16 copy, so O stays as given and the GEMVs of the solve
-  // apply the centring on the fly; the float64 / float32 Gram reads O, which is centred in place
+  // the bf16 Gram reads only the bf16 copy of O shifted by the mean of the first kBf16MeanRows
+  // walkers, so O stays as given and the GEMVs of the solve apply the shift on the fly (the
+  // deflation and the residual's projection make the result that of exact centring); the float64 /
+  // float32 Gram reads O, which is centred in place with the mean over all walkers
   const bool keep = d->gram == NQS_GRAM_BF16;
-  NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, keep ? 0 : 1, w.part, st));
+  NQS_TRY(minsr_center<T>(O, ldo, W, P, w.mean, w.Ob, keep ? 0 : 1, keep ? std::min(W, kBf16MeanRows) : W, w.part,
+                          st));
   NQS_TRY(minsr_mark(1, st));
   if (d->gram == NQS_GRAM_BF16) {
     // A64 is written only after the Gram, so it lends its space to the Gram's partials
@@ -365,6 +368,8 @@ static int minsr_solve(const nqs_minsr_desc* d, T* O, const double* eps, double*
     NQS_TRY(gemv_rows_c<T>(O, ldo, W, P, w.u, mc, w.red + 2, w.v, st));
     NQS_TRY(axpby(n, 1.0, w.et, -1.0, w.v, w.r, st));
     NQS_TRY(axpby(n, 1.0, w.r, -d->lam, w.y, w.r, st));
+    // with the shifted O (bf16 Gram), v carries the shift's constant per half: r <- Pi r
+    if (mc) NQS_TRY(proj_halves(w.r, W, st));
     NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.r, st));
     NQS_TRY(axpby(n, 1.0, w.y, 1.0, w.r, w.y, st));
   }
@@ -438,7 +443,8 @@ static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov
   const bool keep = d[0].gram == NQS_GRAM_BF16;
   for (int s = 0; s < nsys; ++s) {
     const MinsrSysWs& q = w.sys[s];
-    NQS_TRY(minsr_center<T>((T*)Ov[s], d[s].ldo, W, d[s].n_params, q.mean, q.Ob, keep ? 0 : 1, q.part, st));
+    NQS_TRY(minsr_center<T>((T*)Ov[s], d[s].ldo, W, d[s].n_params, q.mean, q.Ob, keep ? 0 : 1,
+                            keep ? std::min(W, kBf16MeanRows) : W, q.part, st));
     NQS_TRY(make_etil(eps[s], W, q.et, st));
   }
   if (keep) {
@@ -499,6 +505,7 @@ static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov
       NQS_TRY(gemv_rows_c<T>(O, d[s].ldo, W, d[s].n_params, q.u, mc, q.red + 2, q.v, st));
       NQS_TRY(axpby(n, 1.0, q.et, -1.0, q.v, r, st));
       NQS_TRY(axpby(n, 1.0, r, -d[s].lam, y, r, st));
+      if (mc) NQS_TRY(proj_halves(r, W, st));
     }
     NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Rr, st, nsys, n * n, n * 64, n));
     NQS_TRY(axpby(nsys * n, 1.0, w.Y, 1.0, w.Rr, w.Y, st));

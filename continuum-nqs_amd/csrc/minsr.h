@@ -4,12 +4,20 @@
 
 namespace nqs {
 
-// column means of O per half into mean [2][P]; then, if inplace, O -= mean per half and, if Ob,
-// the centred bf16 copy (blocked layout when gram_bf16_blocked).  part: colsum_part_elems doubles.
+// column means of O per half over the half's first mean_rows rows into mean [2][P]; then, if
+// inplace, O -= mean per half and, if Ob, the centred bf16 copy (blocked layout when
+// gram_bf16_blocked).  part: colsum_part_elems doubles.
 int64_t colsum_part_elems(int64_t ldo);
 template <typename T>
-int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, int inplace, double* part,
-                 hipStream_t st);
+int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, int inplace, int64_t mean_rows,
+                 double* part, hipStream_t st);
+// Walkers whose mean shifts O before the bf16 down-cast.  The deflation projects the Gram of the
+// shifted rows onto the exactly centred one (Pi T' Pi = T for any shift common to a half), and the
+// solve projects its residual the same way, so the shift only has to keep the down-cast's rounding
+// relative to the walker spread: a 512-walker mean differs from the full mean by about 4% of it.
+constexpr int64_t kBf16MeanRows = 512;
+// x <- Pi x: each half of x (2W) minus its mean
+int proj_halves(double* x, int64_t W, hipStream_t st);
 // out = (O - 1 mean^T)^T y  and  v = (O - 1 mean^T) u  (mean null: O is already centred)
 template <typename T>
 int gemv_cols_c(const T* O, int64_t ldo, int64_t W, int64_t P, const double* y, const double* mean, double* part,

@@ -20,17 +20,18 @@ namespace nqs {
 // writes its partial column sums, which a second kernel adds up in a fixed order.
 constexpr int kSplit = 8;  // row chunks per half
 
-// part[s][c] = sum over rows r of chunk s of x(r) O[r][c]  (x = 1 when x is null)
+// part[s][c] = sum over rows r of chunk s of x(r) O[r][c]  (x = 1 when x is null), over the first
+// Wr <= W rows of each half
 template <typename T>
-__global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ O, int64_t ldo, int64_t W,
+__global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ O, int64_t ldo, int64_t W, int64_t Wr,
                                                      const double* __restrict__ x, double* __restrict__ part) {
   typedef T v4 __attribute__((ext_vector_type(4)));
   const int64_t c0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
   if (c0 >= ldo) return;
   const int s = blockIdx.y, half = s / kSplit, chunk = s % kSplit;
-  const int64_t len = (W + kSplit - 1) / kSplit;
+  const int64_t len = (Wr + kSplit - 1) / kSplit;
   const int64_t r0 = half * W + chunk * len;
-  const int64_t r1 = min(half * W + W, r0 + len);
+  const int64_t r1 = min(half * W + Wr, r0 + len);
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   const T* p = O + r0 * ldo + c0;
   int64_t r = r0;
@@ -96,6 +97,27 @@ __global__ __launch_bounds__(256) void k_half_sums(const double* __restrict__ y,
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) ysum[h] = red[0] + red[1] + red[2] + red[3];
+}
+
+// x[h W + i] -= (1/W) sum_i' x[h W + i'] for both halves (x <- Pi x, Pi the projector off the
+// centring null space); one workgroup per half, the sum in a fixed order
+__global__ __launch_bounds__(256) void k_proj_halves(double* __restrict__ x, int64_t W) {
+  __shared__ double red[4];
+  const int h = blockIdx.x;
+  double* xh = x + h * W;
+  double s = 0;
+  for (int64_t r = threadIdx.x; r < W; r += 256) s += xh[r];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double m = ((red[0] + red[1]) + (red[2] + red[3])) / (double)W;
+  for (int64_t r = threadIdx.x; r < W; r += 256) xh[r] -= m;
+}
+
+int proj_halves(double* x, int64_t W, hipStream_t st) {
+  hipLaunchKernelGGL(k_proj_halves, dim3(2), dim3(256), 0, st, x, W);
+  NQS_LAUNCH_CHECK();
+  return NQS_OK;
 }
 
 // mdot[h] = mean[h] . u over P columns; two workgroups of 1024 threads
@@ -173,14 +195,15 @@ bool gram_bf16_blocked(int64_t n, int64_t K) { return n % 256 == 0 && K % 64 == 
 int64_t colsum_part_elems(int64_t ldo) { return 2LL * kSplit * ldo; }
 
 template <typename T>
-int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, int inplace, double* part,
-                 hipStream_t st) {
+int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* Ob, int inplace, int64_t mean_rows,
+                 double* part, hipStream_t st) {
   NQS_REQUIRE(ldo % 64 == 0, NQS_ERR_ARG, "ldo must be a multiple of 64");
+  NQS_REQUIRE(mean_rows >= 1 && mean_rows <= W, NQS_ERR_ARG, "bad centring row count");
   hipLaunchKernelGGL(k_colsum_part<T>, dim3((unsigned)cdiv(ldo, 1024), 2 * kSplit), dim3(256), 0, st, O, ldo, W,
-                     nullptr, part);
+                     mean_rows, nullptr, part);
   NQS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(P, 256)), dim3(256), 0, st, part, ldo, P, 0,
-                     1.0 / (double)W, nullptr, nullptr, mean, P);
+                     1.0 / (double)mean_rows, nullptr, nullptr, mean, P);
   NQS_LAUNCH_CHECK();
   if (!Ob && !inplace) return NQS_OK;
   const int blocked = gram_bf16_blocked(2 * W, ldo) ? 1 : 0;
@@ -195,8 +218,8 @@ int minsr_center(T* O, int64_t ldo, int64_t W, int64_t P, double* mean, __bf16* 
 template <typename T>
 int gemv_cols_c(const T* O, int64_t ldo, int64_t W, int64_t P, const double* y, const double* mean, double* part,
                 double* ysum, double* out, int64_t nout, hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum_part<T>, dim3((unsigned)cdiv(ldo, 1024), 2 * kSplit), dim3(256), 0, st, O, ldo, W, y,
-                     part);
+  hipLaunchKernelGGL(k_colsum_part<T>, dim3((unsigned)cdiv(ldo, 1024), 2 * kSplit), dim3(256), 0, st, O, ldo, W, W,
+                     y, part);
   NQS_LAUNCH_CHECK();
   if (mean) {
     hipLaunchKernelGGL(k_half_sums, dim3(2), dim3(256), 0, st, y, W, ysum);
@@ -1251,8 +1274,10 @@ int make_etil(const double* e, int64_t W, double* et, hipStream_t st) {
   return NQS_OK;
 }
 
-template int minsr_center<double>(double*, int64_t, int64_t, int64_t, double*, __bf16*, int, double*, hipStream_t);
-template int minsr_center<float>(float*, int64_t, int64_t, int64_t, double*, __bf16*, int, double*, hipStream_t);
+template int minsr_center<double>(double*, int64_t, int64_t, int64_t, double*, __bf16*, int, int64_t, double*,
+                                  hipStream_t);
+template int minsr_center<float>(float*, int64_t, int64_t, int64_t, double*, __bf16*, int, int64_t, double*,
+                                 hipStream_t);
 template int gemv_cols_c<double>(const double*, int64_t, int64_t, int64_t, const double*, const double*, double*,
                                  double*, double*, int64_t, hipStream_t);
 template int gemv_cols_c<float>(const float*, int64_t, int64_t, int64_t, const double*, const double*, double*,
