@@ -244,6 +244,7 @@ struct MinsrWs {
   double* panel;
   double *et, *y, *r, *u, *v, *ts, *sc;
   double* part;  // partial column sums (colsum_part_elems)
+  int32_t* tsync;  // the chained triangular solve's counters and flags (chol_sync_elems)
   double* red;   // ysum[2], mdot[2]
   int64_t bytes;
 };
@@ -271,6 +272,7 @@ static MinsrWs minsr_ws(const nqs_minsr_desc* d, char* base) {
   w.sc = (double*)grab(8 * 8);
   w.part = (double*)grab(colsum_part_elems(d->ldo) * 8);
   w.red = (double*)grab(4 * 8);
+  w.tsync = (int32_t*)grab(chol_sync_elems(n) * 4);
   w.bytes = p - base;
   return w;
 }
@@ -361,7 +363,7 @@ static int minsr_solve(const nqs_minsr_desc* d, T* O, const double* eps, double*
   NQS_TRY(minsr_mark(6, st));
   NQS_TRY(make_etil(eps, W, w.et, st));
   NQS_CHECK_HIP(hipMemcpyAsync(w.y, w.et, n * 8, hipMemcpyDeviceToDevice, st));
-  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.y, st));
+  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.y, w.tsync, st));
   for (int it = 0; it < d->refine; ++it) {
     // r = etil - Otil (Otil^T y) - lam y ;  y += (L L^T)^-1 r
     NQS_TRY(gemv_cols_c<T>(O, ldo, W, P, w.y, mc, w.part, w.red, w.u, ldo, st));
@@ -370,7 +372,7 @@ static int minsr_solve(const nqs_minsr_desc* d, T* O, const double* eps, double*
     NQS_TRY(axpby(n, 1.0, w.r, -d->lam, w.y, w.r, st));
     // with the shifted O (bf16 Gram), v carries the shift's constant per half: r <- Pi r
     if (mc) NQS_TRY(proj_halves(w.r, W, st));
-    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.r, st));
+    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.r, w.tsync, st));
     NQS_TRY(axpby(n, 1.0, w.y, 1.0, w.r, w.y, st));
   }
   NQS_TRY(minsr_mark(4, st));
@@ -399,6 +401,7 @@ struct MinsrSysWs {
 struct MinsrBatchWs {
   double *A64, *Linv, *panel, *Y, *Rr;
   float* T32;
+  int32_t* tsync;
   std::vector<MinsrSysWs> sys;
   int64_t bytes;
 };
@@ -417,6 +420,7 @@ static MinsrBatchWs minsr_batch_ws(const nqs_minsr_desc* d, int nsys, char* base
   w.Y = (double*)grab(nsys * n * 8);
   w.Rr = (double*)grab(nsys * n * 8);
   w.T32 = d[0].gram != NQS_GRAM_FP64 ? (float*)grab(nsys * n * n * 4) : nullptr;
+  w.tsync = (int32_t*)grab(nsys * chol_sync_elems(n) * 4);
   for (int s = 0; s < nsys; ++s) {
     MinsrSysWs q;
     q.mean = (double*)grab(2 * d[s].n_params * 8);
@@ -493,7 +497,7 @@ static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov
   NQS_TRY(cholesky(w.A64, n, n, w.Linv, w.panel, info, st, nsys, n * n, n * 64, cholesky_panel_elems(n)));
   for (int s = 0; s < nsys; ++s)
     NQS_CHECK_HIP(hipMemcpyAsync(w.Y + s * n, w.sys[s].et, n * 8, hipMemcpyDeviceToDevice, st));
-  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Y, st, nsys, n * n, n * 64, n));
+  NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Y, w.tsync, st, nsys, n * n, n * 64, n));
   for (int it = 0; it < d[0].refine; ++it) {
     for (int s = 0; s < nsys; ++s) {
       const MinsrSysWs& q = w.sys[s];
@@ -507,7 +511,7 @@ static int minsr_batched_impl(const nqs_minsr_desc* d, int nsys, void* const* Ov
       NQS_TRY(axpby(n, 1.0, r, -d[s].lam, y, r, st));
       if (mc) NQS_TRY(proj_halves(r, W, st));
     }
-    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Rr, st, nsys, n * n, n * 64, n));
+    NQS_TRY(chol_solve(w.A64, n, n, w.Linv, w.Rr, w.tsync, st, nsys, n * n, n * 64, n));
     NQS_TRY(axpby(nsys * n, 1.0, w.Y, 1.0, w.Rr, w.Y, st));
   }
   for (int s = 0; s < nsys; ++s) {

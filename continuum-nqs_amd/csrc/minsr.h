@@ -52,8 +52,11 @@ int deflate_centring(double* A, int64_t lda, int64_t W, double lam, double* ts, 
 int64_t cholesky_panel_elems(int64_t n);
 int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int* info, hipStream_t st, int nb = 1,
              int64_t sA = 0, int64_t sL = 0, int64_t sP = 0);
-int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb = 1,
-               int64_t sA = 0, int64_t sL = 0, int64_t sx = 0);
+// L L^T x = b in place (x holds b): one launch per direction (k_trsv_chain).  sync: scratch of
+// chol_sync_elems(n) int32 per system (at that stride), 8-byte aligned.
+int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, int32_t* sync, hipStream_t st,
+               int nb = 1, int64_t sA = 0, int64_t sL = 0, int64_t sx = 0);
+int64_t chol_sync_elems(int64_t n);
 int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 int axpby(int64_t n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
 int clip_stats(const double* e, int64_t W, double clip_k, double* e_clip, double* stats, hipStream_t st);

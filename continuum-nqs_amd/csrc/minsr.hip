@@ -980,197 +980,156 @@ int cholesky(double* A, int64_t lda, int64_t n, double* Linv, double* panel, int
 }
 
 // ------------------------------------------------------------------ triangular solves with the factor
-// L y = b then L^T x = y, in super-blocks of TS_SB = 4 x 64 rows.  Per super-block one workgroup
-// solves the diagonal part (the four 64 x 64 blocks through their inverses Linv, with the
-// in-super-block updates between them), then a grid of workgroups applies the super-block to the
-// rest of the vector: forward, b[r] -= L[r, J0:J0+256] y[J0:J0+256] for the rows below; backward,
-// x[c] -= sum_r L[J0 + r][c] x[J0 + r] for the columns to the left.
-#define TS_SB 256
+// L y = b (forward) or L^T x = y (backward) in one launch of n / 64 workgroups.  Workgroup t (the
+// order in which workgroups start: a ticket from an atomic counter) solves the 64-row block
+// b = t (forward) or nb - 1 - t (backward): it subtracts the contributions of the blocks solved
+// before it, each as soon as that block's values appear, applies the diagonal block's inverse and
+// writes its block.  The output vector starts filled with kTsPending, a signalling-NaN bit pattern
+// that no arithmetic produces (a NaN from arithmetic is quiet), and every value is written once
+// with an agent-scope atomic store, so a reader polls the values themselves (agent-scope atomic
+// loads) until none is pending: no flags, no fences.  A workgroup waits only for blocks of lower
+// tickets, which started before it and in turn wait only for lower tickets, so the chain advances
+// whatever the number of resident workgroups.  Thread (lane, wave w) holds 16 entries of every
+// block it applies (row lane of the block, columns 16 w .. 16 w + 15; transposed for the backward
+// solve) and accumulates its partial dot product over the blocks in publishing order; the four
+// waves' partials are added in a fixed order, so the result does not depend on timing.  The next
+// off-diagonal block is loaded while the current one's values are awaited.  A wait longer than
+// kTsSpinLimit polls gives up and its values count as NaN, so a broken chain ends the launch with a
+// non-finite solution instead of a hang.
+constexpr int kTsSpinLimit = 1 << 22;
+constexpr uint64_t kTsPending = 0x7FF40000000DEDEDull;
 
-// The super-block's diagonal part touches at most ten 64 x 64 blocks: step s (s = 0..3, in solve
-// order) applies the diagonal inverse (offset g = 0) and then the off-diagonal blocks g = 1..3 - s
-// to the parts not yet solved.  16 waves = 4 groups of 4; block (s, g) belongs to group ts_grp(s, g) and
-// register slot ts_slot(s) (each group holds at most three blocks, every slot of a step is the same
-// compile-time index), and is loaded at kernel start, so the steps themselves only read LDS.
-// Thread (i = lane, p = wave & 3) of a group holds elements (i, 16p .. 16p + 15) of the block as
-// applied (row i of M for forward, column i of M for the transposed backward) and forms the partial
-// dot product of output i; the four partials are summed in LDS in a fixed order.
-__host__ __device__ constexpr int ts_grp(int s, int g) {
-  return s == 0 ? g : (s == 1 ? 1 + g : (s == 2 ? (g == 0 ? 2 : 0) : 3));
-}
-__host__ __device__ constexpr int ts_slot(int s) { return s < 2 ? s : 2; }
-// inverse of the two maps: (group, slot) -> step s (-1 when the slot is unused) and offset g
-__device__ __forceinline__ void ts_owner(int grp, int slot, int& s, int& g) {
-  s = slot == 0 ? 0 : (slot == 1 ? (grp == 0 ? -1 : 1) : (grp == 1 ? -1 : (grp == 3 ? 3 : 2)));
-  g = slot == 0 ? grp : (slot == 1 ? grp - 1 : (grp == 0 ? 1 : 0));
+// per system: n doubles (the forward solve's output) and two ticket counters
+int64_t chol_sync_elems(int64_t n) { return 2 * n + 4; }
+
+// out[0, n) = kTsPending and, if cnt, cnt[0] = cnt[1] = 0, for each of nb systems (strides in elements)
+__global__ void k_ts_reset(uint64_t* __restrict__ out, int64_t n, int64_t so, int32_t* __restrict__ cnt,
+                           int64_t sc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[blockIdx.z * so + i] = kTsPending;
+  if (cnt && i < 2) cnt[blockIdx.z * sc + i] = 0;
 }
 
-// address of element (i, k) of block (dq, tq) as applied: diagonal (tq == dq) through Linv,
-// off-diagonal through L (forward: L(tq, dq); backward: L(dq, tq) transposed)
-__device__ __forceinline__ const double* ts_elem_base(const double* A, int64_t lda, const double* Linv, int64_t J0,
-                                                      int dq, int tq, int trans, int i, int p, int64_t& step) {
-  if (dq == tq) {
-    const double* Li = Linv + ((J0 + 64 * dq) / CH_NB) * CH_NB * CH_NB;
-    step = trans ? CH_NB : 1;
-    return trans ? Li + (16 * p) * CH_NB + i : Li + i * CH_NB + 16 * p;
+// this lane's value (entry `lane & 15` of a 16-value slice) once all 64 lanes see published values
+__device__ __forceinline__ double ts_poll(const double* p, bool& ok) {
+  uint64_t* q = reinterpret_cast<uint64_t*>(const_cast<double*>(p));
+  uint64_t v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (ok) {
+    int spins = 0;
+    while (!__all(v != kTsPending)) {
+      if (++spins > kTsSpinLimit) {
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
-  if (!trans) {
-    step = 1;
-    return A + (J0 + 64 * tq + i) * lda + J0 + 64 * dq + 16 * p;
-  }
-  step = lda;
-  return A + (J0 + 64 * dq + 16 * p) * lda + J0 + 64 * tq + i;
+  return v == kTsPending ? __builtin_nan("") : __builtin_bit_cast(double, v);
 }
 
-// sb = rows of this super-block (a multiple of 64, at most TS_SB; the last one may be short)
-__global__ __launch_bounds__(1024) void k_trsv_super(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                     const double* __restrict__ Linv, double* __restrict__ b,
-                                                     int trans, int64_t sA, int64_t sL, int64_t sx) {
+template <int TRANS>
+__device__ __forceinline__ void ts_block(const double* __restrict__ A, int64_t lda, int b, int j, int wave, int lane,
+                                         double (&m)[16]) {
+  if (TRANS == 0) {
+    const double* p = A + (int64_t)(64 * b + lane) * lda + 64 * j + 16 * wave;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = p[k];
+  } else {
+    const double* p = A + (int64_t)(64 * j + 16 * wave) * lda + 64 * b + lane;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = p[(int64_t)k * lda];
+  }
+}
+
+// rhs -> out (both n per system, at strides sr, so); cnt: this direction's ticket counter
+template <int TRANS>
+__global__ __launch_bounds__(256) void k_trsv_chain(const double* __restrict__ A, int64_t lda, int nb,
+                                                    const double* __restrict__ Linv, const double* rhs, int64_t sr,
+                                                    double* out, int64_t so, int32_t* cnt, int64_t sc, int64_t sA,
+                                                    int64_t sL) {
   A += blockIdx.z * sA;
   Linv += blockIdx.z * sL;
-  b += blockIdx.z * sx;
-  __shared__ double x[TS_SB];
-  __shared__ double red[4][4][CH_NB];  // [group][p][i]
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, grp = wave >> 2, p = wave & 3;
-  const int nblk = sb / CH_NB;
-  double m[3][16];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    int s, g;
-    ts_owner(grp, k, s, g);
-    const bool have = s >= 0 && s + g < nblk;
-    const int dq = trans ? nblk - 1 - s : s, tq = trans ? dq - g : dq + g;
-    // an unused slot reads the first Linv block instead (never applied), so the loads need no branch
-    int64_t step = 1;
-    const double* e = have ? ts_elem_base(A, lda, Linv, J0, dq, tq, trans, lane, p, step) : Linv + lane;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) m[k][j] = e[j * step];
-  }
-  if (t < sb) x[t] = b[J0 + t];
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    if (s >= nblk) break;
-    const int dq = trans ? nblk - 1 - s : s;
-    // phase A: x_dq <- Linv_dq (or its transpose) x_dq
-    if (grp == ts_grp(s, 0)) {
-      const double* v = x + 64 * dq + 16 * p;
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) acc += m[ts_slot(s)][j] * v[j];
-      red[0][p][lane] = acc;
-    }
-    __syncthreads();
-    if (t < CH_NB) x[64 * dq + t] = (red[0][0][t] + red[0][1][t]) + (red[0][2][t] + red[0][3][t]);
-    __syncthreads();
-    // phase B: x_tq -= M(dq, tq) x_dq for the blocks not yet solved
-    if (s + 1 < nblk) {
-      int gb = 0;
-#pragma unroll
-      for (int g = 1; g + s < 4; ++g)
-        if (grp == ts_grp(s, g)) gb = g;
-      if (gb > 0 && s + gb < nblk) {
-        const double* v = x + 64 * dq + 16 * p;
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) acc += m[ts_slot(s)][j] * v[j];
-        red[gb][p][lane] = acc;
-      }
-      __syncthreads();
-      if (t < 3 * CH_NB) {
-        const int g = 1 + t / CH_NB, i = t % CH_NB;
-        if (s + g < nblk) {
-          const int tq = trans ? dq - g : dq + g;
-          x[64 * tq + i] -= (red[g][0][i] + red[g][1][i]) + (red[g][2][i] + red[g][3][i]);
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if (t < sb) b[J0 + t] = x[t];
-}
-
-// forward update: rows r in [r0, n): b[r] -= L[r, J0:J0+sb] . b[J0:J0+sb]; 8 rows per wave, the
-// row's loads all issued before the reductions
-__global__ __launch_bounds__(256) void k_trsv_fwd_rows(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                       int64_t r0, int64_t n, double* __restrict__ b, int64_t sA,
-                                                       int64_t sx) {
-  A += blockIdx.z * sA;
-  b += blockIdx.z * sx;
-  __shared__ double z[TS_SB];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  z[t] = t < sb ? b[J0 + t] : 0.0;
-  __syncthreads();
-  const int64_t rbase = r0 + ((int64_t)blockIdx.x * 4 + wave) * 8;
-  double acc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int64_t r = rbase + q < n ? rbase + q : n - 1;
-    const double* row = A + r * lda + J0;
-    double s = 0;
-#pragma unroll
-    for (int c4 = 0; c4 < TS_SB / 64; ++c4) {
-      const int c = lane + 64 * c4;
-      s += row[c < sb ? c : sb - 1] * z[c];  // z is zero past sb: clamped loads, no branches
-    }
-    acc[q] = s;
-  }
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double s = wave_sum(acc[q]);
-    if (lane == 0 && rbase + q < n) b[rbase + q] -= s;
-  }
-}
-
-// backward update: columns c < J0: x[c] -= sum_{r < sb} L[J0 + r][c] x[J0 + r]; 64 columns per
-// workgroup, its four waves taking a quarter of the rows each, summed in a fixed order
-__global__ __launch_bounds__(256) void k_trsv_bwd_cols(const double* __restrict__ A, int64_t lda, int64_t J0, int sb,
-                                                       double* __restrict__ x, int64_t sA, int64_t sx) {
-  A += blockIdx.z * sA;
-  x += blockIdx.z * sx;
-  __shared__ double z[TS_SB];
+  rhs += blockIdx.z * sr;
+  out += blockIdx.z * so;
+  cnt += blockIdx.z * sc;
+  __shared__ int tk;
+  __shared__ int bad;
   __shared__ double part[4][64];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  z[t] = t < sb ? x[J0 + t] : 0.0;
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
-  const int64_t cc = c < J0 ? c : J0 - 1;
-  const int rq = TS_SB / 4;
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int r = 0; r < rq; r += 8) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int rr = wave * rq + r + u;
-      s[u] += A[(J0 + (rr < sb ? rr : sb - 1)) * lda + cc] * z[rr];  // z is zero past sb
-    }
+  __shared__ double accs[64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    tk = atomicAdd(cnt, 1);
+    bad = 0;
   }
-  part[wave][lane] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
   __syncthreads();
-  if (wave == 0 && c < J0) x[c] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  const int t = tk;
+  const int b = TRANS ? nb - 1 - t : t;
+  // this thread's 16 entries of the diagonal block's inverse, as applied
+  const double* Li = Linv + (int64_t)b * CH_NB * CH_NB;
+  double li[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) li[k] = TRANS ? Li[(16 * wave + k) * CH_NB + lane] : Li[lane * CH_NB + 16 * wave + k];
+  double acc = 0.0;
+  bool ok = true;
+  // step s: await block j(s)'s 16 values of this wave's slice (one per lane of a 16-lane group,
+  // broadcast by readlane), issue the loads of block j(s + 1) into the other buffer, then
+  // accumulate block j(s) from the buffer loaded one step earlier
+  auto step = [&](int s, const double (&cur)[16], double (&nxt)[16]) {
+    const int j = TRANS ? nb - 1 - s : s;
+    const double xl = ts_poll(out + 64 * j + 16 * wave + (lane & 15), ok);
+    // unconditional (the last step reloads block j, in bounds), so that these loads stay in
+    // flight on every path
+    ts_block<TRANS>(A, lda, b, s + 1 < t ? (TRANS ? j - 1 : j + 1) : j, wave, lane, nxt);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += cur[k] * readlane_f64(xl, k);
+  };
+  double ma[16], mb[16];
+  if (t > 0) ts_block<TRANS>(A, lda, b, TRANS ? nb - 1 : 0, wave, lane, ma);
+  for (int s = 0; s < t; s += 2) {
+    step(s, ma, mb);
+    if (s + 1 < t) step(s + 1, mb, ma);
+  }
+  if (!ok && lane == 0) bad = 1;  // benign race: every writer stores 1
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0) accs[lane] = rhs[64 * b + lane] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+  __syncthreads();
+  double s2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s2 += li[k] * accs[16 * wave + k];
+  part[wave][lane] = s2;
+  __syncthreads();
+  if (wave == 0) {
+    double v = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    if (bad || v != v) v = __builtin_nan("");  // a quiet NaN, never the pending pattern
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(out + 64 * b + lane), __builtin_bit_cast(uint64_t, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
-int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, hipStream_t st, int nb,
-               int64_t sA, int64_t sL, int64_t sx) {
+int chol_solve(const double* L, int64_t lda, int64_t n, const double* Linv, double* x, int32_t* sync, hipStream_t st,
+               int nb, int64_t sA, int64_t sL, int64_t sx) {
   NQS_REQUIRE(n % CH_NB == 0, NQS_ERR_UNSUPPORTED, "triangular solve size must be a multiple of 64");
   NQS_REQUIRE(nb >= 1 && nb <= 65535, NQS_ERR_ARG, "bad batch count");
+  NQS_REQUIRE(sync && ((uintptr_t)sync & 7) == 0, NQS_ERR_ARG, "triangular solve sync buffer must be 8-byte aligned");
+  // per system: y (n doubles) then the two ticket counters; x is the forward's right-hand side,
+  // then (refilled with kTsPending once the forward is done) the backward's output
   const unsigned z = (unsigned)nb;
-  for (int64_t J0 = 0; J0 < n; J0 += TS_SB) {
-    const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(1024), 0, st, L, lda, J0, sb, Linv, x, 0, sA, sL, sx);
-    const int64_t r0 = J0 + sb;
-    if (r0 < n)
-      hipLaunchKernelGGL(k_trsv_fwd_rows, dim3((unsigned)cdiv(n - r0, 32), 1, z), dim3(256), 0, st, L, lda, J0, sb, r0,
-                         n, x, sA, sx);
-  }
-  const int64_t last = ((n - 1) / TS_SB) * TS_SB;
-  for (int64_t J0 = last; J0 >= 0; J0 -= TS_SB) {
-    const int sb = (int)(n - J0 < TS_SB ? n - J0 : TS_SB);
-    hipLaunchKernelGGL(k_trsv_super, dim3(1, 1, z), dim3(1024), 0, st, L, lda, J0, sb, Linv, x, 1, sA, sL, sx);
-    if (J0 > 0)
-      hipLaunchKernelGGL(k_trsv_bwd_cols, dim3((unsigned)cdiv(J0, 64), 1, z), dim3(256), 0, st, L, lda, J0, sb, x, sA,
-                         sx);
-  }
+  const int nbk = (int)(n / CH_NB);
+  const int64_t ss = chol_sync_elems(n);  // int32 elements per system
+  double* y = reinterpret_cast<double*>(sync);
+  int32_t* cnt = sync + 2 * n;
+  const dim3 fg((unsigned)cdiv(n, 256), 1, z);
+  hipLaunchKernelGGL(k_ts_reset, fg, dim3(256), 0, st, reinterpret_cast<uint64_t*>(y), n, ss / 2, cnt, ss);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_trsv_chain<0>, dim3((unsigned)nbk, 1, z), dim3(256), 0, st, L, lda, nbk, Linv, x, sx, y, ss / 2,
+                     cnt, ss, sA, sL);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_ts_reset, fg, dim3(256), 0, st, reinterpret_cast<uint64_t*>(x), n, sx, nullptr, 0);
+  NQS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_trsv_chain<1>, dim3((unsigned)nbk, 1, z), dim3(256), 0, st, L, lda, nbk, Linv, y, ss / 2, x, sx,
+                     cnt + 1, ss, sA, sL);
   NQS_LAUNCH_CHECK();
   return NQS_OK;
 }

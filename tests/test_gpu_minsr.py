@@ -99,10 +99,10 @@ def test_minsr_matches_oracle_and_paramsr(lam):
 
 
 @pytest.mark.parametrize("W", [160, 224, 416])
-def test_minsr_short_last_super_block(W):
-    """2W = 320, 448, 832: the triangular solves run full 256-row super-blocks and then a last one of
-    one, three and one 64-row blocks; the Cholesky's last outer panel is short too.  The float64
-    direction against the oracle's LU solve."""
+def test_minsr_short_last_outer_panel(W):
+    """2W = 320, 448, 832: the Cholesky's last 512-column outer panel is short (five, seven and
+    five 64-column blocks past the last full panel), and the chained triangular solves run 5, 7
+    and 13 blocks.  The float64 direction against the oracle's LU solve."""
     O, eps = _minsr_case(W=W, seed=6)
     d = nms.minsr_direction(O, eps, 1e-4, gram="fp64").cpu()
     assert int(nms.minsr_direction.last_info.item()) == 0
