@@ -1201,13 +1201,28 @@ __global__ __launch_bounds__(1024) void k_clip_stats(const double* __restrict__ 
     e_clip[2 * i] = x < lo ? lo : (x > hi ? hi : x);
     e_clip[2 * i + 1] = e[2 * i + 1];
   }
+  // mean and variance: per-thread strided sums, then the waves' sums in wave order (a fixed
+  // order, so the statistics do not depend on timing)
+  __shared__ double red[16];
+  double s = 0;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) s += e[2 * i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double tot = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) tot += red[q];
+  const double mean = tot / W;
+  __syncthreads();
+  double s2 = 0;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) s2 += (e[2 * i] - mean) * (e[2 * i] - mean);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0, s2 = 0;
-    for (int i = 0; i < W; ++i) s += e[2 * i];
-    const double mean = s / W;
-    for (int i = 0; i < W; ++i) s2 += (e[2 * i] - mean) * (e[2 * i] - mean);
+    double var = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) var += red[q];
     stats[0] = mean;
-    stats[1] = sqrt(s2 / W) / sqrt((double)W);
+    stats[1] = sqrt(var / W) / sqrt((double)W);
     stats[2] = med;
     stats[3] = mad;
   }
